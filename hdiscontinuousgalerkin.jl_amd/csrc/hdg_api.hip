@@ -1,0 +1,770 @@
+// C ABI of libhdg (include/hdg.h): context, device memory, orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hdg.h"
+#include "hdg_internal.h"
+#include "hdg_tables.h"
+
+using namespace hdg;
+
+struct hdg_ctx {
+  int device = 0;
+  size_t budget = 0, used = 0;
+  std::unordered_map<void*, size_t> allocs;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  // mesh
+  bool has_mesh = false;
+  DevMesh mesh;
+  std::vector<int> h_cf;  // face ids per cell, 0-based
+  int64_t nblock_cols = 0;  // stored nt x nt blocks of K
+  DevPattern pat;
+
+  // discretisation
+  bool has_setup = false;
+  hdg_options opt{};
+  int k = 0, nu = 0, nq = 0, m = 0, nt = 0, ntr = 0, nqp = 0, nfq = 0;
+  RefTables tab;
+  double* d_tab = nullptr;
+  TabLayoutDev lay{};
+  SrcParams src;
+  double* d_fqp = nullptr;
+
+  // Dirichlet
+  unsigned char* d_dmask = nullptr;
+  double* d_uD = nullptr;
+  bool jacobi_dirty = true;
+
+  // per-cell data
+  double *d_Ate = nullptr, *d_bte = nullptr, *d_Ke = nullptr, *d_be = nullptr, *d_qu = nullptr;
+  int* d_err = nullptr;
+
+  // trace system and solver
+  TraceSystem T;
+  CgVectors cg{};
+  double* d_uhat = nullptr;
+  CgState* h_st = nullptr;  // pinned, 2 slots
+  double* d_scal = nullptr; // [0] errornorm, [1] true residual^2
+  double* d_part2 = nullptr;
+  unsigned* d_ticket2 = nullptr;
+
+  bool assembled = false, solved = false, recovered = false;
+};
+
+namespace {
+
+int fail(hdg_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail(ctx, HDG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+  } while (0)
+
+#define LAUNCHCHK(ctx, call)                                                                 \
+  do {                                                                                       \
+    int e_ = (call);                                                                         \
+    if (e_ != 0)                                                                             \
+      return fail(ctx, HDG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString((hipError_t)e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(hdg_ctx* c, T** p, size_t count) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (c->budget && c->used + bytes > c->budget)
+    return fail(c, HDG_ERR_HBM_BUDGET, "device allocation of %zu bytes exceeds the budget (%zu of %zu used)",
+                bytes, c->used, c->budget);
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(c, HDG_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  c->allocs[q] = bytes;
+  c->used += bytes;
+  *p = static_cast<T*>(q);
+  return HDG_OK;
+}
+
+template <class T>
+void dfree(hdg_ctx* c, T*& p) {
+  if (!p) return;
+  auto it = c->allocs.find((void*)p);
+  if (it != c->allocs.end()) {
+    c->used -= it->second;
+    c->allocs.erase(it);
+  }
+  (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+#define TRY(x)             \
+  do {                     \
+    int r_ = (x);          \
+    if (r_ != HDG_OK) return r_; \
+  } while (0)
+
+void free_problem(hdg_ctx* c) {
+  dfree(c, c->d_tab);
+  dfree(c, c->d_fqp);
+  dfree(c, c->d_dmask);
+  dfree(c, c->d_uD);
+  dfree(c, c->d_Ate);
+  dfree(c, c->d_bte);
+  dfree(c, c->d_Ke);
+  dfree(c, c->d_be);
+  dfree(c, c->d_qu);
+  dfree(c, c->T.vals);
+  dfree(c, c->T.rhs);
+  dfree(c, c->T.Dinv);
+  dfree(c, c->cg.x);
+  dfree(c, c->cg.r);
+  dfree(c, c->cg.z);
+  dfree(c, c->cg.p0);
+  dfree(c, c->cg.p1);
+  dfree(c, c->cg.q);
+  dfree(c, c->cg.part);
+  dfree(c, c->d_uhat);
+  dfree(c, c->d_part2);
+  c->has_setup = c->assembled = c->solved = c->recovered = false;
+}
+
+void free_mesh(hdg_ctx* c) {
+  free_problem(c);
+  dfree(c, const_cast<double*&>(c->mesh.xy));
+  dfree(c, const_cast<int*&>(c->mesh.cn));
+  dfree(c, const_cast<int*&>(c->mesh.cf));
+  dfree(c, const_cast<int*&>(c->mesh.fcell));
+  dfree(c, c->pat.bcol);
+  dfree(c, c->pat.src);
+  dfree(c, c->pat.nslot);
+  dfree(c, c->pat.self);
+  dfree(c, c->pat.tslot);
+  c->has_mesh = false;
+  c->mesh = DevMesh{};
+}
+
+int select_device(hdg_ctx* c) {
+  HIPCHK(c, hipSetDevice(c->device));
+  return HDG_OK;
+}
+
+int check_err_flag(hdg_ctx* c) {
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flag & 1) return fail(c, HDG_ERR_NEG_JACOBIAN, "det(J) is not positive for at least one cell");
+  if (flag & 2)
+    return fail(c, HDG_ERR_LOCAL_SINGULAR,
+                "local (q,u) matrix is singular for at least one cell (k=%d, quad_degree=%d); "
+                "raise quad_degree", c->k, c->opt.quad_degree);
+  return HDG_OK;
+}
+
+int assemble_impl(hdg_ctx* c) {
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  CondenseArgs a;
+  a.mesh = c->mesh;
+  a.tables = c->d_tab;
+  a.lay = c->lay;
+  a.src = c->src;
+  a.src.f_qp = c->d_fqp;
+  a.tau = c->opt.tau;
+  a.Ate = c->d_Ate;
+  a.bte = c->d_bte;
+  a.Ke = c->d_Ke;
+  a.be = c->d_be;
+  a.err = c->d_err;
+  LAUNCHCHK(c, launch_condense(c->k, a, false, c->stream));
+  LAUNCHCHK(c, launch_fill(c->pat, c->d_Ate, c->d_bte, c->T, c->stream));
+  LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
+  c->jacobi_dirty = false;
+  c->assembled = true;
+  c->solved = c->recovered = false;
+  return HDG_OK;
+}
+
+constexpr int kChunk = 64;  // PCG iterations enqueued between convergence checks
+
+int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  if (c->jacobi_dirty) {
+    LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
+    c->jacobi_dirty = false;
+  }
+  const double rtol = c->opt.rtol > 0 ? c->opt.rtol : 1e-12;
+  const int maxit = c->opt.maxit > 0 ? c->opt.maxit : 200000;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, rtol, maxit, c->stream));
+  // keep one chunk in flight while the previous chunk's state is read back
+  int slot = 0;
+  for (int i = 0; i < kChunk; ++i) LAUNCHCHK(c, launch_cg_iteration(c->pat, c->T, c->cg, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+  hipEvent_t evs[2];
+  HIPCHK(c, hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+  HIPCHK(c, hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+  HIPCHK(c, hipEventRecord(evs[slot], c->stream));
+  for (;;) {
+    const int nxt = slot ^ 1;
+    for (int i = 0; i < kChunk; ++i) LAUNCHCHK(c, launch_cg_iteration(c->pat, c->T, c->cg, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->h_st[nxt], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
+    HIPCHK(c, hipEventSynchronize(evs[slot]));
+    if (c->h_st[slot].done) break;
+    slot = nxt;
+  }
+  HIPCHK(c, hipEventSynchronize(evs[slot ^ 1]));
+  (void)hipEventDestroy(evs[0]);
+  (void)hipEventDestroy(evs[1]);
+  LAUNCHCHK(c, launch_cg_finish(c->T, c->cg, c->d_uhat, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  LAUNCHCHK(c, launch_true_residual(c->pat, c->T, c->d_uhat, c->d_scal + 1, c->cg, c->stream));
+  CgState st;
+  double tr2 = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&st, c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&tr2, c->d_scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (info) {
+    info->iters = st.it;
+    info->refinements = 0;
+    info->relres = st.bb > 0 ? std::sqrt(st.rr / st.bb) : 0.0;
+    info->true_relres = st.bb > 0 ? std::sqrt(tr2 / st.bb) : 0.0;
+    info->seconds = ms * 1e-3;
+  }
+  c->solved = true;
+  c->recovered = false;
+  if (st.done == 3)
+    return fail(c, HDG_ERR_NOT_CONVERGED, "trace solve broke down at iteration %d (non-positive curvature or NaN)",
+                st.it);
+  if (st.done == 2)
+    return fail(c, HDG_ERR_NOT_CONVERGED, "trace solve reached maxit=%d at relres %.3e", st.maxit,
+                st.bb > 0 ? std::sqrt(st.rr / st.bb) : 0.0);
+  return HDG_OK;
+}
+
+int recover_impl(hdg_ctx* c) {
+  if (!c->solved) return fail(c, HDG_ERR_STATE, "hdg_solve has not been called");
+  LAUNCHCHK(c, launch_recover(c->k, c->mesh, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream));
+  c->recovered = true;
+  return HDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdg_create(hdg_ctx** out, int device, size_t hbm_budget) {
+  if (!out) return HDG_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HDG_ERR_HIP;
+  if (device < 0 || device >= ndev) return HDG_ERR_ARG;
+  hdg_ctx* c = new hdg_ctx();
+  c->device = device;
+  c->budget = hbm_budget;
+  int r = select_device(c);
+  if (r) { delete c; return r; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_st, 2 * sizeof(CgState), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return HDG_ERR_HIP;
+  }
+  if (dalloc(c, &c->d_err, 1) || dalloc(c, &c->cg.st, 1) || dalloc(c, &c->d_scal, 4) ||
+      dalloc(c, &c->d_ticket2, 4)) {
+    delete c;
+    return HDG_ERR_HIP;
+  }
+  (void)hipMemset(c->cg.st, 0, sizeof(CgState));
+  (void)hipMemset(c->d_ticket2, 0, 4 * sizeof(unsigned));
+  *out = c;
+  return HDG_OK;
+}
+
+void hdg_destroy(hdg_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  dfree(c, c->d_err);
+  dfree(c, c->cg.st);
+  dfree(c, c->d_scal);
+  dfree(c, c->d_ticket2);
+  if (c->h_st) (void)hipHostFree(c->h_st);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* hdg_last_error(const hdg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hdg_memory_used(const hdg_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return HDG_ERR_ARG;
+  *bytes = c->used;
+  return HDG_OK;
+}
+
+int hdg_set_mesh(hdg_ctx* c, int64_t nn, const double* xy, int64_t nc, const int64_t* cells,
+                 int64_t nf, const int64_t* faces) {
+  if (!c) return HDG_ERR_ARG;
+  if (nn < 3 || nc < 1 || nf < 3 || !xy || !cells || !faces)
+    return fail(c, HDG_ERR_ARG, "empty mesh or null array");
+  if (nn > INT32_MAX || nf > INT32_MAX || nc > INT32_MAX / 9)
+    return fail(c, HDG_ERR_ARG, "mesh too large for 32-bit device indices");
+  TRY(select_device(c));
+  // validate and convert to 0-based int32
+  std::vector<int> cn(3 * nc), cf(3 * nc), fcell(2 * nf);
+  for (int64_t f = 0; f < nf; ++f) {
+    const int64_t c1 = faces[f + 2 * nf], c2 = faces[f + 3 * nf];
+    if (c1 < 1 || c1 > nc || c2 < 0 || c2 > nc || c1 == c2)
+      return fail(c, HDG_ERR_ARG, "face %lld has invalid cells (%lld, %lld)", (long long)f + 1,
+                  (long long)c1, (long long)c2);
+    fcell[2 * f] = (int)(c1 - 1);
+    fcell[2 * f + 1] = (int)(c2 - 1);
+  }
+  int64_t nblk = 0;
+  for (int64_t f = 0; f < nf; ++f) nblk += faces[f + 3 * nf] > 0 ? 5 : 3;
+  for (int64_t e = 0; e < nc; ++e)
+    for (int l = 0; l < 3; ++l) {
+      const int64_t nd = cells[6 * e + l], fc = cells[6 * e + 3 + l];
+      if (nd < 1 || nd > nn) return fail(c, HDG_ERR_ARG, "cell %lld: node id out of range", (long long)e + 1);
+      if (fc < 1 || fc > nf) return fail(c, HDG_ERR_ARG, "cell %lld: face id out of range", (long long)e + 1);
+      if (fcell[2 * (fc - 1)] != e && fcell[2 * (fc - 1) + 1] != e)
+        return fail(c, HDG_ERR_ARG, "cell %lld: face %lld does not list the cell", (long long)e + 1,
+                    (long long)fc);
+      cn[3 * e + l] = (int)(nd - 1);
+      cf[3 * e + l] = (int)(fc - 1);
+    }
+  free_mesh(c);
+  double* dxy;
+  int *dcn, *dcf, *dfc;
+  TRY(dalloc(c, &dxy, 2 * nn));
+  TRY(dalloc(c, &dcn, 3 * nc));
+  TRY(dalloc(c, &dcf, 3 * nc));
+  TRY(dalloc(c, &dfc, 2 * nf));
+  HIPCHK(c, hipMemcpyAsync(dxy, xy, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dcn, cn.data(), sizeof(int) * 3 * nc, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dcf, cf.data(), sizeof(int) * 3 * nc, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dfc, fcell.data(), sizeof(int) * 2 * nf, hipMemcpyHostToDevice, c->stream));
+  c->mesh.nn = nn;
+  c->mesh.nc = nc;
+  c->mesh.nf = nf;
+  c->mesh.xy = dxy;
+  c->mesh.cn = dcn;
+  c->mesh.cf = dcf;
+  c->mesh.fcell = dfc;
+  TRY(dalloc(c, &c->pat.bcol, kSlots * nf));
+  TRY(dalloc(c, &c->pat.src, 2 * kSlots * nf));
+  TRY(dalloc(c, &c->pat.nslot, nf));
+  TRY(dalloc(c, &c->pat.self, nf));
+  TRY(dalloc(c, &c->pat.tslot, kSlots * nf));
+  LAUNCHCHK(c, launch_pattern(c->mesh, c->pat, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->h_cf.swap(cf);
+  c->nblock_cols = nblk;
+  c->has_mesh = true;
+  return HDG_OK;
+}
+
+int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
+  if (!c || !opt) return HDG_ERR_ARG;
+  if (!c->has_mesh) return fail(c, HDG_ERR_STATE, "hdg_set_mesh has not been called");
+  if (opt->k < 1 || opt->k > 4) return fail(c, HDG_ERR_UNSUPPORTED, "k=%d: supported orders are 1..4", opt->k);
+  if (opt->precision != HDG_FP64 && opt->precision != HDG_MIXED)
+    return fail(c, HDG_ERR_ARG, "unknown precision mode %d", opt->precision);
+  TRY(select_device(c));
+  hdg_options o = *opt;
+  if (o.quad_degree <= 0) o.quad_degree = o.k + 1;
+  RefTables tab;
+  if (!tab.build(o.k, o.quad_degree))
+    return fail(c, HDG_ERR_QUAD, "Quadrature rule of order %d not available", o.quad_degree);
+  free_problem(c);
+  c->opt = o;
+  c->k = o.k;
+  c->nu = tab.lay.nu;
+  c->nq = 2 * c->nu;
+  c->m = 3 * c->nu;
+  c->nt = tab.lay.nt;
+  c->ntr = 3 * c->nt;
+  c->nqp = tab.lay.nqp;
+  c->nfq = tab.lay.nfq;
+  c->tab = tab;
+  const TableLayout& L = tab.lay;
+  c->lay = TabLayoutDev{L.nu, L.nt, L.nqp, L.nfq, L.Mref, L.Dref, L.Cref, L.G, L.Href, L.Nw, L.N, L.Mg, L.W, L.total};
+  TRY(dalloc(c, &c->d_tab, L.total));
+  HIPCHK(c, hipMemcpyAsync(c->d_tab, tab.data.data(), sizeof(double) * L.total, hipMemcpyHostToDevice, c->stream));
+  const int64_t nc = c->mesh.nc, nf = c->mesh.nf, N = nf * c->nt;
+  TRY(dalloc(c, &c->d_Ate, nc * c->ntr * c->ntr));
+  TRY(dalloc(c, &c->d_bte, nc * c->ntr));
+  TRY(dalloc(c, &c->d_Ke, nc * c->m * c->ntr));
+  TRY(dalloc(c, &c->d_be, nc * c->m));
+  TRY(dalloc(c, &c->d_qu, nc * c->m));
+  c->T = TraceSystem{};
+  c->T.nt = c->nt;
+  c->T.nf = nf;
+  c->T.N = N;
+  TRY(dalloc(c, &c->T.vals, (size_t)kSlots * c->nt * N));
+  TRY(dalloc(c, &c->T.rhs, N));
+  TRY(dalloc(c, &c->T.Dinv, (size_t)c->nt * N));
+  TRY(dalloc(c, &c->d_dmask, N));
+  TRY(dalloc(c, &c->d_uD, N));
+  HIPCHK(c, hipMemsetAsync(c->d_dmask, 0, N, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_uD, 0, sizeof(double) * N, c->stream));
+  c->T.dmask = c->d_dmask;
+  c->T.uD = c->d_uD;
+  c->cg.nblocks = cg_blocks(N, c->nt);
+  TRY(dalloc(c, &c->cg.x, N));
+  TRY(dalloc(c, &c->cg.r, N));
+  TRY(dalloc(c, &c->cg.z, N));
+  TRY(dalloc(c, &c->cg.p0, N));
+  TRY(dalloc(c, &c->cg.p1, N));
+  TRY(dalloc(c, &c->cg.q, N));
+  TRY(dalloc(c, &c->cg.part, 2 * (size_t)c->cg.nblocks + 64));
+  TRY(dalloc(c, &c->d_uhat, N));
+  TRY(dalloc(c, &c->d_part2, 4096));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->src = SrcParams{};
+  c->jacobi_dirty = true;
+  c->has_setup = true;
+  return HDG_OK;
+}
+
+int hdg_set_source(hdg_ctx* c, const hdg_source* s) {
+  if (!c || !s) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  if (s->kind < 0 || s->kind > 3) return fail(c, HDG_ERR_ARG, "unknown source kind %d", s->kind);
+  TRY(select_device(c));
+  SrcParams p;
+  p.kind = s->kind;
+  p.kappa_on = s->kappa_on ? 1 : 0;
+  p.ka = s->ka;
+  p.kb = s->kb;
+  for (int i = 0; i < 16; ++i) p.coef[i] = s->coef[i];
+  dfree(c, c->d_fqp);
+  if (s->kind == 3) {
+    if (!s->f_qp) return fail(c, HDG_ERR_ARG, "HDG_SRC_VALUES needs f_qp");
+    const size_t n = (size_t)c->mesh.nc * c->nqp;
+    TRY(dalloc(c, &c->d_fqp, n));
+    HIPCHK(c, hipMemcpyAsync(c->d_fqp, s->f_qp, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->src = p;
+  c->assembled = c->solved = c->recovered = false;
+  return HDG_OK;
+}
+
+int hdg_set_dirichlet(hdg_ctx* c, int64_t n, const int64_t* dofs, const double* vals) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  if (n < 0 || (n > 0 && (!dofs || !vals))) return fail(c, HDG_ERR_ARG, "bad Dirichlet arrays");
+  TRY(select_device(c));
+  const int64_t N = c->T.N;
+  std::vector<unsigned char> mask(N, 0);
+  std::vector<double> uD(N, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (dofs[i] < 1 || dofs[i] > N)
+      return fail(c, HDG_ERR_ARG, "Dirichlet dof %lld out of range 1..%lld", (long long)dofs[i], (long long)N);
+    mask[dofs[i] - 1] = 1;
+    uD[dofs[i] - 1] = vals[i];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_dmask, mask.data(), N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_uD, uD.data(), sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->jacobi_dirty = true;
+  c->solved = c->recovered = false;
+  return HDG_OK;
+}
+
+int hdg_query_sizes(const hdg_ctx* c, hdg_sizes* s) {
+  if (!c || !s) return HDG_ERR_ARG;
+  std::memset(s, 0, sizeof(*s));
+  s->n_nodes = c->mesh.nn;
+  s->n_cells = c->mesh.nc;
+  s->n_faces = c->mesh.nf;
+  if (!c->has_setup) return HDG_OK;
+  s->n_dofs = c->T.N;
+  s->nu = c->nu;
+  s->nq = c->nq;
+  s->nt = c->nt;
+  s->m = c->m;
+  s->ntr = c->ntr;
+  s->nqp = c->nqp;
+  s->nfq = c->nfq;
+  // rows of a face with two cells hold 5 block columns, with one cell 3
+  s->nnz = c->nblock_cols * (int64_t)c->nt * c->nt;
+  return HDG_OK;
+}
+
+int hdg_assemble_condense(hdg_ctx* c) {
+  if (!c) return HDG_ERR_ARG;
+  TRY(select_device(c));
+  TRY(assemble_impl(c));
+  return check_err_flag(c);
+}
+
+int hdg_solve(hdg_ctx* c, double* uhat_out, hdg_solve_info* info) {
+  if (!c) return HDG_ERR_ARG;
+  TRY(select_device(c));
+  int r = solve_impl(c, info);
+  if (r != HDG_OK && r != HDG_ERR_NOT_CONVERGED) return r;
+  if (uhat_out) {
+    HIPCHK(c, hipMemcpyAsync(uhat_out, c->d_uhat, sizeof(double) * c->T.N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return r;
+}
+
+int hdg_set_trace(hdg_ctx* c, const double* uhat) {
+  if (!c || !uhat) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  HIPCHK(c, hipMemcpyAsync(c->d_uhat, uhat, sizeof(double) * c->T.N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->solved = true;
+  c->recovered = false;
+  return HDG_OK;
+}
+
+int hdg_recover(hdg_ctx* c, double* q, double* u, double* uhat_cell) {
+  if (!c) return HDG_ERR_ARG;
+  TRY(select_device(c));
+  if (!c->recovered) TRY(recover_impl(c));
+  const int64_t nc = c->mesh.nc;
+  const int M = c->m, nq = c->nq, nu = c->nu, nt = c->nt;
+  if (q || u) {
+    std::vector<double> qu((size_t)nc * M);
+    HIPCHK(c, hipMemcpyAsync(qu.data(), c->d_qu, sizeof(double) * qu.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t e = 0; e < nc; ++e) {
+      if (q)
+        for (int i = 0; i < nq; ++i) q[e + nc * i] = qu[e * M + i];
+      if (u)
+        for (int i = 0; i < nu; ++i) u[e + nc * i] = qu[e * M + nq + i];
+    }
+  }
+  if (uhat_cell) {
+    std::vector<double> uh(c->T.N);
+    HIPCHK(c, hipMemcpyAsync(uh.data(), c->d_uhat, sizeof(double) * uh.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t e = 0; e < nc; ++e)
+      for (int lf = 0; lf < 3; ++lf)
+        for (int t = 0; t < nt; ++t) uhat_cell[e + nc * (t + nt * lf)] = uh[(int64_t)c->h_cf[3 * e + lf] * nt + t];
+  }
+  return HDG_OK;
+}
+
+int hdg_errornorm(hdg_ctx* c, int kind, const double* par, double* sq) {
+  if (!c || !sq) return HDG_ERR_ARG;
+  if (kind != HDG_UEX_SINSIN && kind != HDG_UEX_LINEAR) return fail(c, HDG_ERR_ARG, "unknown exact solution %d", kind);
+  if (kind == HDG_UEX_LINEAR && !par) return fail(c, HDG_ERR_ARG, "HDG_UEX_LINEAR needs 3 parameters");
+  TRY(select_device(c));
+  if (!c->recovered) TRY(recover_impl(c));
+  const double p3[3] = {par ? par[0] : 0.0, par ? par[1] : 0.0, par ? par[2] : 0.0};
+  LAUNCHCHK(c, launch_errornorm(c->mesh, c->d_tab, c->lay, c->k, c->d_qu, kind, p3, c->d_part2, c->d_scal,
+                                c->d_ticket2, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sq, c->d_scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HDG_OK;
+}
+
+int hdg_run(hdg_ctx* c, hdg_solve_info* info) {
+  if (!c) return HDG_ERR_ARG;
+  TRY(select_device(c));
+  TRY(assemble_impl(c));
+  int r = solve_impl(c, info);
+  if (r != HDG_OK) return r;
+  TRY(recover_impl(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_err_flag(c);
+}
+
+int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  const int64_t nf = c->mesh.nf, N = c->T.N;
+  const int nt = c->nt;
+  std::vector<int> bcol(kSlots * nf);
+  std::vector<unsigned char> nslot(nf), tslot(kSlots * nf);
+  HIPCHK(c, hipMemcpyAsync(bcol.data(), c->pat.bcol, sizeof(int) * bcol.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(nslot.data(), c->pat.nslot, nf, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(tslot.data(), c->pat.tslot, tslot.size(), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> vals;
+  if (nzval) {
+    vals.resize((size_t)kSlots * nt * N);
+    HIPCHK(c, hipMemcpyAsync(vals.data(), c->T.vals, sizeof(double) * vals.size(), hipMemcpyDeviceToHost, c->stream));
+  }
+  if (rhs) HIPCHK(c, hipMemcpyAsync(rhs, c->T.rhs, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // CSC of K: column (g, b) holds rows (f, a) for the faces f of row g's
+  // pattern (the pattern is structurally symmetric), in ascending order.
+  int64_t w = 0;
+  if (colptr) colptr[0] = 1;
+  for (int64_t g = 0; g < nf; ++g)
+    for (int b = 0; b < nt; ++b) {
+      for (int s = 0; s < nslot[g]; ++s) {
+        const int64_t f = bcol[kSlots * g + s];
+        const int ts = tslot[kSlots * g + s];
+        for (int a = 0; a < nt; ++a) {
+          if (rowval) rowval[w] = f * nt + a + 1;
+          if (nzval) nzval[w] = vals[(size_t)(ts * nt + b) * N + f * nt + a];
+          ++w;
+        }
+      }
+      if (colptr) colptr[g * nt + b + 1] = w + 1;
+    }
+  return HDG_OK;
+}
+
+int hdg_get_local_ops(hdg_ctx* c, double* Ke, double* be) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  const int64_t nc = c->mesh.nc;
+  if (Ke) HIPCHK(c, hipMemcpyAsync(Ke, c->d_Ke, sizeof(double) * nc * c->m * c->ntr, hipMemcpyDeviceToHost, c->stream));
+  if (be) HIPCHK(c, hipMemcpyAsync(be, c->d_be, sizeof(double) * nc * c->m, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HDG_OK;
+}
+
+int hdg_get_condensed(hdg_ctx* c, double* Ate, double* bte) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  const int64_t nc = c->mesh.nc;
+  const int ntr = c->ntr;
+  if (Ate) {
+    std::vector<double> a((size_t)nc * ntr * ntr);
+    HIPCHK(c, hipMemcpyAsync(a.data(), c->d_Ate, sizeof(double) * a.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t e = 0; e < nc; ++e)
+      for (int i = 0; i < ntr; ++i)
+        for (int j = 0; j < ntr; ++j) Ate[e * ntr * ntr + i + ntr * j] = a[e * ntr * ntr + i * ntr + j];
+  }
+  if (bte) {
+    HIPCHK(c, hipMemcpyAsync(bte, c->d_bte, sizeof(double) * nc * ntr, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return HDG_OK;
+}
+
+int hdg_get_element_blocks(hdg_ctx* c, int64_t first, int64_t count, double* A, double* B, double* C,
+                           double* E, double* F, double* H, double* be) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  if (first < 0 || count < 1 || first + count > c->mesh.nc) return fail(c, HDG_ERR_ARG, "cell range out of bounds");
+  TRY(select_device(c));
+  const int nq = c->nq, nu = c->nu, ntr = c->ntr;
+  const size_t sz[7] = {(size_t)nq * nq, (size_t)nq * nu, (size_t)nu * nu, (size_t)nq * ntr,
+                        (size_t)nu * ntr, (size_t)ntr * ntr, (size_t)nu};
+  double* host[7] = {A, B, C, E, F, H, be};
+  double* dev[7] = {nullptr};
+  int rc = HDG_OK;
+  for (int i = 0; i < 7 && rc == HDG_OK; ++i)
+    if (host[i]) rc = dalloc(c, &dev[i], sz[i] * count);
+  if (rc == HDG_OK) {
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    CondenseArgs a;
+    a.mesh = c->mesh;
+    a.tables = c->d_tab;
+    a.lay = c->lay;
+    a.src = c->src;
+    a.src.f_qp = c->d_fqp;
+    a.tau = c->opt.tau;
+    a.err = c->d_err;
+    a.first = first;
+    a.count = count;
+    a.A = dev[0]; a.B = dev[1]; a.C = dev[2]; a.E = dev[3]; a.F = dev[4]; a.H = dev[5]; a.bev = dev[6];
+    int e = launch_condense(c->k, a, true, c->stream);
+    if (e) rc = fail(c, HDG_ERR_HIP, "element block kernel failed: %s", hipGetErrorString((hipError_t)e));
+    for (int i = 0; i < 7 && rc == HDG_OK; ++i)
+      if (host[i] && hipMemcpyAsync(host[i], dev[i], sizeof(double) * sz[i] * count, hipMemcpyDeviceToHost,
+                                    c->stream) != hipSuccess)
+        rc = fail(c, HDG_ERR_HIP, "copy of element blocks failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == HDG_OK) rc = fail(c, HDG_ERR_HIP, "sync failed");
+  }
+  for (int i = 0; i < 7; ++i) dfree(c, dev[i]);
+  return rc;
+}
+
+int hdg_dubiner(int k, double r, double s, double* val, double* grad) {
+  if (k < 0 || k > 14 || !val) return HDG_ERR_ARG;
+  dubiner_eval(k, r, s, val, grad);
+  return HDG_OK;
+}
+
+int hdg_quadrature(int dim, int order, int cap, double* pts, double* w) {
+  std::vector<double> p, ww;
+  if (dim == 2) {
+    if (!triangle_rule(order, p, ww)) return -HDG_ERR_QUAD;
+  } else if (dim == 1) {
+    if (order < 1) return -HDG_ERR_QUAD;
+    gauss_legendre01(order, p, ww);
+  } else {
+    return -HDG_ERR_ARG;
+  }
+  const int n = (int)ww.size();
+  if (n > cap || !pts || !w) return -HDG_ERR_ARG;
+  std::copy(p.begin(), p.end(), pts);
+  std::copy(ww.begin(), ww.end(), w);
+  return n;
+}
+
+int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
+  if (!c || !avg_us || reps < 1) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "nothing assembled to time");
+  TRY(select_device(c));
+  // make sure the solver state exists (pq, rz) so the kernels divide by
+  // finite numbers; timing runs leave x and r untouched
+  if (!c->solved) LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, 1e-12, 1, c->stream));
+  CondenseArgs a;
+  a.mesh = c->mesh;
+  a.tables = c->d_tab;
+  a.lay = c->lay;
+  a.src = c->src;
+  a.src.f_qp = c->d_fqp;
+  a.tau = c->opt.tau;
+  a.Ate = c->d_Ate;
+  a.bte = c->d_bte;
+  a.Ke = c->d_Ke;
+  a.be = c->d_be;
+  a.err = c->d_err;
+  auto once = [&]() -> int {
+    if (which == 0) return launch_cg_spmv_only(c->pat, c->T, c->cg, 1, c->stream);
+    if (which == 1) return launch_cg_update_only(c->T, c->cg, 1, c->stream);
+    return launch_condense(c->k, a, false, c->stream);
+  };
+  LAUNCHCHK(c, once());  // warm-up
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < reps; ++i) LAUNCHCHK(c, once());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *avg_us = 1e3 * ms / reps;
+  c->solved = c->recovered = false;  // the CG state was overwritten
+  return HDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Internal interface between the C ABI (hdg_api.hip) and the device
+// kernels (hdg_kernels.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hdg {
+
+constexpr int kSlots = 5;  // block columns per face row: self + 2 faces of each cell
+
+// Device-resident mesh, 0-based, int32 (meshes up to 2^31 faces).
+struct DevMesh {
+  int64_t nn = 0, nc = 0, nf = 0;
+  const double* xy = nullptr;  // [2*nn]
+  const int* cn = nullptr;     // [3*nc] node ids
+  const int* cf = nullptr;     // [3*nc] face ids
+  const int* fcell = nullptr;  // [2*nf] cell1, cell2 (-1 none)
+};
+
+// Face-level pattern of the trace matrix.  Row block f has nslot[f] (3 or
+// 5) block columns bcol[f*5+s], sorted ascending; slots beyond nslot repeat
+// f with zero values so that every row has 5 slots.  src[(f*5+s)*2+t] is
+// the t-th contributing element block, encoded cell*9 + la*3 + lb (local
+// face of f, local face of the column face), or -1.  tslot[f*5+s] is the
+// slot of f in the row of bcol[f*5+s] (the transpose map).
+struct DevPattern {
+  int* bcol = nullptr;
+  int* src = nullptr;
+  unsigned char* nslot = nullptr;
+  unsigned char* self = nullptr;
+  unsigned char* tslot = nullptr;
+};
+
+struct SrcParams {
+  int kind = 0, kappa_on = 0;
+  double ka = 0, kb = 0;
+  double coef[16] = {0};
+  const double* f_qp = nullptr;  // [ncells x nqp] column-major
+};
+
+struct TabLayoutDev {
+  int nu, nt, nqp, nfq;
+  int Mref, Dref, Cref, G, Href, Nw, N, Mg, W, total;
+};
+
+// Element build + condensation.  Per cell outputs: Ate row-major ntr x ntr,
+// bte[ntr], Ke column-major m x ntr, be[m].  Element blocks mode writes the
+// A..H blocks instead (column-major per cell) for cells [first, first+count).
+struct CondenseArgs {
+  DevMesh mesh;
+  const double* tables = nullptr;
+  TabLayoutDev lay;
+  SrcParams src;
+  double tau = 1.0;
+  double* Ate = nullptr;
+  double* bte = nullptr;
+  double* Ke = nullptr;
+  double* be = nullptr;
+  int* err = nullptr;  // bit 1: det(J) <= 0; bit 2: singular local matrix
+  // element-block mode
+  int64_t first = 0, count = 0;
+  double *A = nullptr, *B = nullptr, *C = nullptr, *E = nullptr, *F = nullptr, *H = nullptr,
+         *bev = nullptr;
+};
+
+// PCG scalar state, resident on the device.
+struct CgState {
+  double rz[2];    // (r, z) of the current and previous iterate, by parity
+  double pq;       // (p, q) of the current iteration
+  double rr;       // (r, r) after the current update
+  double bb;       // (b, b)
+  double rtol2;    // rtol^2 * bb
+  int it;          // completed iterations
+  int maxit;
+  int done;        // 1 converged, 2 iteration cap, 3 breakdown
+  int pad;
+  unsigned ticket[4];
+};
+
+struct TraceSystem {
+  int nt = 0;
+  int64_t nf = 0, N = 0;
+  double* vals = nullptr;  // ELL, column-major: vals[(s*nt + b)*N + row]
+  double* rhs = nullptr;   // assembled rhs (N)
+  double* Dinv = nullptr;  // block-Jacobi inverse: Dinv[b*N + row] (0 on Dirichlet dofs)
+  const unsigned char* dmask = nullptr;  // 1 = Dirichlet dof
+  const double* uD = nullptr;            // Dirichlet values (0 elsewhere)
+};
+
+struct CgVectors {
+  double *x, *r, *z, *p0, *p1, *q, *part;
+  CgState* st;
+  int nblocks;
+};
+
+// launchers (return hipError_t as int)
+int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s);
+int launch_condense(int k, const CondenseArgs& a, bool blocks_mode, hipStream_t s);
+int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
+                hipStream_t s);
+int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);
+int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, double rtol, int maxit,
+                   hipStream_t s);
+int launch_cg_iteration(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s);
+// force = 1 runs the kernel regardless of the convergence flag and leaves
+// x, r untouched (timing only; the CG state is not usable afterwards)
+int launch_cg_spmv_only(const DevPattern& P, const TraceSystem& T, CgVectors& v, int force,
+                        hipStream_t s);
+int launch_cg_update_only(const TraceSystem& T, CgVectors& v, int force, hipStream_t s);
+int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s);
+int launch_true_residual(const DevPattern& P, const TraceSystem& T, const double* uhat, double* out2,
+                         CgVectors& v, hipStream_t s);
+int launch_recover(int k, const DevMesh& m, const double* Ke, const double* be, const double* uhat,
+                   double* qu, hipStream_t s);
+int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k,
+                     const double* qu, int uex_kind, const double* par3, double* part,
+                     double* result, unsigned* ticket, hipStream_t s);
+int cg_blocks(int64_t N, int nt);
+
+}  // namespace hdg

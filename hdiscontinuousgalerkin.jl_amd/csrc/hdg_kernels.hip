@@ -1,0 +1,947 @@
+// Device kernels of the HDG Poisson pipeline for gfx950 (CDNA4).
+//
+//   pattern    face-level sparsity of the trace matrix (once per mesh)
+//   condense   element build + static condensation: one segment of lanes
+//              per element, one lane per row of the bordered local system
+//              [[Me, EFe, Bte], [Ge, H, 0]]; Gauss-Jordan with partial
+//              pivoting in registers leaves K_e = Me^-1 EFe, b_e = Me^-1 Bte
+//              in the pivot lanes and -Ate, bte in the border lanes
+//              (examples/poisson2D_HDG.jl:77-174)
+//   fill       gather of element blocks into the trace matrix (no atomics:
+//              each entry has at most two contributions, summed in order)
+//   jacobi     inverse of each face's diagonal block (block-Jacobi)
+//   cg_*       preconditioned CG on S = -K_II, two kernels per iteration,
+//              deterministic reductions (last-block tickets)
+//   recover    [q;u]_e = K_e uhat_e + b_e (poisson2D_HDG.jl:197-212)
+//   errornorm  squared L2 error (src/DiscreteFunctions.jl:97-120)
+#include <hip/hip_runtime.h>
+
+#include "hdg_internal.h"
+
+namespace hdg {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979311600;
+
+// ------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum in a fixed order; result valid in thread 0.
+template <int NW>
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NW; ++i) s += sh[i];
+  __syncthreads();
+  return s;
+}
+
+// Deterministic grid reduction: every block stores its partial; the block
+// that arrives last sums all partials in index order and calls `fin`.
+template <int NW, typename Fin>
+__device__ inline void grid_finish(const double* part_in, int nparts, int nvals, unsigned* ticket,
+                                   double* sh, Fin fin) {
+  __shared__ bool last;
+  __threadfence();
+  if (threadIdx.x == 0) last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double tot[2] = {0.0, 0.0};
+  for (int v = 0; v < nvals; ++v) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part_in[(size_t)v * nparts + i];
+    tot[v] = block_sum<NW>(s, sh);
+  }
+  if (threadIdx.x == 0) {
+    *ticket = 0u;
+    fin(tot);
+  }
+}
+
+// ------------------------------------------------------------- sources
+struct Src {
+  __device__ __forceinline__ static double kappa(const SrcParams& S, double x, double y) {
+    return S.kappa_on ? 1.0 + 0.5 * sin(S.ka * x) * cos(S.kb * y) : 1.0;
+  }
+  __device__ __forceinline__ static double f(const SrcParams& S, double x, double y) {
+    switch (S.kind) {
+      case 0: return 2.0 * (kPi * kPi) * sin(kPi * x) * sin(kPi * y);
+      case 1: {
+        double sx, cx, sy, cy;
+        sincos(kPi * x, &sx, &cx);
+        sincos(kPi * y, &sy, &cy);
+        const double u = sx * sy, ux = kPi * cx * sy, uy = kPi * sx * cy;
+        double k = 1.0, kx = 0.0, ky = 0.0;
+        if (S.kappa_on) {
+          double sa, ca, sb, cb;
+          sincos(S.ka * x, &sa, &ca);
+          sincos(S.kb * y, &sb, &cb);
+          k = 1.0 + 0.5 * sa * cb;
+          kx = 0.5 * S.ka * ca * cb;
+          ky = -0.5 * S.kb * sa * sb;
+        }
+        return k * 2.0 * kPi * kPi * u - (kx * ux + ky * uy);
+      }
+      case 2: {
+        double sxp[4], syq[4];
+        for (int p = 0; p < 4; ++p) {
+          sxp[p] = sin((p + 1) * kPi * x);
+          syq[p] = sin((p + 1) * kPi * y);
+        }
+        double v = 0.0;
+        for (int p = 0; p < 4; ++p)
+          for (int q = 0; q < 4; ++q) v += S.coef[p * 4 + q] * sxp[p] * syq[q];
+        return v;
+      }
+      default: return 0.0;
+    }
+  }
+};
+
+// ------------------------------------------------------------- geometry
+// reinit! (src/ScalarFunctionSpaces.jl:101-132) for an affine triangle.
+struct Geo {
+  double x[6];
+  double detJ;
+  double Ji[4];   // J^-1 column-major: Ji[a + 2b]
+  double nrm[6];  // unit outward normals of local faces 0..2
+  double len[3];  // edge lengths (detJf)
+  int ori[3];     // face_orientation (src/mesh.jl:51-54)
+};
+
+__device__ inline void cell_geometry(const DevMesh& m, int64_t c, Geo& g) {
+  int nd[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    nd[v] = __ldg(m.cn + 3 * c + v);
+    g.x[2 * v] = __ldg(m.xy + 2 * (int64_t)nd[v]);
+    g.x[2 * v + 1] = __ldg(m.xy + 2 * (int64_t)nd[v] + 1);
+  }
+  const double J0 = g.x[2] - g.x[0], J1 = g.x[3] - g.x[1];  // column 1
+  const double J2 = g.x[4] - g.x[0], J3 = g.x[5] - g.x[1];  // column 2
+  g.detJ = J0 * J3 - J2 * J1;
+  const double di = 1.0 / g.detJ;
+  g.Ji[0] = J3 * di;
+  g.Ji[1] = -J1 * di;
+  g.Ji[2] = -J2 * di;
+  g.Ji[3] = J0 * di;
+  const double wn[6] = {-(J1 - J3), J0 - J2, -J3, J2, J1, -J0};  // src/shapes.jl:169-176
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const double l = sqrt(wn[2 * f] * wn[2 * f] + wn[2 * f + 1] * wn[2 * f + 1]);
+    g.len[f] = l;
+    g.nrm[2 * f] = wn[2 * f] / l;
+    g.nrm[2 * f + 1] = wn[2 * f + 1] / l;
+  }
+  g.ori[0] = nd[2] > nd[1];
+  g.ori[1] = nd[0] > nd[2];
+  g.ori[2] = nd[1] > nd[0];
+}
+
+template <int K>
+struct Dims {
+  static constexpr int NU = (K + 1) * (K + 2) / 2;
+  static constexpr int NQ = 2 * NU, M = 3 * NU, NT = K + 1, NTR = 3 * NT;
+  static constexpr int ROWS = M + NTR, NC = M + NTR + 1;
+  static constexpr int SEG = ROWS <= 16 ? 16 : (ROWS <= 32 ? 32 : 64);
+  static constexpr int EPW = 64 / SEG;  // elements per wavefront
+};
+
+// Broadcast of lane `src` (absolute lane in the wavefront) within a segment.
+template <int SEG>
+__device__ inline double seg_bcast(double v, int src) {
+  if constexpr (SEG == 64) {
+    const int s = __builtin_amdgcn_readfirstlane(src);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), s);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), s);
+    return __hiloint2double(hi, lo);
+  } else {
+    return __shfl(v, src, 64);
+  }
+}
+
+// ------------------------------------------------------------- condense
+// One row of the bordered local system, for lane row r of the segment.
+//   r <  NQ : [ A(r,:) | -B(r,:) | -E(r,:) | 0      ]   q test functions
+//   r <  M  : [ B(:,i)'|  C(i,:) |  F(i,:) | be(i)  ]   u test functions
+//   r < ROWS: [ E(:,a)'|  F(:,a)'|  H(a,:) | 0      ]   trace test functions
+// with A = (kappa^-1 v_j, v_i), B = (u_j, div v_i), C = tau<u_j, w_i>,
+// E = <uhat_j, v_i.n>, F = tau<uhat_j, w_i>, H = <uhat_j, mu_i>
+// (examples/poisson2D_HDG.jl:88-153; the vector basis is block-by-component,
+// VectorFunctionSpaces.jl:189-195).
+template <int K>
+__device__ inline void build_row(int r, const double* T, const TabLayoutDev& L, const Geo& g,
+                                 double tau, const double* Aacc, double beacc,
+                                 double (&row)[Dims<K>::NC]) {
+  using D = Dims<K>;
+  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NT = D::NT, NTR = D::NTR;
+  // lane-dependent indices select among register values with ternaries
+  // (a dynamic index into a register array would move it to scratch)
+  if (r < NQ) {
+    const int c = r / NU, i = r - c * NU;
+#pragma unroll
+    for (int s = 0; s < NQ; ++s) row[s] = (s / NU == c) ? Aacc[s % NU] : 0.0;
+    const double j0 = c ? g.Ji[2] : g.Ji[0], j1 = c ? g.Ji[3] : g.Ji[1];
+#pragma unroll
+    for (int j = 0; j < NU; ++j)
+      row[NQ + j] = -g.detJ * (j0 * T[L.Dref + i * NU + j] + j1 * T[L.Dref + NU * NU + i * NU + j]);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double sc = -g.len[f] * (c ? g.nrm[2 * f + 1] : g.nrm[2 * f]);
+      const double* Gf = T + L.G + ((f * 2 + g.ori[f]) * NU + i) * NT;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) row[M + f * NT + t] = sc * Gf[t];
+    }
+    row[M + NTR] = 0.0;
+  } else if (r < M) {
+    const int i = r - NQ;
+#pragma unroll
+    for (int s = 0; s < NQ; ++s) {
+      const int cs = s / NU, is = s % NU;
+      row[s] = g.detJ * (g.Ji[0 + 2 * cs] * T[L.Dref + is * NU + i] +
+                         g.Ji[1 + 2 * cs] * T[L.Dref + NU * NU + is * NU + i]);
+    }
+#pragma unroll
+    for (int j = 0; j < NU; ++j)
+      row[NQ + j] = tau * (g.len[0] * T[L.Cref + (0 * NU + i) * NU + j] +
+                           g.len[1] * T[L.Cref + (1 * NU + i) * NU + j] +
+                           g.len[2] * T[L.Cref + (2 * NU + i) * NU + j]);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double sc = tau * g.len[f];
+      const double* Gf = T + L.G + ((f * 2 + g.ori[f]) * NU + i) * NT;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) row[M + f * NT + t] = sc * Gf[t];
+    }
+    row[M + NTR] = beacc;
+  } else if (r < D::ROWS) {
+    const int a = r - M, fa = a / NT, ta = a - fa * NT;
+    const int of = fa == 0 ? g.ori[0] : (fa == 1 ? g.ori[1] : g.ori[2]);
+    const double* Gf = T + L.G + (fa * 2 + of) * NU * NT;
+    const double el = fa == 0 ? g.len[0] : (fa == 1 ? g.len[1] : g.len[2]);
+    const double nx = el * (fa == 0 ? g.nrm[0] : (fa == 1 ? g.nrm[2] : g.nrm[4]));
+    const double ny = el * (fa == 0 ? g.nrm[1] : (fa == 1 ? g.nrm[3] : g.nrm[5]));
+#pragma unroll
+    for (int s = 0; s < NQ; ++s) row[s] = (s < NU ? nx : ny) * Gf[(s % NU) * NT + ta];
+#pragma unroll
+    for (int j = 0; j < NU; ++j) row[NQ + j] = tau * el * Gf[j * NT + ta];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        row[M + f * NT + t] = (f == fa) ? el * T[L.Href + ta * NT + t] : 0.0;
+    row[M + NTR] = 0.0;
+  } else {
+#pragma unroll
+    for (int c = 0; c < D::NC; ++c) row[c] = 0.0;
+  }
+}
+
+template <int K, bool BLOCKS>
+__global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
+  using D = Dims<K>;
+  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, SEG = D::SEG, EPW = D::EPW;
+  extern __shared__ double T[];
+  const TabLayoutDev L = a.lay;
+  for (int i = threadIdx.x; i < L.total; i += blockDim.x) T[i] = __ldg(a.tables + i);
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int seg = lane / SEG, ls = lane - seg * SEG, sbase = seg * SEG;
+  const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t first = BLOCKS ? a.first : 0;
+  const int64_t count = BLOCKS ? a.count : a.mesh.nc;
+  const int nqp = L.nqp;
+
+  for (int64_t base = wid * EPW; base < count; base += nw * EPW) {
+    const bool valid = base + seg < count;
+    const int64_t cell = first + (valid ? base + seg : count - 1);
+    Geo g;
+    cell_geometry(a.mesh, cell, g);
+    // quadrature-point data shared by the segment: lane q evaluates
+    // detJ * f(x_q) and detJ / kappa(x_q); rows collect them by shuffles
+    const int ui = (ls < NQ) ? ls % NU : (ls < M ? ls - NQ : 0);  // scalar basis index of this row
+    double Aacc[NU];
+#pragma unroll
+    for (int j = 0; j < NU; ++j) Aacc[j] = g.detJ * T[L.Mref + ui * NU + j];
+    double beacc = 0.0;
+    for (int q0 = 0; q0 < nqp; q0 += SEG) {
+      const int q = q0 + ls;
+      double fv = 0.0, kv = 0.0;
+      if (q < nqp) {
+        double xq = 0.0, yq = 0.0;
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          xq += T[L.Mg + v * nqp + q] * g.x[2 * v];
+          yq += T[L.Mg + v * nqp + q] * g.x[2 * v + 1];
+        }
+        fv = g.detJ * (a.src.kind == 3 ? __ldg(a.src.f_qp + cell + a.mesh.nc * (int64_t)q)
+                                       : Src::f(a.src, xq, yq));
+        kv = g.detJ / Src::kappa(a.src, xq, yq);
+      }
+      const int qn = min(SEG, nqp - q0);
+      for (int qq = 0; qq < qn; ++qq) {
+        const double fq = __shfl(fv, sbase + qq, 64);
+        beacc += T[L.Nw + ui * nqp + q0 + qq] * fq;
+        if (a.src.kappa_on) {
+          const double kq = __shfl(kv, sbase + qq, 64);
+          const double wn = T[L.Nw + ui * nqp + q0 + qq] * kq;
+          if (qq == 0 && q0 == 0) {
+#pragma unroll
+            for (int j = 0; j < NU; ++j) Aacc[j] = 0.0;
+          }
+#pragma unroll
+          for (int j = 0; j < NU; ++j) Aacc[j] += wn * T[L.N + j * nqp + q0 + qq];
+        }
+      }
+    }
+    double row[D::NC];
+    build_row<K>(ls, T, L, g, a.tau, Aacc, beacc, row);
+
+    if constexpr (BLOCKS) {
+      if (valid) {
+        const int64_t e = cell - first;
+        if (ls < NQ) {
+          for (int s = 0; s < NQ; ++s) if (a.A) a.A[e * NQ * NQ + ls + NQ * s] = row[s];
+          for (int j = 0; j < NU; ++j) if (a.B) a.B[e * NQ * NU + ls + NQ * j] = -row[NQ + j];
+          for (int t = 0; t < NTR; ++t) if (a.E) a.E[e * NQ * NTR + ls + NQ * t] = -row[M + t];
+        } else if (ls < M) {
+          const int i = ls - NQ;
+          for (int j = 0; j < NU; ++j) if (a.C) a.C[e * NU * NU + i + NU * j] = row[NQ + j];
+          for (int t = 0; t < NTR; ++t) if (a.F) a.F[e * NU * NTR + i + NU * t] = row[M + t];
+          if (a.bev) a.bev[e * NU + i] = row[M + NTR];
+        } else if (ls < D::ROWS) {
+          const int r = ls - M;
+          for (int t = 0; t < NTR; ++t) if (a.H) a.H[e * NTR * NTR + r + NTR * t] = row[M + t];
+        }
+      }
+      continue;
+    } else {
+      // scale of Me for the singularity test
+      double sc = 0.0;
+      if (ls < M) {
+#pragma unroll
+        for (int c = 0; c < M; ++c) sc = fmax(sc, fabs(row[c]));
+      }
+#pragma unroll
+      for (int off = SEG / 2; off > 0; off >>= 1) sc = fmax(sc, __shfl_xor(sc, off, SEG));
+      const double thr = 1e-10 * sc;
+      bool bad = !(g.detJ > 0.0);
+      bool used = !(ls < M);
+      int mycol = 0;
+      // Gauss-Jordan on the first M columns with partial pivoting among
+      // the rows not yet used as pivots (implicit row exchanges).
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double cand = used ? -1.0 : fabs(row[j]);
+        int bl = ls;
+#pragma unroll
+        for (int off = SEG / 2; off > 0; off >>= 1) {
+          const double oc = __shfl_xor(cand, off, SEG);
+          const int ob = __shfl_xor(bl, off, SEG);
+          if (oc > cand || (oc == cand && ob < bl)) {
+            cand = oc;
+            bl = ob;
+          }
+        }
+        const int src = sbase + bl;
+        const double pv = seg_bcast<SEG>(row[j], src);
+        bad |= !(fabs(pv) > thr);
+        const double inv = 1.0 / pv;
+        const bool me = (ls == bl);
+        const double l = row[j] * inv;
+#pragma unroll
+        for (int c = j + 1; c < D::NC; ++c) {
+          const double pc = seg_bcast<SEG>(row[c], src);
+          row[c] = me ? pc * inv : fma(-l, pc, row[c]);
+        }
+        if (me) {
+          used = true;
+          mycol = j;
+        }
+      }
+      if (valid) {
+        if (bad && ls == 0) atomicOr(a.err, g.detJ > 0.0 ? 2 : 1);
+        if (ls < M) {
+          double* ke = a.Ke + cell * (int64_t)(M * NTR);
+#pragma unroll
+          for (int b = 0; b < NTR; ++b) ke[b * M + mycol] = row[M + b];
+          a.be[cell * M + mycol] = row[M + NTR];
+        } else if (ls < D::ROWS) {
+          const int r = ls - M;
+          double* at = a.Ate + cell * (int64_t)(NTR * NTR) + r * NTR;
+#pragma unroll
+          for (int b = 0; b < NTR; ++b) at[b] = -row[M + b];
+          a.bte[cell * NTR + r] = row[M + NTR];
+        }
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------- pattern
+__global__ void pattern_kernel(DevMesh m, DevPattern P) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.nf) return;
+  const int c1 = m.fcell[2 * f], c2 = m.fcell[2 * f + 1];
+  int la1 = 0, la2 = 0;
+  for (int l = 0; l < 3; ++l) {
+    if (m.cf[3 * c1 + l] == f) la1 = l;
+    if (c2 >= 0 && m.cf[3 * c2 + l] == f) la2 = l;
+  }
+  int col[5], s1[5], s2[5], n = 0;
+  col[n] = (int)f;
+  s1[n] = c1 * 9 + la1 * 3 + la1;
+  s2[n] = c2 >= 0 ? c2 * 9 + la2 * 3 + la2 : -1;
+  ++n;
+  for (int l = 0; l < 3; ++l)
+    if (l != la1) { col[n] = m.cf[3 * c1 + l]; s1[n] = c1 * 9 + la1 * 3 + l; s2[n] = -1; ++n; }
+  if (c2 >= 0)
+    for (int l = 0; l < 3; ++l)
+      if (l != la2) { col[n] = m.cf[3 * c2 + l]; s1[n] = c2 * 9 + la2 * 3 + l; s2[n] = -1; ++n; }
+  for (int i = 1; i < n; ++i) {  // insertion sort by column
+    int cc = col[i], a1 = s1[i], a2 = s2[i], j = i - 1;
+    while (j >= 0 && col[j] > cc) { col[j + 1] = col[j]; s1[j + 1] = s1[j]; s2[j + 1] = s2[j]; --j; }
+    col[j + 1] = cc; s1[j + 1] = a1; s2[j + 1] = a2;
+  }
+  int self = 0;
+  for (int i = 0; i < n; ++i)
+    if (col[i] == f) self = i;
+  for (int i = n; i < kSlots; ++i) { col[i] = (int)f; s1[i] = -1; s2[i] = -1; }
+  for (int i = 0; i < kSlots; ++i) {
+    P.bcol[f * kSlots + i] = col[i];
+    P.src[(f * kSlots + i) * 2] = s1[i];
+    P.src[(f * kSlots + i) * 2 + 1] = s2[i];
+  }
+  P.nslot[f] = (unsigned char)n;
+  P.self[f] = (unsigned char)self;
+}
+
+__global__ void tslot_kernel(int64_t nf, DevPattern P) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int n = P.nslot[f];
+  for (int s = 0; s < kSlots; ++s) {
+    unsigned char t = P.self[f];
+    if (s < n) {
+      const int g = P.bcol[f * kSlots + s];
+      const int ng = P.nslot[g];
+      for (int u = 0; u < ng; ++u)
+        if (P.bcol[(int64_t)g * kSlots + u] == f) t = (unsigned char)u;
+    }
+    P.tslot[f * kSlots + s] = t;
+  }
+}
+
+// ----------------------------------------------------------------- fill
+// Row (f, a): slot s, entry b of the trace matrix K (assembler.jl:31-49,
+// duplicates summed in cell order) and the assembled rhs.
+template <int NT>
+__global__ __launch_bounds__(256) void fill_kernel(DevPattern P, const double* __restrict__ Ate,
+                                                   const double* __restrict__ bte, TraceSystem T) {
+  constexpr int NTR = 3 * NT;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= T.N) return;
+  const int64_t f = r / NT;
+  const int a = (int)(r - f * NT);
+  const int* sp = P.src + f * kSlots * 2;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const int c1 = sp[2 * s], c2 = sp[2 * s + 1];
+    double v[NT];
+#pragma unroll
+    for (int b = 0; b < NT; ++b) v[b] = 0.0;
+    if (c1 >= 0) {
+      const int64_t cell = c1 / 9;
+      const int la = (c1 / 3) % 3, lb = c1 % 3;
+      const double* blk = Ate + cell * NTR * NTR + (la * NT + a) * NTR + lb * NT;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) v[b] = blk[b];
+    }
+    if (c2 >= 0) {
+      const int64_t cell = c2 / 9;
+      const int la = (c2 / 3) % 3, lb = c2 % 3;
+      const double* blk = Ate + cell * NTR * NTR + (la * NT + a) * NTR + lb * NT;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) v[b] = v[b] + blk[b];
+    }
+#pragma unroll
+    for (int b = 0; b < NT; ++b) T.vals[(int64_t)(s * NT + b) * T.N + r] = v[b];
+  }
+  const int sf = P.self[f];
+  const int c1 = sp[2 * sf], c2 = sp[2 * sf + 1];
+  double rh = bte[(int64_t)(c1 / 9) * NTR + ((c1 / 3) % 3) * NT + a];
+  if (c2 >= 0) rh = rh + bte[(int64_t)(c2 / 9) * NTR + ((c2 / 3) % 3) * NT + a];
+  T.rhs[r] = rh;
+}
+
+// Block-Jacobi: inverse of -K_ff restricted to the non-Dirichlet dofs.
+template <int NT>
+__global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= T.nf) return;
+  const int s0 = P.self[f];
+  double a[NT][2 * NT];
+  bool dm[NT];
+#pragma unroll
+  for (int i = 0; i < NT; ++i) dm[i] = T.dmask[f * NT + i] != 0;
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      double v = -T.vals[(int64_t)(s0 * NT + j) * T.N + f * NT + i];
+      if (dm[i] || dm[j]) v = (i == j) ? 1.0 : 0.0;
+      a[i][j] = v;
+      a[i][NT + j] = (i == j) ? 1.0 : 0.0;
+    }
+#pragma unroll
+  for (int c = 0; c < NT; ++c) {
+    int p = c;
+    double mx = fabs(a[c][c]);
+#pragma unroll
+    for (int i = c + 1; i < NT; ++i)
+      if (fabs(a[i][c]) > mx) {
+        mx = fabs(a[i][c]);
+        p = i;
+      }
+    // row exchange through a fully unrolled select keeps `a` in registers
+#pragma unroll
+    for (int i = c + 1; i < NT; ++i)
+      if (i == p)
+#pragma unroll
+        for (int j = 0; j < 2 * NT; ++j) { const double t = a[c][j]; a[c][j] = a[i][j]; a[i][j] = t; }
+    const double inv = 1.0 / a[c][c];
+#pragma unroll
+    for (int j = 0; j < 2 * NT; ++j) a[c][j] *= inv;
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+      if (i != c) {
+        const double l = a[i][c];
+#pragma unroll
+        for (int j = 0; j < 2 * NT; ++j) a[i][j] -= l * a[c][j];
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < NT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+      T.Dinv[(int64_t)j * T.N + f * NT + i] = (dm[i] || dm[j]) ? 0.0 : a[i][NT + j];
+}
+
+// ------------------------------------------------------------------- CG
+// S = -K on the non-Dirichlet dofs.  Iteration i (it = i-1 completed):
+//   spmv:   beta = rz[it]/rz[it-1] (0 at it = 0); p_new = z + beta p_old;
+//           q = S p_new; (p_new, q)
+//   update: alpha = rz/pq; x += alpha p; r -= alpha q; z = Dinv r;
+//           (r, z), (r, r); convergence test; it += 1
+constexpr int kCgThreads = 256;
+
+__device__ inline double pnew_of(double z, double p, double beta) { return fma(beta, p, z); }
+
+template <int NT>
+__global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceSystem T, CgVectors v,
+                                                             int force) {
+  __shared__ double sh[kCgThreads / 64];
+  const CgState* st = v.st;
+  if (!force && st->done) return;
+  const int it = st->it;
+  const double beta = it == 0 ? 0.0 : st->rz[it & 1] / st->rz[(it + 1) & 1];
+  const double* __restrict__ pold = (it & 1) ? v.p1 : v.p0;
+  double* __restrict__ pnw = (it & 1) ? v.p0 : v.p1;
+  const double* __restrict__ z = v.z;
+  double part = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = r / NT;
+    const int* bc = P.bcol + f * kSlots;
+    double acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const int64_t g = (int64_t)__ldg(bc + s) * NT;
+#pragma unroll
+      for (int b = 0; b < NT; ++b)
+        acc = fma(__ldg(T.vals + (int64_t)(s * NT + b) * T.N + r), pnew_of(z[g + b], pold[g + b], beta), acc);
+    }
+    const double q = T.dmask[r] ? 0.0 : -acc;
+    const double pn = pnew_of(z[r], pold[r], beta);
+    pnw[r] = pn;
+    v.q[r] = q;
+    part = fma(pn, q, part);
+  }
+  const double bs = block_sum<kCgThreads / 64>(part, sh);
+  if (threadIdx.x == 0) v.part[blockIdx.x] = bs;
+  CgState* stw = v.st;
+  grid_finish<kCgThreads / 64>(v.part, gridDim.x, 1, &stw->ticket[0], sh,
+                               [&](const double* t) { stw->pq = t[0]; });
+}
+
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, int force) {
+  constexpr int NTH = 64 * NT;
+  __shared__ double rs[NTH];
+  __shared__ double sh[NT];
+  CgState* st = v.st;
+  if (!force && st->done) return;
+  const int it = st->it;
+  const double rz = st->rz[it & 1];
+  const double alpha = rz / st->pq;
+  const double* __restrict__ p = (it & 1) ? v.p0 : v.p1;
+  double prz = 0.0, prr = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+    const int64_t r = r0 + threadIdx.x;
+    double rn = 0.0;
+    if (r < T.N) {
+      const double xn = fma(alpha, p[r], v.x[r]);
+      rn = fma(-alpha, v.q[r], v.r[r]);
+      if (!force) {
+        v.x[r] = xn;
+        v.r[r] = rn;
+      }
+    }
+    rs[threadIdx.x] = rn;
+    __syncthreads();
+    if (r < T.N) {
+      const int lb = (threadIdx.x / NT) * NT;
+      double zr = 0.0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) zr = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], zr);
+      v.z[r] = zr;
+      prz = fma(rn, zr, prz);
+      prr = fma(rn, rn, prr);
+    }
+    __syncthreads();
+  }
+  const double a = block_sum<NT>(prz, sh);
+  const double b = block_sum<NT>(prr, sh);
+  if (threadIdx.x == 0) {
+    v.part[blockIdx.x] = a;
+    v.part[gridDim.x + blockIdx.x] = b;
+  }
+  if (force) return;
+  grid_finish<NT>(v.part, gridDim.x, 2, &st->ticket[1], sh, [&](const double* t) {
+    st->rz[(it + 1) & 1] = t[0];
+    st->rr = t[1];
+    st->it = it + 1;
+    if (t[1] <= st->rtol2) st->done = 1;
+    else if (!(t[1] == t[1]) || !(st->pq > 0.0)) st->done = 3;  // breakdown
+    else if (it + 1 >= st->maxit) st->done = 2;
+  });
+}
+
+// r0 = b~ = -(rhs - K uD) on free dofs, x = 0, z = Dinv r0, p0 = 0.
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSystem T, CgVectors v,
+                                                          double rtol, int maxit) {
+  constexpr int NTH = 64 * NT;
+  __shared__ double rs[NTH];
+  __shared__ double sh[NT];
+  double prz = 0.0, prr = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+    const int64_t r = r0 + threadIdx.x;
+    double b = 0.0;
+    if (r < T.N) {
+      if (!T.dmask[r]) {
+        const int64_t f = r / NT;
+        double s = T.rhs[r];
+#pragma unroll
+        for (int sl = 0; sl < kSlots; ++sl) {
+          const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+#pragma unroll
+          for (int c = 0; c < NT; ++c) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
+        }
+        b = -s;
+      }
+      v.r[r] = b;
+      v.x[r] = 0.0;
+      v.p0[r] = 0.0;
+      v.p1[r] = 0.0;
+    }
+    rs[threadIdx.x] = b;
+    __syncthreads();
+    if (r < T.N) {
+      const int lb = (threadIdx.x / NT) * NT;
+      double zr = 0.0;
+#pragma unroll
+      for (int c = 0; c < NT; ++c) zr = fma(T.Dinv[(int64_t)c * T.N + r], rs[lb + c], zr);
+      v.z[r] = zr;
+      prz = fma(b, zr, prz);
+      prr = fma(b, b, prr);
+    }
+    __syncthreads();
+  }
+  const double a = block_sum<NT>(prz, sh);
+  const double bb = block_sum<NT>(prr, sh);
+  if (threadIdx.x == 0) {
+    v.part[blockIdx.x] = a;
+    v.part[gridDim.x + blockIdx.x] = bb;
+  }
+  CgState* st = v.st;
+  grid_finish<NT>(v.part, gridDim.x, 2, &st->ticket[1], sh, [&](const double* t) {
+    st->rz[0] = t[0];
+    st->rz[1] = t[0];
+    st->bb = t[1];
+    st->rr = t[1];
+    st->rtol2 = rtol * rtol * t[1];
+    st->it = 0;
+    st->maxit = maxit;
+    st->pq = 0.0;
+    st->done = (t[1] == 0.0) ? 1 : 0;
+  });
+}
+
+__global__ void cg_finish_kernel(TraceSystem T, const double* x, double* uhat) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= T.N) return;
+  uhat[r] = T.dmask[r] ? T.uD[r] : x[r];
+}
+
+// ||K uhat - rhs||^2 over the free dofs (the FP64 residual of the solve).
+template <int NT>
+__global__ __launch_bounds__(kCgThreads) void true_residual_kernel(DevPattern P, TraceSystem T,
+                                                                   const double* uhat, double* part,
+                                                                   unsigned* ticket, double* out) {
+  __shared__ double sh[kCgThreads / 64];
+  double acc2 = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    if (T.dmask[r]) continue;
+    const int64_t f = r / NT;
+    double s = -T.rhs[r];
+#pragma unroll
+    for (int sl = 0; sl < kSlots; ++sl) {
+      const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+#pragma unroll
+      for (int c = 0; c < NT; ++c) s = fma(T.vals[(int64_t)(sl * NT + c) * T.N + r], uhat[g + c], s);
+    }
+    acc2 = fma(s, s, acc2);
+  }
+  const double bs = block_sum<kCgThreads / 64>(acc2, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  grid_finish<kCgThreads / 64>(part, gridDim.x, 1, ticket, sh, [&](const double* t) { *out = t[0]; });
+}
+
+// -------------------------------------------------------------- recover
+template <int K>
+__global__ __launch_bounds__(256) void recover_kernel(DevMesh m, const double* __restrict__ Ke,
+                                                      const double* __restrict__ be,
+                                                      const double* __restrict__ uhat, double* qu) {
+  using D = Dims<K>;
+  constexpr int M = D::M, NT = D::NT, NTR = D::NTR;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nc * M) return;
+  const int64_t c = t / M;
+  const int i = (int)(t - c * M);
+  const double* ke = Ke + c * (M * NTR);
+  double s = 0.0;
+#pragma unroll
+  for (int lf = 0; lf < 3; ++lf) {
+    const int64_t f = m.cf[3 * c + lf];
+#pragma unroll
+    for (int b = 0; b < NT; ++b) s = fma(ke[(lf * NT + b) * M + i], uhat[f * NT + b], s);
+  }
+  qu[t] = s + be[t];
+}
+
+// ------------------------------------------------------------ errornorm
+template <int K>
+__global__ __launch_bounds__(256) void errornorm_kernel(DevMesh m, const double* __restrict__ Tg,
+                                                        TabLayoutDev L, const double* __restrict__ qu,
+                                                        int kind, double c0, double c1, double c2,
+                                                        double* part, unsigned* ticket, double* out) {
+  using D = Dims<K>;
+  constexpr int NU = D::NU, NQ = D::NQ, M = D::M;
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < m.nc;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    Geo g;
+    cell_geometry(m, c, g);
+    double el = 0.0;
+    for (int q = 0; q < L.nqp; ++q) {
+      double uh = 0.0;
+#pragma unroll
+      for (int i = 0; i < NU; ++i) uh = fma(qu[c * M + NQ + i], Tg[L.N + i * L.nqp + q], uh);
+      double x = 0.0, y = 0.0;
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        x += Tg[L.Mg + v * L.nqp + q] * g.x[2 * v];
+        y += Tg[L.Mg + v * L.nqp + q] * g.x[2 * v + 1];
+      }
+      const double ue = kind == 1 ? c0 + c1 * x + c2 * y : sin(kPi * x) * sin(kPi * y);
+      const double d = uh - ue;
+      el += d * d * (g.detJ * Tg[L.W + q]);
+    }
+    acc += el;
+  }
+  const double bs = block_sum<4>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  grid_finish<4>(part, gridDim.x, 1, ticket, sh, [&](const double* t) { *out = t[0]; });
+}
+
+int grid_for(int64_t n, int threads, int cap) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+template <int K>
+int condense_launch(const CondenseArgs& a, bool blocks, hipStream_t s) {
+  using D = Dims<K>;
+  const int64_t n = blocks ? a.count : a.mesh.nc;
+  const int64_t per_block = 4 * D::EPW;
+  const int grid = grid_for(n, (int)per_block, 8192);
+  const size_t lds = sizeof(double) * a.lay.total;
+  if (blocks)
+    hipLaunchKernelGGL((condense_kernel<K, true>), dim3(grid), dim3(256), lds, s, a);
+  else
+    hipLaunchKernelGGL((condense_kernel<K, false>), dim3(grid), dim3(256), lds, s, a);
+  return (int)hipGetLastError();
+}
+
+}  // namespace
+
+int cg_blocks(int64_t N, int nt) {
+  (void)nt;
+  return grid_for(N, kCgThreads, 4096);
+}
+
+int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s) {
+  const int g = grid_for(m.nf, 256, 1 << 30);
+  hipLaunchKernelGGL(pattern_kernel, dim3(g), dim3(256), 0, s, m, P);
+  hipLaunchKernelGGL(tslot_kernel, dim3(g), dim3(256), 0, s, m.nf, P);
+  return (int)hipGetLastError();
+}
+
+int launch_condense(int k, const CondenseArgs& a, bool blocks, hipStream_t s) {
+  switch (k) {
+    case 1: return condense_launch<1>(a, blocks, s);
+    case 2: return condense_launch<2>(a, blocks, s);
+    case 3: return condense_launch<3>(a, blocks, s);
+    case 4: return condense_launch<4>(a, blocks, s);
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
+#define HDG_NT_DISPATCH(nt, CALL) \
+  switch (nt) {                   \
+    case 2: CALL(2); break;       \
+    case 3: CALL(3); break;       \
+    case 4: CALL(4); break;       \
+    case 5: CALL(5); break;       \
+    default: return (int)hipErrorInvalidValue; \
+  }
+
+int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
+                hipStream_t s) {
+  const int gr = grid_for(T.N, 256, 1 << 30);
+#define FILL(NT) hipLaunchKernelGGL(fill_kernel<NT>, dim3(gr), dim3(256), 0, s, P, Ate, bte, T);
+  HDG_NT_DISPATCH(T.nt, FILL)
+#undef FILL
+  return (int)hipGetLastError();
+}
+
+int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s) {
+  const int gf = grid_for(T.nf, 256, 1 << 30);
+#define JAC(NT) hipLaunchKernelGGL(jacobi_kernel<NT>, dim3(gf), dim3(256), 0, s, P, T);
+  HDG_NT_DISPATCH(T.nt, JAC)
+#undef JAC
+  return (int)hipGetLastError();
+}
+
+int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, double rtol, int maxit,
+                   hipStream_t s) {
+#define INIT(NT) \
+  hipLaunchKernelGGL(cg_init_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v, rtol, maxit);
+  HDG_NT_DISPATCH(T.nt, INIT)
+#undef INIT
+  return (int)hipGetLastError();
+}
+
+int launch_cg_spmv_only(const DevPattern& P, const TraceSystem& T, CgVectors& v, int force,
+                        hipStream_t s) {
+#define SPMV(NT) \
+  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, force);
+  HDG_NT_DISPATCH(T.nt, SPMV)
+#undef SPMV
+  return (int)hipGetLastError();
+}
+
+int launch_cg_update_only(const TraceSystem& T, CgVectors& v, int force, hipStream_t s) {
+#define UPD(NT) \
+  hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, force);
+  HDG_NT_DISPATCH(T.nt, UPD)
+#undef UPD
+  return (int)hipGetLastError();
+}
+
+int launch_cg_iteration(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s) {
+  int e = launch_cg_spmv_only(P, T, v, 0, s);
+  if (e) return e;
+  return launch_cg_update_only(T, v, 0, s);
+}
+
+int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s) {
+  hipLaunchKernelGGL(cg_finish_kernel, dim3(grid_for(T.N, 256, 1 << 30)), dim3(256), 0, s, T, v.x, uhat);
+  return (int)hipGetLastError();
+}
+
+int launch_true_residual(const DevPattern& P, const TraceSystem& T, const double* uhat, double* out2,
+                         CgVectors& v, hipStream_t s) {
+#define TR(NT)                                                                            \
+  hipLaunchKernelGGL(true_residual_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, uhat, \
+                     v.part, &v.st->ticket[2], out2);
+  HDG_NT_DISPATCH(T.nt, TR)
+#undef TR
+  return (int)hipGetLastError();
+}
+
+int launch_recover(int k, const DevMesh& m, const double* Ke, const double* be, const double* uhat,
+                   double* qu, hipStream_t s) {
+#define REC(KK)                                                                             \
+  {                                                                                         \
+    const int64_t n = m.nc * Dims<KK>::M;                                                   \
+    hipLaunchKernelGGL(recover_kernel<KK>, dim3(grid_for(n, 256, 1 << 30)), dim3(256), 0, s, m, Ke, \
+                       be, uhat, qu);                                                       \
+  }
+  switch (k) {
+    case 1: REC(1); break;
+    case 2: REC(2); break;
+    case 3: REC(3); break;
+    case 4: REC(4); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+#undef REC
+  return (int)hipGetLastError();
+}
+
+int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k,
+                     const double* qu, int uex_kind, const double* par3, double* part, double* result,
+                     unsigned* ticket, hipStream_t s) {
+  const int g = grid_for(m.nc, 256, 2048);
+#define ERR(KK)                                                                                    \
+  hipLaunchKernelGGL(errornorm_kernel<KK>, dim3(g), dim3(256), 0, s, m, tables, lay, qu, uex_kind, \
+                     par3[0], par3[1], par3[2], part, ticket, result);
+  switch (k) {
+    case 1: ERR(1); break;
+    case 2: ERR(2); break;
+    case 3: ERR(3); break;
+    case 4: ERR(4); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+#undef ERR
+  return (int)hipGetLastError();
+}
+
+}  // namespace hdg

@@ -1,0 +1,203 @@
+/*
+ * hdg.h -- C ABI of libhdg, the MI355X HDG Poisson pipeline.
+ *
+ * The library replaces the hot path of HDiscontinuousGalerkin.jl's HDG
+ * driver: the per-cell loop of `doassemble` (examples/poisson2D_HDG.jl:58-186),
+ * `apply!` + `K \ b` (:189-195, src/boundary.jl:121-175), `get_uσ!`
+ * (:197-212) and `errornorm` (src/DiscreteFunctions.jl:97-120).  A foreign
+ * caller (Julia `ccall`, Python `ctypes`) binds these symbols directly: all
+ * arguments are plain C scalars, pointers and sizes; no C++ types cross the
+ * boundary and nothing allocated by the library is returned to the caller.
+ *
+ * Layouts are the reference's in-memory layouts, so a Julia caller passes
+ * `pointer(mesh.nodes)` etc. without copies:
+ *   nodes  xy[2*n + c]                       == Vector{Node{2,Float64}}
+ *   cells  cells[6*c + 0..2] node ids, cells[6*c + 3..5] face ids, 1-based
+ *                                            == Vector{Cell{2,3,3}}
+ *   faces  faces[f + nf*j], j = v1 v2 cell1 cell2 (0 = none), 1-based,
+ *          column-major                      == PolygonalMesh.faces
+ *   q, u   ncells x nq / ncells x nu column-major == TrialFunction.m_values
+ *
+ * Ownership: the caller owns every host buffer; the library reads or writes
+ * them only during a call.  The context owns all device memory until
+ * hdg_destroy.  Calls on one context are serialised by the caller; distinct
+ * contexts may be used from distinct host threads.  Every entry point
+ * returns an hdg_status (0 = HDG_OK); hdg_last_error gives the message.
+ */
+#ifndef HDG_H
+#define HDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HDG_OK = 0,
+  HDG_ERR_ARG = 1,            /* ArgumentError: bad size, order, pointer */
+  HDG_ERR_NEG_JACOBIAN = 2,   /* det(J) <= 0, src/ScalarFunctionSpaces.jl:110 */
+  HDG_ERR_QUAD = 3,           /* quadrature order unavailable, src/quadrature.jl:24 */
+  HDG_ERR_LOCAL_SINGULAR = 4, /* local (q,u) matrix singular (the reference
+                                 returns garbage silently, e.g. k=3 with the
+                                 default quad_degree) */
+  HDG_ERR_NOT_CONVERGED = 5,  /* trace solve hit maxit */
+  HDG_ERR_HBM_BUDGET = 6,     /* device allocation beyond the context budget */
+  HDG_ERR_HIP = 7,            /* HIP runtime failure */
+  HDG_ERR_STATE = 8,          /* entry point called out of order */
+  HDG_ERR_UNSUPPORTED = 9,    /* e.g. k outside 1..4 */
+  HDG_ERR_BOUNDARY = 10       /* face on one cell but not on the box,
+                                 src/generate_mesh.jl:80 */
+} hdg_status;
+
+typedef struct hdg_ctx hdg_ctx;
+
+/* Precision modes of the trace solve. */
+enum { HDG_FP64 = 0, HDG_MIXED = 1 };
+
+/* Sources.  HDG_SRC_REFERENCE is the reference's f = 2 pi^2 sin(pi x) sin(pi y)
+   (examples/poisson2D_HDG.jl:55).  HDG_SRC_VALUES takes f already evaluated
+   at the cell quadrature points (for arbitrary Julia closures). */
+enum {
+  HDG_SRC_REFERENCE = 0,
+  HDG_SRC_KAPPA_MANUFACTURED = 1, /* f = -div(kappa grad(sin(pi x) sin(pi y))) */
+  HDG_SRC_SINE_SERIES = 2,        /* f = sum c_pq sin(p pi x) sin(q pi y), p,q=1..4 */
+  HDG_SRC_VALUES = 3              /* f_qp: ncells x nqp, column-major */
+};
+
+/* Exact solutions known to hdg_errornorm. */
+enum { HDG_UEX_SINSIN = 0, HDG_UEX_LINEAR = 1 };
+
+typedef struct {
+  int32_t k;           /* Dubiner order of q, u and Legendre order of the trace, 1..4 */
+  int32_t quad_degree; /* quad_degree keyword; 0 selects the reference default k+1 */
+  double tau;          /* stabilisation (poisson2D_HDG.jl:58) */
+  int32_t precision;   /* HDG_FP64 or HDG_MIXED */
+  double rtol;         /* trace solve: stop at ||r|| <= rtol ||b|| (0 -> 1e-12) */
+  int32_t maxit;       /* trace solve iteration cap (0 -> 200000) */
+  int32_t schedule;    /* kernel schedule selector, 0 = default */
+} hdg_options;
+
+typedef struct {
+  int32_t kind;        /* HDG_SRC_* */
+  int32_t kappa_on;    /* 1: kappa = 1 + 0.5 sin(ka x) cos(kb y) weights (q,v) */
+  double ka, kb;
+  double coef[16];     /* sine-series coefficients c_pq, p-major */
+  const double* f_qp;  /* HDG_SRC_VALUES only; read during hdg_set_source */
+} hdg_source;
+
+typedef struct {
+  int64_t n_nodes, n_cells, n_faces;
+  int64_t n_dofs;      /* trace dofs = n_faces * nt */
+  int64_t nnz;         /* stored entries of K (sparse(I,J,V) semantics) */
+  int32_t nu, nq, nt, m, ntr, nqp, nfq;
+} hdg_sizes;
+
+typedef struct {
+  int32_t iters;       /* PCG iterations */
+  int32_t refinements; /* outer FP64 refinement steps (HDG_MIXED) */
+  double relres;       /* final recursive ||r|| / ||b|| */
+  double true_relres;  /* ||b - S x|| / ||b|| recomputed in FP64 */
+  double seconds;      /* device time of the solve */
+} hdg_solve_info;
+
+/* ---- context ---------------------------------------------------------- */
+/* A context on HIP device `device`; hbm_budget bytes caps its device
+   memory (0 = no cap). */
+int hdg_create(hdg_ctx** ctx, int device, size_t hbm_budget);
+void hdg_destroy(hdg_ctx* ctx);
+const char* hdg_last_error(const hdg_ctx* ctx);
+/* Bytes of device memory the context holds now. */
+int hdg_memory_used(const hdg_ctx* ctx, size_t* bytes);
+
+/* ---- mesh: replaces rectangle_mesh (src/generate_mesh.jl:267-309) ------ */
+/* Face count of rectangle_mesh(TriangleCell, (nx, ny), ...). */
+int64_t hdg_rectangle_nfaces(int64_t nx, int64_t ny);
+/* rectangle_mesh(TriangleCell, (nx, ny), LL, UR) into caller buffers
+   xy[2*(nx+1)*(ny+1)], cells[6*2*nx*ny], faces[4*nf] (column-major), and
+   face_side[nf] (0 interior, 1 bottom, 2 right, 3 top, 4 left: the
+   facesets of generate_mesh.jl:60-89).  jitter > 0 moves interior nodes by
+   up to jitter*h per axis after generation (splitmix64 stream `seed`). */
+int hdg_rectangle_mesh(int64_t nx, int64_t ny, double x0, double y0, double x1,
+                       double y1, double jitter, uint64_t seed, double* xy,
+                       int64_t* cells, int64_t* faces, int8_t* face_side);
+
+/* ---- problem set-up ----------------------------------------------------- */
+/* Uploads a PolygonalMesh (src/mesh.jl:43-49) and builds the face-level
+   sparsity pattern of the trace matrix on the device. */
+int hdg_set_mesh(hdg_ctx* ctx, int64_t n_nodes, const double* xy,
+                 int64_t n_cells, const int64_t* cells, int64_t n_faces,
+                 const int64_t* faces);
+/* ScalarFunctionSpace / VectorFunctionSpace / ScalarTraceFunctionSpace
+   construction (src/ScalarFunctionSpaces.jl:24-29, VectorFunctionSpaces.jl:
+   171-179, TraceFunctionSpaces.jl:235-252): reference tables for (k, quad
+   degree) go to the device. */
+int hdg_setup(hdg_ctx* ctx, const hdg_options* opt);
+int hdg_set_source(hdg_ctx* ctx, const hdg_source* src);
+/* Dirichlet(u_hat, mesh, faceset, g) (src/boundary.jl:11-42): prescribed
+   trace dofs (1-based) and values.  n = 0 clears. */
+int hdg_set_dirichlet(hdg_ctx* ctx, int64_t n, const int64_t* dofs,
+                      const double* vals);
+int hdg_query_sizes(const hdg_ctx* ctx, hdg_sizes* sizes);
+
+/* ---- the pipeline -------------------------------------------------------- */
+/* doassemble (poisson2D_HDG.jl:58-186) on the device: element blocks, static
+   condensation, trace matrix K and rhs. */
+int hdg_assemble_condense(hdg_ctx* ctx);
+/* end_assemble's SparseMatrixCSC{Float64,Int} of K (1-based colptr[n+1],
+   rowval[nnz], nzval[nnz]) and the assembled rhs[n]; any pointer may be
+   NULL. */
+int hdg_get_system(hdg_ctx* ctx, int64_t* colptr, int64_t* rowval,
+                   double* nzval, double* rhs);
+/* K_element / b_element of doassemble: Ke[c] m x ntr column-major, stored
+   cell after cell; be[c] of length m.  Either pointer may be NULL. */
+int hdg_get_local_ops(hdg_ctx* ctx, double* Ke, double* be);
+/* Condensed element blocks Ate[c] (ntr x ntr column-major) and bte[c]. */
+int hdg_get_condensed(hdg_ctx* ctx, double* Ate, double* bte);
+/* Element blocks A, B, C, E, F, H, be of cells [first, first+count) as
+   built inside doassemble (poisson2D_HDG.jl:77-153), each column-major per
+   cell, cell after cell.  Any pointer may be NULL. */
+int hdg_get_element_blocks(hdg_ctx* ctx, int64_t first, int64_t count,
+                           double* A, double* B, double* C, double* E,
+                           double* F, double* H, double* be);
+/* apply!(K, b, dbc) + K \ b (poisson2D_HDG.jl:194-195): the trace solve with
+   Dirichlet dofs eliminated.  uhat_out[n_dofs] may be NULL (the solution
+   stays on the device for hdg_recover). */
+int hdg_solve(hdg_ctx* ctx, double* uhat_out, hdg_solve_info* info);
+/* Replaces the device trace solution by a caller's uhat[n_dofs] (e.g. the
+   result of the caller's own K \ b), so that hdg_recover applies get_uσ! to
+   it.  Requires hdg_assemble_condense. */
+int hdg_set_trace(hdg_ctx* ctx, const double* uhat);
+/* get_uσ! (poisson2D_HDG.jl:197-212): q (ncells x nq), u (ncells x nu),
+   uhat_cell (ncells x nt x 3), all column-major; any may be NULL. */
+int hdg_recover(hdg_ctx* ctx, double* q, double* u, double* uhat_cell);
+/* errornorm(u_h, u_ex) (DiscreteFunctions.jl:97-120): squared L2 error of
+   the recovered u against HDG_UEX_SINSIN or HDG_UEX_LINEAR (par c0,c1,c2). */
+int hdg_errornorm(hdg_ctx* ctx, int uex_kind, const double* uex_par,
+                  double* sq_l2);
+/* Fused assemble + condense + solve + recover with everything kept on the
+   device; the timed unit of the benchmark. */
+int hdg_run(hdg_ctx* ctx, hdg_solve_info* info);
+
+/* ---- reference-element numerics (host) ----------------------------------- */
+/* value / gradient of the orthonormal Dubiner basis j = 1..nu at (r, s)
+   (src/basis.jl:65-123): val[nu], grad[2*nu]. */
+int hdg_dubiner(int k, double r, double s, double* val, double* grad);
+/* QuadratureRule{2,RefTetrahedron}(DefaultQuad(), order) (quadrature.jl:
+   17-26) and the 1-D rule on [0,1] (:29-39).  Returns the point count, or
+   -HDG_ERR_QUAD; pts holds 2 (or 1) coordinates per point; cap = capacity
+   in points. */
+int hdg_quadrature(int dim, int order, int cap, double* pts, double* w);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Average device duration (microseconds) of one launch of an internal
+   kernel, timed with HIP events around `reps` back-to-back launches on the
+   context stream with the data currently resident.  which: 0 = trace SpMV
+   (the dominant PCG kernel), 1 = PCG update, 2 = condensation. */
+int hdg_time_kernel(hdg_ctx* ctx, int which, int reps, double* avg_us);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDG_H */
