@@ -57,7 +57,6 @@ struct hdg_ctx {
   CgState* h_st = nullptr;  // pinned, 2 slots
   double* d_scal = nullptr; // [0] errornorm, [1] true residual^2
   double* d_part2 = nullptr;
-  unsigned* d_ticket2 = nullptr;
 
   bool assembled = false, solved = false, recovered = false;
 };
@@ -204,6 +203,14 @@ int assemble_impl(hdg_ctx* c) {
 
 constexpr int kChunk = 64;  // PCG iterations enqueued between convergence checks
 
+int enqueue_chunk(hdg_ctx* c, int first, double rtol2, int maxit) {
+  for (int i = first; i < first + kChunk; ++i) {
+    LAUNCHCHK(c, launch_cg_spmv(c->pat, c->T, c->cg, i, rtol2, maxit, 0, c->stream));
+    LAUNCHCHK(c, launch_cg_update(c->T, c->cg, i, 0, c->stream));
+  }
+  return HDG_OK;
+}
+
 int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
   if (c->jacobi_dirty) {
@@ -213,18 +220,21 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   const double rtol = c->opt.rtol > 0 ? c->opt.rtol : 1e-12;
   const int maxit = c->opt.maxit > 0 ? c->opt.maxit : 200000;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, rtol, maxit, c->stream));
-  // keep one chunk in flight while the previous chunk's state is read back
-  int slot = 0;
-  for (int i = 0; i < kChunk; ++i) LAUNCHCHK(c, launch_cg_iteration(c->pat, c->T, c->cg, c->stream));
-  HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+  LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
+  // one chunk stays in flight while the previous chunk's state is read
+  // back; launches after convergence return at once
   hipEvent_t evs[2];
   HIPCHK(c, hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   HIPCHK(c, hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+  int slot = 0, next = 1;
+  TRY(enqueue_chunk(c, next, rtol * rtol, maxit));
+  next += kChunk;
+  HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(evs[slot], c->stream));
   for (;;) {
     const int nxt = slot ^ 1;
-    for (int i = 0; i < kChunk; ++i) LAUNCHCHK(c, launch_cg_iteration(c->pat, c->T, c->cg, c->stream));
+    TRY(enqueue_chunk(c, next, rtol * rtol, maxit));
+    next += kChunk;
     HIPCHK(c, hipMemcpyAsync(&c->h_st[nxt], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
     HIPCHK(c, hipEventSynchronize(evs[slot]));
@@ -257,7 +267,7 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     return fail(c, HDG_ERR_NOT_CONVERGED, "trace solve broke down at iteration %d (non-positive curvature or NaN)",
                 st.it);
   if (st.done == 2)
-    return fail(c, HDG_ERR_NOT_CONVERGED, "trace solve reached maxit=%d at relres %.3e", st.maxit,
+    return fail(c, HDG_ERR_NOT_CONVERGED, "trace solve reached maxit=%d at relres %.3e", maxit,
                 st.bb > 0 ? std::sqrt(st.rr / st.bb) : 0.0);
   return HDG_OK;
 }
@@ -290,13 +300,11 @@ int hdg_create(hdg_ctx** out, int device, size_t hbm_budget) {
     delete c;
     return HDG_ERR_HIP;
   }
-  if (dalloc(c, &c->d_err, 1) || dalloc(c, &c->cg.st, 1) || dalloc(c, &c->d_scal, 4) ||
-      dalloc(c, &c->d_ticket2, 4)) {
+  if (dalloc(c, &c->d_err, 1) || dalloc(c, &c->cg.st, 1) || dalloc(c, &c->d_scal, 4)) {
     delete c;
     return HDG_ERR_HIP;
   }
   (void)hipMemset(c->cg.st, 0, sizeof(CgState));
-  (void)hipMemset(c->d_ticket2, 0, 4 * sizeof(unsigned));
   *out = c;
   return HDG_OK;
 }
@@ -309,7 +317,6 @@ void hdg_destroy(hdg_ctx* c) {
   dfree(c, c->d_err);
   dfree(c, c->cg.st);
   dfree(c, c->d_scal);
-  dfree(c, c->d_ticket2);
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -440,7 +447,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   TRY(dalloc(c, &c->cg.p0, N));
   TRY(dalloc(c, &c->cg.p1, N));
   TRY(dalloc(c, &c->cg.q, N));
-  TRY(dalloc(c, &c->cg.part, 2 * (size_t)c->cg.nblocks + 64));
+  TRY(dalloc(c, &c->cg.part, 4 * (size_t)c->cg.nblocks + 64));
   TRY(dalloc(c, &c->d_uhat, N));
   TRY(dalloc(c, &c->d_part2, 4096));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -582,7 +589,7 @@ int hdg_errornorm(hdg_ctx* c, int kind, const double* par, double* sq) {
   if (!c->recovered) TRY(recover_impl(c));
   const double p3[3] = {par ? par[0] : 0.0, par ? par[1] : 0.0, par ? par[2] : 0.0};
   LAUNCHCHK(c, launch_errornorm(c->mesh, c->d_tab, c->lay, c->k, c->d_qu, kind, p3, c->d_part2, c->d_scal,
-                                c->d_ticket2, c->stream));
+                                c->stream));
   HIPCHK(c, hipMemcpyAsync(sq, c->d_scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return HDG_OK;
@@ -737,7 +744,7 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   TRY(select_device(c));
   // make sure the solver state exists (pq, rz) so the kernels divide by
   // finite numbers; timing runs leave x and r untouched
-  if (!c->solved) LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, 1e-12, 1, c->stream));
+  if (!c->solved) LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   CondenseArgs a;
   a.mesh = c->mesh;
   a.tables = c->d_tab;
@@ -751,8 +758,8 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   a.be = c->d_be;
   a.err = c->d_err;
   auto once = [&]() -> int {
-    if (which == 0) return launch_cg_spmv_only(c->pat, c->T, c->cg, 1, c->stream);
-    if (which == 1) return launch_cg_update_only(c->T, c->cg, 1, c->stream);
+    if (which == 0) return launch_cg_spmv(c->pat, c->T, c->cg, 2, 0.0, 0, 1, c->stream);
+    if (which == 1) return launch_cg_update(c->T, c->cg, 2, 1, c->stream);
     return launch_condense(c->k, a, false, c->stream);
   };
   LAUNCHCHK(c, once());  // warm-up

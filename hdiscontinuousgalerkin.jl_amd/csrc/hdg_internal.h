@@ -64,18 +64,18 @@ struct CondenseArgs {
          *bev = nullptr;
 };
 
-// PCG scalar state, resident on the device.
+// PCG scalar state, resident on the device; written by block 0 of the
+// kernels that own each field, read by later launches.
 struct CgState {
-  double rz[2];    // (r, z) of the current and previous iterate, by parity
-  double pq;       // (p, q) of the current iteration
-  double rr;       // (r, r) after the current update
-  double bb;       // (b, b)
-  double rtol2;    // rtol^2 * bb
-  int it;          // completed iterations
+  double rz[2];  // (r, z) by iteration parity
+  double pq;
+  double rr;     // (r, r) at the last convergence test
+  double bb;     // (b~, b~)
+  double pad0;
+  int it;        // completed iterations when the solve stopped
   int maxit;
-  int done;        // 1 converged, 2 iteration cap, 3 breakdown
-  int pad;
-  unsigned ticket[4];
+  int done;      // 0 running, 1 converged, 2 iteration cap, 3 breakdown
+  int pad1;
 };
 
 struct TraceSystem {
@@ -89,7 +89,8 @@ struct TraceSystem {
 };
 
 struct CgVectors {
-  double *x, *r, *z, *p0, *p1, *q, *part;
+  double *x, *r, *z, *p0, *p1, *q;
+  double* part;  // 4 * nblocks partial sums
   CgState* st;
   int nblocks;
 };
@@ -100,14 +101,12 @@ int launch_condense(int k, const CondenseArgs& a, bool blocks_mode, hipStream_t 
 int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s);
 int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);
-int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, double rtol, int maxit,
-                   hipStream_t s);
-int launch_cg_iteration(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s);
-// force = 1 runs the kernel regardless of the convergence flag and leaves
+int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s);
+// force = 1 runs the kernel regardless of the convergence state and leaves
 // x, r untouched (timing only; the CG state is not usable afterwards)
-int launch_cg_spmv_only(const DevPattern& P, const TraceSystem& T, CgVectors& v, int force,
-                        hipStream_t s);
-int launch_cg_update_only(const TraceSystem& T, CgVectors& v, int force, hipStream_t s);
+int launch_cg_spmv(const DevPattern& P, const TraceSystem& T, CgVectors& v, int i, double rtol2, int maxit,
+                   int force, hipStream_t s);
+int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipStream_t s);
 int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s);
 int launch_true_residual(const DevPattern& P, const TraceSystem& T, const double* uhat, double* out2,
                          CgVectors& v, hipStream_t s);
@@ -115,7 +114,7 @@ int launch_recover(int k, const DevMesh& m, const double* Ke, const double* be, 
                    double* qu, hipStream_t s);
 int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k,
                      const double* qu, int uex_kind, const double* par3, double* part,
-                     double* result, unsigned* ticket, hipStream_t s);
+                     double* result, hipStream_t s);
 int cg_blocks(int64_t N, int nt);
 
 }  // namespace hdg

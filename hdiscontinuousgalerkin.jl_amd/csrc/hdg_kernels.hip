@@ -11,7 +11,7 @@
 //              each entry has at most two contributions, summed in order)
 //   jacobi     inverse of each face's diagonal block (block-Jacobi)
 //   cg_*       preconditioned CG on S = -K_II, two kernels per iteration,
-//              deterministic reductions (last-block tickets)
+//              deterministic reductions without atomics or fences
 //   recover    [q;u]_e = K_e uhat_e + b_e (poisson2D_HDG.jl:197-212)
 //   errornorm  squared L2 error (src/DiscreteFunctions.jl:97-120)
 #include <hip/hip_runtime.h>
@@ -43,29 +43,6 @@ __device__ inline double block_sum(double v, double* sh) {
     for (int i = 0; i < NW; ++i) s += sh[i];
   __syncthreads();
   return s;
-}
-
-// Deterministic grid reduction: every block stores its partial; the block
-// that arrives last sums all partials in index order and calls `fin`.
-template <int NW, typename Fin>
-__device__ inline void grid_finish(const double* part_in, int nparts, int nvals, unsigned* ticket,
-                                   double* sh, Fin fin) {
-  __shared__ bool last;
-  __threadfence();
-  if (threadIdx.x == 0) last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double tot[2] = {0.0, 0.0};
-  for (int v = 0; v < nvals; ++v) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part_in[(size_t)v * nparts + i];
-    tot[v] = block_sum<NW>(s, sh);
-  }
-  if (threadIdx.x == 0) {
-    *ticket = 0u;
-    fin(tot);
-  }
 }
 
 // ------------------------------------------------------------- sources
@@ -540,26 +517,73 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
 }
 
 // ------------------------------------------------------------------- CG
-// S = -K on the non-Dirichlet dofs.  Iteration i (it = i-1 completed):
-//   spmv:   beta = rz[it]/rz[it-1] (0 at it = 0); p_new = z + beta p_old;
-//           q = S p_new; (p_new, q)
-//   update: alpha = rz/pq; x += alpha p; r -= alpha q; z = Dinv r;
-//           (r, z), (r, r); convergence test; it += 1
+// Preconditioned CG on S = -K restricted to the non-Dirichlet dofs, two
+// kernels per iteration and no atomics or device-wide fences: each kernel
+// writes one partial sum per block, and the kernel that consumes a scalar
+// re-reduces those partials itself, in a fixed order, so every block gets
+// the same bits.  Iteration i (i >= 1), host-numbered:
+//   spmv(i):   rz_i = (r, z) and rr_i = (r, r) from update(i-1)'s partials;
+//              stop if rr_i <= rtol^2 bb or i-1 >= maxit; beta = rz_i/rz_{i-1}
+//              (0 for i = 1); p_i = z + beta p_{i-1}; q = S p_i; (p_i, q)
+//   update(i): alpha = rz_i/(p_i, q); x += alpha p_i; r -= alpha q;
+//              z = Dinv r; partials of (r, z) and (r, r)
+// The init kernel plays update(0): r = b~, z = Dinv r, x = p = 0.
+// Partials: part[0:nb) (p,q); part[nb(1+par):...) (r,z) by parity; part[3nb:4nb) (r,r).
 constexpr int kCgThreads = 256;
 
 __device__ inline double pnew_of(double z, double p, double beta) { return fma(beta, p, z); }
 
+// Sum of n partials in a fixed order, returned to every thread of the block.
+template <int NW>
+__device__ inline double sum_partials(const double* p, int n, double* sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  s = wave_sum(s);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = s;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
 template <int NT>
 __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceSystem T, CgVectors v,
-                                                             int force) {
-  __shared__ double sh[kCgThreads / 64];
-  const CgState* st = v.st;
+                                                             int i, double rtol2, int maxit, int force) {
+  constexpr int NW = kCgThreads / 64;
+  __shared__ double sh[NW];
+  CgState* st = v.st;
   if (!force && st->done) return;
-  const int it = st->it;
-  const double beta = it == 0 ? 0.0 : st->rz[it & 1] / st->rz[(it + 1) & 1];
-  const double* __restrict__ pold = (it & 1) ? v.p1 : v.p0;
-  double* __restrict__ pnw = (it & 1) ? v.p0 : v.p1;
+  const int nb = v.nblocks;
+  const double rz = sum_partials<NW>(v.part + (size_t)nb * (1 + (i & 1)), nb, sh);
+  const double rr = sum_partials<NW>(v.part + (size_t)nb * 3, nb, sh);
+  const double bb = (i == 1) ? rr : st->bb;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!force) {
+    const bool conv = rr <= rtol2 * bb;
+    if (conv || i - 1 >= maxit) {
+      if (lead) {
+        st->done = conv ? 1 : 2;
+        st->it = i - 1;
+        st->rr = rr;
+        st->bb = bb;
+      }
+      return;
+    }
+    if (lead) {
+      st->rz[i & 1] = rz;
+      st->it = i - 1;
+      st->rr = rr;
+      if (i == 1) st->bb = rr;
+    }
+  }
+  const double beta = (i == 1) ? 0.0 : rz / st->rz[(i - 1) & 1];
+  const double* __restrict__ pold = (i & 1) ? v.p0 : v.p1;
+  double* __restrict__ pnw = (i & 1) ? v.p1 : v.p0;
   const double* __restrict__ z = v.z;
+  double* __restrict__ qo = v.q;
   double part = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N;
        r += (int64_t)gridDim.x * blockDim.x) {
@@ -576,27 +600,32 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
     const double q = T.dmask[r] ? 0.0 : -acc;
     const double pn = pnew_of(z[r], pold[r], beta);
     pnw[r] = pn;
-    v.q[r] = q;
+    qo[r] = q;
     part = fma(pn, q, part);
   }
-  const double bs = block_sum<kCgThreads / 64>(part, sh);
+  const double bs = block_sum<NW>(part, sh);
   if (threadIdx.x == 0) v.part[blockIdx.x] = bs;
-  CgState* stw = v.st;
-  grid_finish<kCgThreads / 64>(v.part, gridDim.x, 1, &stw->ticket[0], sh,
-                               [&](const double* t) { stw->pq = t[0]; });
 }
 
 template <int NT>
-__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, int force) {
+__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, int i, int force) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
   CgState* st = v.st;
   if (!force && st->done) return;
-  const int it = st->it;
-  const double rz = st->rz[it & 1];
-  const double alpha = rz / st->pq;
-  const double* __restrict__ p = (it & 1) ? v.p0 : v.p1;
+  const int nb = v.nblocks;
+  const double pq = sum_partials<NT>(v.part, nb, sh);
+  const double rz = st->rz[i & 1];
+  if (!force && !(pq > 0.0)) {  // loss of positive curvature or NaN
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 3;
+      st->it = i - 1;
+    }
+    return;
+  }
+  const double alpha = rz / pq;
+  const double* __restrict__ p = (i & 1) ? v.p1 : v.p0;
   double prz = 0.0, prr = 0.0;
   for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
     const int64_t r = r0 + threadIdx.x;
@@ -624,25 +653,16 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
   }
   const double a = block_sum<NT>(prz, sh);
   const double b = block_sum<NT>(prr, sh);
-  if (threadIdx.x == 0) {
-    v.part[blockIdx.x] = a;
-    v.part[gridDim.x + blockIdx.x] = b;
+  if (threadIdx.x == 0 && !force) {
+    v.part[(size_t)nb * (1 + ((i + 1) & 1)) + blockIdx.x] = a;
+    v.part[(size_t)nb * 3 + blockIdx.x] = b;
   }
-  if (force) return;
-  grid_finish<NT>(v.part, gridDim.x, 2, &st->ticket[1], sh, [&](const double* t) {
-    st->rz[(it + 1) & 1] = t[0];
-    st->rr = t[1];
-    st->it = it + 1;
-    if (t[1] <= st->rtol2) st->done = 1;
-    else if (!(t[1] == t[1]) || !(st->pq > 0.0)) st->done = 3;  // breakdown
-    else if (it + 1 >= st->maxit) st->done = 2;
-  });
 }
 
-// r0 = b~ = -(rhs - K uD) on free dofs, x = 0, z = Dinv r0, p0 = 0.
+// update(0): r0 = b~ = -(rhs - K uD) on free dofs, x = 0, z = Dinv r0,
+// p = 0; partials of (r0, z0) and (r0, r0).
 template <int NT>
-__global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSystem T, CgVectors v,
-                                                          double rtol, int maxit) {
+__global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSystem T, CgVectors v) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
@@ -683,21 +703,22 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
   const double a = block_sum<NT>(prz, sh);
   const double bb = block_sum<NT>(prr, sh);
   if (threadIdx.x == 0) {
-    v.part[blockIdx.x] = a;
-    v.part[gridDim.x + blockIdx.x] = bb;
+    v.part[(size_t)v.nblocks * 2 + blockIdx.x] = a;  // parity of update(0) + 1 = 1
+    v.part[(size_t)v.nblocks * 3 + blockIdx.x] = bb;
   }
-  CgState* st = v.st;
-  grid_finish<NT>(v.part, gridDim.x, 2, &st->ticket[1], sh, [&](const double* t) {
-    st->rz[0] = t[0];
-    st->rz[1] = t[0];
-    st->bb = t[1];
-    st->rr = t[1];
-    st->rtol2 = rtol * rtol * t[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState* st = v.st;
+    st->done = 0;
     st->it = 0;
-    st->maxit = maxit;
     st->pq = 0.0;
-    st->done = (t[1] == 0.0) ? 1 : 0;
-  });
+  }
+}
+
+// Final sum of n partials by one block (a separate launch, so no fences).
+__global__ __launch_bounds__(256) void reduce_kernel(const double* part, int n, double* out) {
+  __shared__ double sh[4];
+  const double s = sum_partials<4>(part, n, sh);
+  if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void cg_finish_kernel(TraceSystem T, const double* x, double* uhat) {
@@ -709,8 +730,7 @@ __global__ void cg_finish_kernel(TraceSystem T, const double* x, double* uhat) {
 // ||K uhat - rhs||^2 over the free dofs (the FP64 residual of the solve).
 template <int NT>
 __global__ __launch_bounds__(kCgThreads) void true_residual_kernel(DevPattern P, TraceSystem T,
-                                                                   const double* uhat, double* part,
-                                                                   unsigned* ticket, double* out) {
+                                                                   const double* uhat, double* part) {
   __shared__ double sh[kCgThreads / 64];
   double acc2 = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N;
@@ -728,7 +748,6 @@ __global__ __launch_bounds__(kCgThreads) void true_residual_kernel(DevPattern P,
   }
   const double bs = block_sum<kCgThreads / 64>(acc2, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  grid_finish<kCgThreads / 64>(part, gridDim.x, 1, ticket, sh, [&](const double* t) { *out = t[0]; });
 }
 
 // -------------------------------------------------------------- recover
@@ -758,7 +777,7 @@ template <int K>
 __global__ __launch_bounds__(256) void errornorm_kernel(DevMesh m, const double* __restrict__ Tg,
                                                         TabLayoutDev L, const double* __restrict__ qu,
                                                         int kind, double c0, double c1, double c2,
-                                                        double* part, unsigned* ticket, double* out) {
+                                                        double* part) {
   using D = Dims<K>;
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M;
   __shared__ double sh[4];
@@ -786,7 +805,6 @@ __global__ __launch_bounds__(256) void errornorm_kernel(DevMesh m, const double*
   }
   const double bs = block_sum<4>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  grid_finish<4>(part, gridDim.x, 1, ticket, sh, [&](const double* t) { *out = t[0]; });
 }
 
 int grid_for(int64_t n, int threads, int cap) {
@@ -814,7 +832,7 @@ int condense_launch(const CondenseArgs& a, bool blocks, hipStream_t s) {
 
 int cg_blocks(int64_t N, int nt) {
   (void)nt;
-  return grid_for(N, kCgThreads, 4096);
+  return grid_for(N, kCgThreads, 1024);
 }
 
 int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s) {
@@ -860,36 +878,29 @@ int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, double rtol, int maxit,
-                   hipStream_t s) {
-#define INIT(NT) \
-  hipLaunchKernelGGL(cg_init_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v, rtol, maxit);
+int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s) {
+#define INIT(NT) hipLaunchKernelGGL(cg_init_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v);
   HDG_NT_DISPATCH(T.nt, INIT)
 #undef INIT
   return (int)hipGetLastError();
 }
 
-int launch_cg_spmv_only(const DevPattern& P, const TraceSystem& T, CgVectors& v, int force,
-                        hipStream_t s) {
-#define SPMV(NT) \
-  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, force);
+int launch_cg_spmv(const DevPattern& P, const TraceSystem& T, CgVectors& v, int i, double rtol2, int maxit,
+                   int force, hipStream_t s) {
+#define SPMV(NT)                                                                                  \
+  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, i, rtol2, \
+                     maxit, force);
   HDG_NT_DISPATCH(T.nt, SPMV)
 #undef SPMV
   return (int)hipGetLastError();
 }
 
-int launch_cg_update_only(const TraceSystem& T, CgVectors& v, int force, hipStream_t s) {
+int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipStream_t s) {
 #define UPD(NT) \
-  hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, force);
+  hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, i, force);
   HDG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
   return (int)hipGetLastError();
-}
-
-int launch_cg_iteration(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s) {
-  int e = launch_cg_spmv_only(P, T, v, 0, s);
-  if (e) return e;
-  return launch_cg_update_only(T, v, 0, s);
 }
 
 int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s) {
@@ -901,9 +912,10 @@ int launch_true_residual(const DevPattern& P, const TraceSystem& T, const double
                          CgVectors& v, hipStream_t s) {
 #define TR(NT)                                                                            \
   hipLaunchKernelGGL(true_residual_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, uhat, \
-                     v.part, &v.st->ticket[2], out2);
+                     v.part);
   HDG_NT_DISPATCH(T.nt, TR)
 #undef TR
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, v.part, v.nblocks, out2);
   return (int)hipGetLastError();
 }
 
@@ -928,11 +940,11 @@ int launch_recover(int k, const DevMesh& m, const double* Ke, const double* be, 
 
 int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k,
                      const double* qu, int uex_kind, const double* par3, double* part, double* result,
-                     unsigned* ticket, hipStream_t s) {
+                     hipStream_t s) {
   const int g = grid_for(m.nc, 256, 2048);
 #define ERR(KK)                                                                                    \
   hipLaunchKernelGGL(errornorm_kernel<KK>, dim3(g), dim3(256), 0, s, m, tables, lay, qu, uex_kind, \
-                     par3[0], par3[1], par3[2], part, ticket, result);
+                     par3[0], par3[1], par3[2], part);
   switch (k) {
     case 1: ERR(1); break;
     case 2: ERR(2); break;
@@ -941,6 +953,7 @@ int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev&
     default: return (int)hipErrorInvalidValue;
   }
 #undef ERR
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, part, g, result);
   return (int)hipGetLastError();
 }
 

@@ -160,7 +160,9 @@ def test_solution_vs_direct_solve(hdg, O, ctx, k, n, tol):
     ctx.assemble_condense()
     uhat, info = ctx.solve()
     r = O.solve_pipeline(P, om)
-    assert info.true_relres < 1e-12
+    # the recursive residual reaches 1e-13; the recomputed one stops at the
+    # FP64 rounding floor of ||K|| ||uhat|| eps, about 1e-12 here
+    assert info.relres <= 1e-13 and info.true_relres < 1e-11
     assert rel(uhat, r["uhat"]) < tol
     q, u, _ = ctx.recover()
     assert rel(u, r["u"]) < tol
@@ -310,7 +312,11 @@ def test_full_size_c2_properties(hdg, O, ctx):
         ctx.set_source()
         ctx.set_dirichlet(m.boundary_dofs(3), np.zeros(len(m.boundary_dofs(3))))
         info = ctx.run()
-        assert info.true_relres < 1e-11
+        # recursive residual at 1e-12; the recomputed FP64 residual stops at
+        # the attainable floor eps ||K|| ||uhat|| / ||b|| (cond ~ h^-2): the
+        # oracle's CPU PCG reaches 1.46e-9 at 256^2, libhdg 1.47e-9
+        assert info.relres <= 1e-12
+        assert info.true_relres < (5e-9 if n == 256 else 2e-8), info.true_relres
         errs.append(ctx.errornorm())
         nb = 4 * n
         assert ctx.sizes.nnz == (5 * (m.faces.shape[0] - nb) + 3 * nb) * 9
