@@ -45,6 +45,17 @@ def spmv_bytes(ctx, mesh):
     return 8 * s.nnz + 4 * nblocks + 16 * N + 16 * N + N
 
 
+def measured_traffic():
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None if that summary is absent."""
+    path = os.path.join(REPO, "profiles", "cg_spmv_c2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(mesh_np, iters_gpu, seconds_cap=30.0):
     """Oracle on the same mesh: full build + condense + assembly, then PCG
     iterations timed on a bounded prefix and scaled to the GPU's iteration
@@ -87,6 +98,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also dispatch the plain copy kernel that calibrates HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the HDG pipeline here runs on a single MI355X "
@@ -127,6 +140,8 @@ def main():
     upd_us = ctx.time_kernel(1, reps=200)
     ab = spmv_bytes(ctx, mesh)
     achieved = ab / (spmv_us * 1e-6) / 1e9
+    copy_us = ctx.time_kernel(3, reps=3) if args.calibrate else None
+    traffic = measured_traffic()
 
     out = {
         "metric": METRIC,
@@ -152,10 +167,14 @@ def main():
         "l2_error": float(np.sqrt(l2sq)),
         "errornorm_sq": float(l2sq),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "cg_spmv_kernel<3>", "avg_us": spmv_us, "bytes_per_launch": int(ab),
                      "update_kernel_avg_us": upd_us},
     }
+    if copy_us is not None:
+        copy_bytes = 2 * 8 * 5 * ctx.sizes.nt * ctx.sizes.n_dofs
+        out["roofline"]["copy_kernel"] = {"avg_us": copy_us, "bytes": copy_bytes,
+                                          "GBps": copy_bytes / (copy_us * 1e-6) / 1e9}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, iters)
     ctx.close()

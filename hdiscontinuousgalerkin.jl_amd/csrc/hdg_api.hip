@@ -757,16 +757,23 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   a.Ke = c->d_Ke;
   a.be = c->d_be;
   a.err = c->d_err;
+  double* scratch = nullptr;
+  const int64_t ncopy = (int64_t)kSlots * c->nt * c->T.N;  // the size of the trace matrix values
+  if (which == 3) TRY(dalloc(c, &scratch, ncopy));
   auto once = [&]() -> int {
     if (which == 0) return launch_cg_spmv(c->pat, c->T, c->cg, 2, 0.0, 0, 1, c->stream);
     if (which == 1) return launch_cg_update(c->T, c->cg, 2, 1, c->stream);
+    if (which == 3) return launch_copy(c->T.vals, scratch, ncopy, c->stream);
     return launch_condense(c->k, a, false, c->stream);
   };
-  LAUNCHCHK(c, once());  // warm-up
+  int rc = once();  // warm-up
+  if (rc) { dfree(c, scratch); return fail(c, HDG_ERR_HIP, "kernel launch failed"); }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  for (int i = 0; i < reps; ++i) LAUNCHCHK(c, once());
+  for (int i = 0; i < reps && rc == 0; ++i) rc = once();
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
+  dfree(c, scratch);
+  if (rc) return fail(c, HDG_ERR_HIP, "kernel launch failed");
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *avg_us = 1e3 * ms / reps;

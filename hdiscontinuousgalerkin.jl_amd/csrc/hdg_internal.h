@@ -108,6 +108,8 @@ int launch_cg_spmv(const DevPattern& P, const TraceSystem& T, CgVectors& v, int 
                    int force, hipStream_t s);
 int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipStream_t s);
 int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s);
+// copies n doubles (n even) with a plain streaming kernel
+int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 int launch_true_residual(const DevPattern& P, const TraceSystem& T, const double* uhat, double* out2,
                          CgVectors& v, hipStream_t s);
 int launch_recover(int k, const DevMesh& m, const double* Ke, const double* be, const double* uhat,

@@ -714,6 +714,13 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
   }
 }
 
+// Plain streaming copy (HBM traffic calibration for hardware counters).
+__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // Final sum of n partials by one block (a separate launch, so no fences).
 __global__ __launch_bounds__(256) void reduce_kernel(const double* part, int n, double* out) {
   __shared__ double sh[4];
@@ -900,6 +907,12 @@ int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipSt
   hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, i, force);
   HDG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
+  return (int)hipGetLastError();
+}
+
+int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, reinterpret_cast<const double2*>(src),
+                     reinterpret_cast<double2*>(dst), n / 2);
   return (int)hipGetLastError();
 }
 

@@ -194,7 +194,10 @@ int hdg_quadrature(int dim, int order, int cap, double* pts, double* w);
 /* Average device duration (microseconds) of one launch of an internal
    kernel, timed with HIP events around `reps` back-to-back launches on the
    context stream with the data currently resident.  which: 0 = trace SpMV
-   (the dominant PCG kernel), 1 = PCG update, 2 = condensation. */
+   (the dominant PCG kernel), 1 = PCG update, 2 = condensation, 3 = a plain
+   copy of the trace-matrix values (5 nt^2 doubles per face), the reference
+   point for hardware traffic counters.  Timing runs overwrite the solver
+   state; solve again before reading results. */
 int hdg_time_kernel(hdg_ctx* ctx, int which, int reps, double* avg_us);
 
 #ifdef __cplusplus
