@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precond", choices=("auto", "bj", "mg"), default="auto",
+                    help="trace-solve preconditioner: auto (multigrid on structured meshes), "
+                         "block-Jacobi, multigrid")
     ap.add_argument("--calibrate", action="store_true",
                     help="also dispatch the plain copy kernel that calibrates HBM traffic counters")
     args = ap.parse_args()
@@ -113,7 +116,9 @@ def main():
     mesh = H.rectangle_mesh((NX, NY), jitter=0.2, seed=SEED)
     ctx = H.Context(0)
     ctx.set_mesh(mesh)
-    ctx.setup(K_ORDER, quad_degree=QDEG, tau=1.0, rtol=1e-12)
+    pc = {"auto": H.PC_AUTO, "bj": H.PC_BLOCK_JACOBI, "mg": H.PC_MULTIGRID}[args.precond]
+    ctx.setup(K_ORDER, quad_degree=QDEG, tau=1.0, rtol=1e-12, precond=pc)
+    pc_name = "block-Jacobi PCG" if args.precond == "bj" else "multigrid-preconditioned CG"
     ctx.set_source()
     nt = K_ORDER + 1
     dofs = mesh.boundary_dofs(nt)
@@ -157,7 +162,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "HDG Poisson 512x512 cells (524288 triangles) P2, jitter 0.2h "
-                               "seed 20261202, tau=1, FP64, block-Jacobi PCG rtol 1e-12",
+                               f"seed 20261202, tau=1, FP64, {pc_name} rtol 1e-12",
+                   "preconditioner": pc_name,
                    "cells": ncells, "k": K_ORDER, "quad_degree": QDEG, "trace_dofs": int(ctx.sizes.n_dofs),
                    "nnz": int(ctx.sizes.nnz)},
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),

@@ -18,6 +18,7 @@ STATUS = {
     9: "HDG_ERR_UNSUPPORTED", 10: "HDG_ERR_BOUNDARY",
 }
 HDG_FP64, HDG_MIXED = 0, 1
+PC_AUTO, PC_BLOCK_JACOBI, PC_MULTIGRID = 0, 1, 2
 SRC_REFERENCE, SRC_KAPPA, SRC_SINE_SERIES, SRC_VALUES = 0, 1, 2, 3
 UEX_SINSIN, UEX_LINEAR = 0, 1
 
@@ -51,7 +52,7 @@ class SingularException(HDGError, ArithmeticError):
 class Options(C.Structure):
     _fields_ = [("k", C.c_int32), ("quad_degree", C.c_int32), ("tau", C.c_double),
                 ("precision", C.c_int32), ("rtol", C.c_double), ("maxit", C.c_int32),
-                ("schedule", C.c_int32)]
+                ("schedule", C.c_int32), ("precond", C.c_int32)]
 
 
 class Source(C.Structure):

@@ -30,6 +30,8 @@ struct hdg_ctx {
   std::vector<int> h_cf;  // face ids per cell, 0-based
   int64_t nblock_cols = 0;  // stored nt x nt blocks of K
   DevPattern pat;
+  bool structured = false;  // cells follow rectangle_mesh's (nx, ny) numbering
+  MgData mg;                // vertex hierarchy (structured meshes)
 
   // discretisation
   bool has_setup = false;
@@ -142,6 +144,9 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->cg.part);
   dfree(c, c->d_uhat);
   dfree(c, c->d_part2);
+  dfree(c, c->mg.cellA);
+  dfree(c, c->mg.e0);
+  dfree(c, c->mg.res0);
   c->has_setup = c->assembled = c->solved = c->recovered = false;
 }
 
@@ -156,8 +161,96 @@ void free_mesh(hdg_ctx* c) {
   dfree(c, c->pat.nslot);
   dfree(c, c->pat.self);
   dfree(c, c->pat.tslot);
+  dfree(c, c->mg.fv);
+  dfree(c, c->mg.vf_ptr);
+  dfree(c, c->mg.vf_idx);
+  for (int l = 0; l < kMaxLevels; ++l) {
+    MgLevel& L = c->mg.lev[l];
+    dfree(c, L.a);
+    dfree(c, L.dinv);
+    dfree(c, L.mask);
+    dfree(c, L.r);
+    dfree(c, L.e);
+    dfree(c, L.t);
+  }
+  c->mg = MgData{};
+  c->structured = false;
   c->has_mesh = false;
   c->mesh = DevMesh{};
+}
+
+// Cells numbered as rectangle_mesh(TriangleCell, (nx, ny)) numbers them
+// (generate_mesh.jl:290-304): lower triangle (n(i,j), n(i+1,j), n(i,j+1)),
+// then upper (n(i+1,j), n(i+1,j+1), n(i,j+1)), n(i,j) = i + (j-1)(nx+1).
+bool detect_structured(int64_t nn, int64_t nc, const int64_t* cells, int* nx_out, int* ny_out) {
+  const int64_t nx = cells[2] - 2;
+  if (nx < 1 || nc % (2 * nx) != 0) return false;
+  const int64_t ny = nc / (2 * nx);
+  if ((nx + 1) * (ny + 1) != nn) return false;
+  for (int64_t j = 1; j <= ny; ++j)
+    for (int64_t i = 1; i <= nx; ++i) {
+      auto n = [&](int64_t a, int64_t b) { return a + (b - 1) * (nx + 1); };
+      const int64_t* lo = cells + 6 * (2 * ((j - 1) * nx + (i - 1)));
+      const int64_t* up = lo + 6;
+      if (lo[0] != n(i, j) || lo[1] != n(i + 1, j) || lo[2] != n(i, j + 1)) return false;
+      if (up[0] != n(i + 1, j) || up[1] != n(i + 1, j + 1) || up[2] != n(i, j + 1)) return false;
+    }
+  *nx_out = (int)nx;
+  *ny_out = (int)ny;
+  return true;
+}
+
+// face end points and the vertex -> face lists (any mesh), and the vertex
+// grid hierarchy when the mesh is structured
+int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* faces, const int64_t* cells,
+                            int64_t nc) {
+  std::vector<int> fv(2 * nf), cnt(nn + 1, 0);
+  for (int64_t f = 0; f < nf; ++f) {
+    const int a = (int)faces[f] - 1, b = (int)faces[f + nf] - 1;
+    fv[2 * f] = std::min(a, b);
+    fv[2 * f + 1] = std::max(a, b);
+    cnt[a + 1]++;
+    cnt[b + 1]++;
+  }
+  for (int64_t v = 0; v < nn; ++v) cnt[v + 1] += cnt[v];
+  std::vector<int> idx(cnt[nn]), pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t f = 0; f < nf; ++f) {
+    idx[pos[fv[2 * f]]++] = (int)f;
+    idx[pos[fv[2 * f + 1]]++] = (int)f;
+  }
+  MgData& M = c->mg;
+  TRY(dalloc(c, &M.fv, 2 * nf));
+  TRY(dalloc(c, &M.vf_ptr, nn + 1));
+  TRY(dalloc(c, &M.vf_idx, idx.size()));
+  HIPCHK(c, hipMemcpyAsync(M.fv, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M.vf_ptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M.vf_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  M.nv = nn;
+  int nx = 0, ny = 0;
+  c->structured = detect_structured(nn, nc, cells, &nx, &ny);
+  if (!c->structured) return HDG_OK;
+  M.nx = nx;
+  M.ny = ny;
+  int l = 1, lx = nx, ly = ny;
+  for (;;) {
+    MgLevel& L = M.lev[l];
+    L.nx = lx;
+    L.ny = ly;
+    L.n = (int64_t)(lx + 1) * (ly + 1);
+    TRY(dalloc(c, &L.a, 7 * L.n));
+    TRY(dalloc(c, &L.dinv, L.n));
+    TRY(dalloc(c, &L.mask, L.n));
+    TRY(dalloc(c, &L.r, L.n));
+    TRY(dalloc(c, &L.e, L.n));
+    TRY(dalloc(c, &L.t, L.n));
+    ++l;
+    if (lx % 2 || ly % 2 || std::min(lx, ly) < 8 || l >= kMaxLevels) break;
+    lx /= 2;
+    ly /= 2;
+  }
+  M.nlev = l;
+  return HDG_OK;
 }
 
 int select_device(hdg_ctx* c) {
@@ -201,12 +294,23 @@ int assemble_impl(hdg_ctx* c) {
   return HDG_OK;
 }
 
-constexpr int kChunk = 64;  // PCG iterations enqueued between convergence checks
+bool use_mg(const hdg_ctx* c) {
+  return c->structured && c->opt.precond != HDG_PC_BLOCK_JACOBI;
+}
+
+// PCG iterations enqueued between convergence checks
+int chunk_of(const hdg_ctx* c) { return use_mg(c) ? 4 : 64; }
 
 int enqueue_chunk(hdg_ctx* c, int first, double rtol2, int maxit) {
-  for (int i = first; i < first + kChunk; ++i) {
+  const int n = chunk_of(c);
+  for (int i = first; i < first + n; ++i) {
     LAUNCHCHK(c, launch_cg_spmv(c->pat, c->T, c->cg, i, rtol2, maxit, 0, c->stream));
-    LAUNCHCHK(c, launch_cg_update(c->T, c->cg, i, 0, c->stream));
+    if (use_mg(c)) {
+      LAUNCHCHK(c, launch_mg_update(c->T, c->cg, c->mg, i, c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, c->T, c->cg, c->mg, i, c->stream));
+    } else {
+      LAUNCHCHK(c, launch_cg_update(c->T, c->cg, i, 0, c->stream));
+    }
   }
   return HDG_OK;
 }
@@ -220,7 +324,13 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   const double rtol = c->opt.rtol > 0 ? c->opt.rtol : 1e-12;
   const int maxit = c->opt.maxit > 0 ? c->opt.maxit : 200000;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
+  if (use_mg(c)) {
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, c->cg, 10, c->stream));
+    LAUNCHCHK(c, launch_mg_init(c->pat, c->T, c->cg, c->mg, c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, c->T, c->cg, c->mg, 0, c->stream));
+  } else {
+    LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
+  }
   // one chunk stays in flight while the previous chunk's state is read
   // back; launches after convergence return at once
   hipEvent_t evs[2];
@@ -228,13 +338,13 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   HIPCHK(c, hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
   int slot = 0, next = 1;
   TRY(enqueue_chunk(c, next, rtol * rtol, maxit));
-  next += kChunk;
+  next += chunk_of(c);
   HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(evs[slot], c->stream));
   for (;;) {
     const int nxt = slot ^ 1;
     TRY(enqueue_chunk(c, next, rtol * rtol, maxit));
-    next += kChunk;
+    next += chunk_of(c);
     HIPCHK(c, hipMemcpyAsync(&c->h_st[nxt], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
     HIPCHK(c, hipEventSynchronize(evs[slot]));
@@ -390,6 +500,7 @@ int hdg_set_mesh(hdg_ctx* c, int64_t nn, const double* xy, int64_t nc, const int
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->h_cf.swap(cf);
   c->nblock_cols = nblk;
+  TRY(build_vertex_structures(c, nn, nf, faces, cells, nc));
   c->has_mesh = true;
   return HDG_OK;
 }
@@ -400,6 +511,10 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   if (opt->k < 1 || opt->k > 4) return fail(c, HDG_ERR_UNSUPPORTED, "k=%d: supported orders are 1..4", opt->k);
   if (opt->precision != HDG_FP64 && opt->precision != HDG_MIXED)
     return fail(c, HDG_ERR_ARG, "unknown precision mode %d", opt->precision);
+  if (opt->precond < HDG_PC_AUTO || opt->precond > HDG_PC_MULTIGRID)
+    return fail(c, HDG_ERR_ARG, "unknown preconditioner %d", opt->precond);
+  if (opt->precond == HDG_PC_MULTIGRID && !c->structured)
+    return fail(c, HDG_ERR_UNSUPPORTED, "the multigrid preconditioner needs a structured rectangle_mesh");
   TRY(select_device(c));
   hdg_options o = *opt;
   if (o.quad_degree <= 0) o.quad_degree = o.k + 1;
@@ -450,6 +565,11 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   TRY(dalloc(c, &c->cg.part, 4 * (size_t)c->cg.nblocks + 64));
   TRY(dalloc(c, &c->d_uhat, N));
   TRY(dalloc(c, &c->d_part2, 4096));
+  if (c->structured) {
+    TRY(dalloc(c, &c->mg.cellA, 9 * nc));
+    TRY(dalloc(c, &c->mg.e0, N));
+    TRY(dalloc(c, &c->mg.res0, N));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->src = SrcParams{};
   c->jacobi_dirty = true;

@@ -71,7 +71,7 @@ struct CgState {
   double pq;
   double rr;     // (r, r) at the last convergence test
   double bb;     // (b~, b~)
-  double pad0;
+  double omega;  // block-Jacobi smoothing weight of the multigrid preconditioner
   int it;        // completed iterations when the solve stopped
   int maxit;
   int done;      // 0 running, 1 converged, 2 iteration cap, 3 breakdown
@@ -93,6 +93,33 @@ struct CgVectors {
   double* part;  // 4 * nblocks partial sums
   CgState* st;
   int nblocks;
+};
+
+// Multigrid hierarchy on the vertex grids of a structured mesh (level 0 is
+// the trace space, levels 1.. are vertex grids with 7-point stencils).
+struct MgLevel {
+  int nx = 0, ny = 0;
+  int64_t n = 0;                 // (nx+1)(ny+1) nodes
+  double* a = nullptr;           // stencils, a[s*n + v]
+  double* dinv = nullptr;        // 1 / a[0], 0 on fixed nodes
+  unsigned char* mask = nullptr; // fixed (Dirichlet) nodes
+  double *r = nullptr, *e = nullptr, *t = nullptr;
+};
+constexpr int kMaxLevels = 16;
+struct MgData {
+  int nlev = 0;                  // levels 1..nlev-1 are vertex grids
+  int nx = 0, ny = 0;
+  int64_t nv = 0;
+  MgLevel lev[kMaxLevels];
+  int* fv = nullptr;             // [2*nf] lower, higher node id of each face
+  int* vf_ptr = nullptr;         // vertex -> incident faces (CSR)
+  int* vf_idx = nullptr;
+  double* cellA = nullptr;       // [9*nc] per-cell P^T (-Ate) P
+  double* e0 = nullptr;          // trace-level correction
+  double* res0 = nullptr;        // trace-level residual
+  double omega_v = 0.8;          // vertex-level Jacobi weight
+  double omega_factor = 1.5;     // trace-level weight = factor / lambda_max(Dinv S)
+  int coarse_sweeps = 20;
 };
 
 // launchers (return hipError_t as int)
@@ -118,5 +145,11 @@ int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev&
                      const double* qu, int uex_kind, const double* par3, double* part,
                      double* result, hipStream_t s);
 int cg_blocks(int64_t N, int nt);
+int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
+                     CgVectors& v, int power_steps, hipStream_t s);
+int launch_mg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, hipStream_t s);
+int launch_mg_update(const TraceSystem& T, CgVectors& v, MgData& M, int i, hipStream_t s);
+int launch_mg_precond(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, int i,
+                      hipStream_t s);
 
 }  // namespace hdg

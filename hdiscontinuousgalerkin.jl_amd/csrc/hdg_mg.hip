@@ -1,0 +1,646 @@
+// Multigrid preconditioner for the HDG trace system on structured meshes.
+//
+// Level 0 is the trace space (N dofs) with damped block-Jacobi smoothing.
+// Level 1 is the continuous P1 space on the mesh vertices: the prolongation
+// maps vertex values to the Legendre modes 0 and 1 of the trace of the
+// piecewise-linear function on every face (mode 0 = mean, mode 1 =
+// (v_hi - v_lo) / (2 sqrt 3), the face parameter running from the lower to
+// the higher node id), and the coarse operator is the Galerkin product
+// P^T S P, assembled cell by cell from the condensed blocks.  Levels 2.. are
+// the vertex grids of a rectangle_mesh coarsened by two, with linear
+// interpolation on the nested triangulation and Galerkin (RAP) operators;
+// every level is a 7-point stencil.  One symmetric V-cycle (damped Jacobi,
+// fixed sweep counts, Jacobi sweeps on the coarsest grid) is a fixed SPD
+// operator, so it preconditions CG.
+//
+// Stencil slots: 0 (0,0), 1 (+1,0), 2 (-1,0), 3 (0,+1), 4 (0,-1), 5 (+1,-1),
+// 6 (-1,+1); node (i, j) of a level with nx cells per row is i + j (nx+1).
+#include <hip/hip_runtime.h>
+
+#include "hdg_internal.h"
+
+namespace hdg {
+
+namespace {
+
+constexpr double kS3 = 0.28867513459481288225;  // 1 / (2 sqrt 3)
+__constant__ int kOffX[7] = {0, 1, -1, 0, 0, 1, -1};
+__constant__ int kOffY[7] = {0, 0, 0, 1, -1, -1, 1};
+
+__device__ inline int slot_of(int dx, int dy) {
+  if (dx == 0 && dy == 0) return 0;
+  if (dy == 0) return dx == 1 ? 1 : (dx == -1 ? 2 : -1);
+  if (dx == 0) return dy == 1 ? 3 : (dy == -1 ? 4 : -1);
+  if (dx == 1 && dy == -1) return 5;
+  if (dx == -1 && dy == 1) return 6;
+  return -1;
+}
+
+__device__ inline double wave_sum_mg(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <int NW>
+__device__ inline double block_sum_mg(double v, double* sh) {
+  v = wave_sum_mg(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NW; ++i) s += sh[i];
+  __syncthreads();
+  return s;
+}
+
+template <int NW>
+__device__ inline double sum_partials_mg(const double* p, int n, double* sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  s = wave_sum_mg(s);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = s;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+// (S x)_row for S = -K on free rows, from the face-ELL matrix.
+template <int NT>
+__device__ inline double s_row(const DevPattern& P, const TraceSystem& T, const double* x, int64_t r) {
+  const int64_t f = r / NT;
+  double acc = 0.0;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const int64_t g = (int64_t)__ldg(P.bcol + f * kSlots + s) * NT;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) acc = fma(__ldg(T.vals + (int64_t)(s * NT + b) * T.N + r), x[g + b], acc);
+  }
+  return T.dmask[r] ? 0.0 : -acc;
+}
+
+// ------------------------------------------------------------ set-up
+// vertex mask: a vertex is fixed when a face through it has a Dirichlet
+// mode-0 dof.
+__global__ void mg_vmask_kernel(MgData M, TraceSystem T) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= M.nv) return;
+  unsigned char m = 0;
+  for (int k = M.vf_ptr[v]; k < M.vf_ptr[v + 1]; ++k) m |= T.dmask[(int64_t)M.vf_idx[k] * T.nt];
+  M.lev[1].mask[v] = m;
+}
+
+// A1_c = P_c^T (-Ate_c) P_c for every cell, masked.
+template <int NT>
+__global__ void mg_cell_coarse_kernel(DevMesh mesh, MgData M, TraceSystem T, const double* __restrict__ Ate) {
+  constexpr int NTR = 3 * NT;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= mesh.nc) return;
+  int vg[3];
+  unsigned char vm[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    vg[v] = mesh.cn[3 * c + v];
+    vm[v] = M.lev[1].mask[vg[v]];
+  }
+  // P_c rows of modes 0 and 1 of each local face: weights on local vertices
+  // local face lf joins local vertices (lf+1)%3, (lf+2)%3
+  double Pw[3][2][3];
+#pragma unroll
+  for (int lf = 0; lf < 3; ++lf) {
+    const int a = (lf + 1) % 3, b = (lf + 2) % 3;
+    const int lo = vg[a] < vg[b] ? a : b, hi = vg[a] < vg[b] ? b : a;
+    const int64_t f = mesh.cf[3 * c + lf];
+    const bool d0 = T.dmask[f * NT] != 0, d1 = T.dmask[f * NT + 1] != 0;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      const double keep = vm[w] ? 0.0 : 1.0;
+      Pw[lf][0][w] = d0 ? 0.0 : keep * ((w == a || w == b) ? 0.5 : 0.0);
+      Pw[lf][1][w] = d1 ? 0.0 : keep * (w == hi ? kS3 : (w == lo ? -kS3 : 0.0));
+    }
+  }
+  const double* at = Ate + c * NTR * NTR;
+  // Q = (-Ate) P_c  (NTR x 3), then A = P_c^T Q
+  double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+  for (int lf = 0; lf < 3; ++lf)
+#pragma unroll
+    for (int md = 0; md < 2; ++md) {
+      const int row = lf * NT + md;
+      double q[3] = {0, 0, 0};
+#pragma unroll
+      for (int lg = 0; lg < 3; ++lg)
+#pragma unroll
+        for (int me = 0; me < 2; ++me) {
+          const double s = -at[row * NTR + lg * NT + me];
+#pragma unroll
+          for (int w = 0; w < 3; ++w) q[w] = fma(s, Pw[lg][me][w], q[w]);
+        }
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int w = 0; w < 3; ++w) A[u][w] = fma(Pw[lf][md][u], q[w], A[u][w]);
+    }
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+#pragma unroll
+    for (int w = 0; w < 3; ++w) M.cellA[c * 9 + u * 3 + w] = A[u][w];
+}
+
+// Level-1 stencils gathered from the (up to) six cells around each vertex.
+__global__ void mg_assemble_l1_kernel(DevMesh mesh, MgData M) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel L = M.lev[1];
+  if (v >= L.n) return;
+  const int nx = M.nx, ny = M.ny;
+  const int i = (int)(v % (nx + 1)), j = (int)(v / (nx + 1));
+  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (!L.mask[v]) {
+    // (cell grid position, upper?, local index of v)
+    const int ca[6] = {i, i - 1, i, i - 1, i - 1, i};
+    const int cb[6] = {j, j, j - 1, j, j - 1, j - 1};
+    const int up[6] = {0, 0, 0, 1, 1, 1};
+    const int lv[6] = {0, 1, 2, 0, 1, 2};
+    for (int t = 0; t < 6; ++t) {
+      const int a = ca[t], b = cb[t];
+      if (a < 0 || a >= nx || b < 0 || b >= ny) continue;
+      const int64_t c = 2 * ((int64_t)b * nx + a) + up[t];
+      // local vertex grid positions of the cell
+      int px[3], py[3];
+      if (!up[t]) { px[0] = a; py[0] = b; px[1] = a + 1; py[1] = b; px[2] = a; py[2] = b + 1; }
+      else { px[0] = a + 1; py[0] = b; px[1] = a + 1; py[1] = b + 1; px[2] = a; py[2] = b + 1; }
+      for (int w = 0; w < 3; ++w) {
+        const int s = slot_of(px[w] - i, py[w] - j);
+        if (s >= 0) st[s] += M.cellA[c * 9 + lv[t] * 3 + w];
+      }
+    }
+  }
+  for (int s = 0; s < 7; ++s) L.a[s * L.n + v] = L.mask[v] ? (s == 0 ? 1.0 : 0.0) : st[s];
+  L.dinv[v] = L.mask[v] ? 0.0 : 1.0 / st[0];
+}
+
+// Galerkin coarse stencil R A P on the next level (coarse node = fine (2I,2J)).
+__global__ void mg_rap_kernel(MgLevel F, MgLevel Cc) {
+  const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (C >= Cc.n) return;
+  const int cnx = Cc.nx, cny = Cc.ny, fnx = F.nx, fny = F.ny;
+  const int I = (int)(C % (cnx + 1)), J = (int)(C / (cnx + 1));
+  const unsigned char cm = Cc.mask[C];  // written by mg_cmask_kernel
+  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (!cm) {
+    const int sx[7] = {0, 1, -1, 0, 0, 1, -1}, sy[7] = {0, 0, 0, 1, -1, -1, 1};
+    for (int t = 0; t < 7; ++t) {  // fine support of C
+      const int fx = 2 * I + sx[t], fy = 2 * J + sy[t];
+      if (fx < 0 || fx > fnx || fy < 0 || fy > fny) continue;
+      const int64_t fi = (int64_t)fy * (fnx + 1) + fx;
+      if (F.mask[fi]) continue;
+      const double w1 = t == 0 ? 1.0 : 0.5;
+      for (int s = 0; s < 7; ++s) {
+        const double av = F.a[s * F.n + fi];
+        if (av == 0.0) continue;
+        const int gx = fx + kOffX[s], gy = fy + kOffY[s];
+        if (gx < 0 || gx > fnx || gy < 0 || gy > fny) continue;
+        if (F.mask[(int64_t)gy * (fnx + 1) + gx]) continue;
+        // coarse parents of fine node g
+        int px[2], py[2], np = 0;
+        double w2;
+        if (!(gx & 1) && !(gy & 1)) { px[0] = gx / 2; py[0] = gy / 2; np = 1; w2 = 1.0; }
+        else if ((gx & 1) && !(gy & 1)) { px[0] = (gx - 1) / 2; py[0] = gy / 2; px[1] = (gx + 1) / 2; py[1] = gy / 2; np = 2; w2 = 0.5; }
+        else if (!(gx & 1) && (gy & 1)) { px[0] = gx / 2; py[0] = (gy - 1) / 2; px[1] = gx / 2; py[1] = (gy + 1) / 2; np = 2; w2 = 0.5; }
+        else { px[0] = (gx + 1) / 2; py[0] = (gy - 1) / 2; px[1] = (gx - 1) / 2; py[1] = (gy + 1) / 2; np = 2; w2 = 0.5; }
+        for (int q = 0; q < np; ++q) {
+          if (Cc.mask[(int64_t)py[q] * (cnx + 1) + px[q]]) continue;  // zero column of P
+          const int sl = slot_of(px[q] - I, py[q] - J);
+          if (sl >= 0) st[sl] = fma(w1 * av, w2, st[sl]);
+        }
+      }
+    }
+  }
+  for (int s = 0; s < 7; ++s) Cc.a[s * Cc.n + C] = cm ? (s == 0 ? 1.0 : 0.0) : st[s];
+  Cc.dinv[C] = cm ? 0.0 : 1.0 / st[0];
+}
+
+// Coarse masks (a coarse node is fixed when its fine node (2I, 2J) is),
+// written before mg_rap_kernel reads the masks of neighbouring nodes.
+__global__ void mg_cmask_kernel(MgLevel F, MgLevel Cc) {
+  const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (C >= Cc.n) return;
+  const int I = (int)(C % (Cc.nx + 1)), J = (int)(C / (Cc.nx + 1));
+  Cc.mask[C] = F.mask[(int64_t)(2 * J) * (F.nx + 1) + 2 * I];
+}
+
+// ---------------------------------------------- power iteration for omega
+__device__ inline double hash01(uint64_t i) {
+  uint64_t z = i * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceSystem T, double* x, double* part) {
+  __shared__ double sh[NT];
+  double s2 = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x) {
+    const double v = T.dmask[r] ? 0.0 : 0.5 + hash01((uint64_t)r);
+    x[r] = v;
+    s2 = fma(v, v, s2);
+  }
+  const double b = block_sum_mg<NT>(s2, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+// y = Dinv S (x / |x|), |x|^2 from the previous step's partials.
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void pow_step_kernel(DevPattern P, TraceSystem T, const double* x,
+                                                           double* y, const double* part_in, double* part_out,
+                                                           int nb) {
+  constexpr int NTH = 64 * NT;
+  __shared__ double rs[NTH];
+  __shared__ double sh[NT];
+  const double nx = sqrt(sum_partials_mg<NT>(part_in, nb, sh));
+  const double inv = nx > 0.0 ? 1.0 / nx : 0.0;
+  double s2 = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+    const int64_t r = r0 + threadIdx.x;
+    rs[threadIdx.x] = r < T.N ? inv * s_row<NT>(P, T, x, r) : 0.0;
+    __syncthreads();
+    if (r < T.N) {
+      const int lb = (threadIdx.x / NT) * NT;
+      double z = 0.0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      y[r] = z;
+      s2 = fma(z, z, s2);
+    }
+    __syncthreads();
+  }
+  const double b = block_sum_mg<NT>(s2, sh);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(256) void pow_final_kernel(const double* part, int nb, CgState* st, double cfac) {
+  __shared__ double sh[4];
+  const double lam = sqrt(sum_partials_mg<4>(part, nb, sh));
+  if (threadIdx.x == 0) st->omega = lam > 0.0 ? cfac / lam : 0.5;
+}
+
+// ------------------------------------------------------- level 0 (trace)
+// update(i) with the multigrid preconditioner: x += alpha p; r -= alpha q;
+// e0 = omega Dinv r (first smoothing step); partial (r, r).
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceSystem T, CgVectors v, MgData M, int i) {
+  constexpr int NTH = 64 * NT;
+  __shared__ double rs[NTH];
+  __shared__ double sh[NT];
+  CgState* st = v.st;
+  if (st->done) return;
+  const int nb = v.nblocks;
+  const double pq = sum_partials_mg<NT>(v.part, nb, sh);
+  const double rz = st->rz[i & 1];
+  if (!(pq > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 3;
+      st->it = i - 1;
+    }
+    return;
+  }
+  const double alpha = rz / pq, om = st->omega;
+  const double* __restrict__ p = (i & 1) ? v.p1 : v.p0;
+  double prr = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+    const int64_t r = r0 + threadIdx.x;
+    double rn = 0.0;
+    if (r < T.N) {
+      v.x[r] = fma(alpha, p[r], v.x[r]);
+      rn = fma(-alpha, v.q[r], v.r[r]);
+      v.r[r] = rn;
+      prr = fma(rn, rn, prr);
+    }
+    rs[threadIdx.x] = rn;
+    __syncthreads();
+    if (r < T.N) {
+      const int lb = (threadIdx.x / NT) * NT;
+      double z = 0.0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      M.e0[r] = om * z;
+    }
+    __syncthreads();
+  }
+  const double b = block_sum_mg<NT>(prr, sh);
+  if (threadIdx.x == 0) v.part[(size_t)nb * 3 + blockIdx.x] = b;
+}
+
+// init (update(0)) for the multigrid path: r0 = b~, x = p = 0, e0 = omega Dinv r0.
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSystem T, CgVectors v, MgData M) {
+  constexpr int NTH = 64 * NT;
+  __shared__ double rs[NTH];
+  __shared__ double sh[NT];
+  const double om = v.st->omega;
+  double prr = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+    const int64_t r = r0 + threadIdx.x;
+    double b = 0.0;
+    if (r < T.N) {
+      if (!T.dmask[r]) {
+        const int64_t f = r / NT;
+        double s = T.rhs[r];
+#pragma unroll
+        for (int sl = 0; sl < kSlots; ++sl) {
+          const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+#pragma unroll
+          for (int c = 0; c < NT; ++c) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
+        }
+        b = -s;
+      }
+      v.r[r] = b;
+      v.x[r] = 0.0;
+      v.p0[r] = 0.0;
+      v.p1[r] = 0.0;
+      prr = fma(b, b, prr);
+    }
+    rs[threadIdx.x] = b;
+    __syncthreads();
+    if (r < T.N) {
+      const int lb = (threadIdx.x / NT) * NT;
+      double z = 0.0;
+#pragma unroll
+      for (int c = 0; c < NT; ++c) z = fma(T.Dinv[(int64_t)c * T.N + r], rs[lb + c], z);
+      M.e0[r] = om * z;
+    }
+    __syncthreads();
+  }
+  const double bb = block_sum_mg<NT>(prr, sh);
+  if (threadIdx.x == 0) v.part[(size_t)v.nblocks * 3 + blockIdx.x] = bb;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    v.st->done = 0;
+    v.st->it = 0;
+    v.st->pq = 0.0;
+  }
+}
+
+// res0 = r - S e0.
+template <int NT>
+__global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceSystem T, CgVectors v, MgData M) {
+  if (v.st->done) return;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x)
+    M.res0[r] = T.dmask[r] ? 0.0 : v.r[r] - s_row<NT>(P, T, M.e0, r);
+}
+
+// r1 = P0^T res0, vertex by vertex over the incident faces.
+template <int NT>
+__global__ __launch_bounds__(256) void mg_restrict0_kernel(CgVectors v, MgData M) {
+  if (v.st->done) return;
+  const MgLevel L = M.lev[1];
+  const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vv >= L.n) return;
+  double s = 0.0;
+  if (!L.mask[vv])
+    for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
+      const int64_t f = M.vf_idx[k];
+      const bool lo = M.fv[2 * f] == vv;
+      s += 0.5 * M.res0[f * NT] + (lo ? -kS3 : kS3) * M.res0[f * NT + 1];
+    }
+  L.r[vv] = s;
+}
+
+// e0 += P0 e1 on the modes 0 and 1 of every free face.
+template <int NT>
+__global__ __launch_bounds__(256) void mg_prolong0_kernel(TraceSystem T, CgVectors v, MgData M) {
+  if (v.st->done) return;
+  const MgLevel L = M.lev[1];
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= T.nf) return;
+  const double ea = L.e[M.fv[2 * f]], eb = L.e[M.fv[2 * f + 1]];
+  if (!T.dmask[f * NT]) M.e0[f * NT] += 0.5 * (ea + eb);
+  if (!T.dmask[f * NT + 1]) M.e0[f * NT + 1] += kS3 * (eb - ea);
+}
+
+// z = e0 + omega Dinv (r - S e0); partial (r, z) for the next iteration.
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(DevPattern P, TraceSystem T, CgVectors v, MgData M,
+                                                            int i) {
+  constexpr int NTH = 64 * NT;
+  __shared__ double rs[NTH];
+  __shared__ double sh[NT];
+  if (v.st->done) return;
+  const double om = v.st->omega;
+  double prz = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+    const int64_t r = r0 + threadIdx.x;
+    double rr = 0.0;
+    if (r < T.N) rr = T.dmask[r] ? 0.0 : v.r[r] - s_row<NT>(P, T, M.e0, r);
+    rs[threadIdx.x] = rr;
+    __syncthreads();
+    if (r < T.N) {
+      const int lb = (threadIdx.x / NT) * NT;
+      double z = 0.0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      z = fma(om, z, M.e0[r]);
+      v.z[r] = z;
+      prz = fma(v.r[r], z, prz);
+    }
+    __syncthreads();
+  }
+  const double b = block_sum_mg<NT>(prz, sh);
+  if (threadIdx.x == 0) v.part[(size_t)v.nblocks * (1 + ((i + 1) & 1)) + blockIdx.x] = b;
+}
+
+// ------------------------------------------------------- vertex levels
+// e = om dinv r
+__global__ __launch_bounds__(256) void v_first_kernel(MgLevel L, double om, const CgState* st) {
+  if (st->done) return;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= L.n) return;
+  L.e[v] = om * L.dinv[v] * L.r[v];
+}
+
+__device__ inline double v_apply(const MgLevel& L, const double* x, int64_t v) {
+  const int i = (int)(v % (L.nx + 1)), j = (int)(v / (L.nx + 1));
+  double s = L.a[v] * x[v];
+  if (i + 1 <= L.nx) s = fma(L.a[1 * L.n + v], x[v + 1], s);
+  if (i - 1 >= 0) s = fma(L.a[2 * L.n + v], x[v - 1], s);
+  if (j + 1 <= L.ny) s = fma(L.a[3 * L.n + v], x[v + L.nx + 1], s);
+  if (j - 1 >= 0) s = fma(L.a[4 * L.n + v], x[v - L.nx - 1], s);
+  if (i + 1 <= L.nx && j - 1 >= 0) s = fma(L.a[5 * L.n + v], x[v + 1 - L.nx - 1], s);
+  if (i - 1 >= 0 && j + 1 <= L.ny) s = fma(L.a[6 * L.n + v], x[v - 1 + L.nx + 1], s);
+  return s;
+}
+
+// out = in + om dinv (r - A in)
+__global__ __launch_bounds__(256) void v_smooth_kernel(MgLevel L, const double* in, double* out, double om,
+                                                       const CgState* st) {
+  if (st->done) return;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= L.n) return;
+  out[v] = in[v] + om * L.dinv[v] * (L.r[v] - v_apply(L, in, v));
+}
+
+// coarse r = R (r - A e) over the fine support of each coarse node
+__global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, const CgState* st) {
+  if (st->done) return;
+  const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (C >= Cc.n) return;
+  double s = 0.0;
+  if (!Cc.mask[C]) {
+    const int I = (int)(C % (Cc.nx + 1)), J = (int)(C / (Cc.nx + 1));
+    const int sx[7] = {0, 1, -1, 0, 0, 1, -1}, sy[7] = {0, 0, 0, 1, -1, -1, 1};
+    for (int t = 0; t < 7; ++t) {
+      const int fx = 2 * I + sx[t], fy = 2 * J + sy[t];
+      if (fx < 0 || fx > F.nx || fy < 0 || fy > F.ny) continue;
+      const int64_t fi = (int64_t)fy * (F.nx + 1) + fx;
+      if (F.mask[fi]) continue;
+      s += (t == 0 ? 1.0 : 0.5) * (F.r[fi] - v_apply(F, F.e, fi));
+    }
+  }
+  Cc.r[C] = s;
+}
+
+// fine e += P coarse e
+__global__ __launch_bounds__(256) void v_prolong_kernel(MgLevel F, MgLevel Cc, const CgState* st) {
+  if (st->done) return;
+  const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= F.n || F.mask[fi]) return;
+  const int x = (int)(fi % (F.nx + 1)), y = (int)(fi / (F.nx + 1));
+  const int cs = Cc.nx + 1;
+  double add;
+  if (!(x & 1) && !(y & 1)) add = Cc.e[(int64_t)(y / 2) * cs + x / 2];
+  else if ((x & 1) && !(y & 1)) add = 0.5 * (Cc.e[(int64_t)(y / 2) * cs + (x - 1) / 2] + Cc.e[(int64_t)(y / 2) * cs + (x + 1) / 2]);
+  else if (!(x & 1) && (y & 1)) add = 0.5 * (Cc.e[(int64_t)((y - 1) / 2) * cs + x / 2] + Cc.e[(int64_t)((y + 1) / 2) * cs + x / 2]);
+  else add = 0.5 * (Cc.e[(int64_t)((y - 1) / 2) * cs + (x + 1) / 2] + Cc.e[(int64_t)((y + 1) / 2) * cs + (x - 1) / 2]);
+  F.e[fi] += add;
+}
+
+// Coarsest level: `sweeps` damped Jacobi sweeps from zero in one block.
+__global__ __launch_bounds__(1024) void v_coarsest_kernel(MgLevel L, double om, int sweeps, const CgState* st) {
+  if (st->done) return;
+  for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x) L.e[v] = om * L.dinv[v] * L.r[v];
+  __syncthreads();
+  for (int it = 1; it < sweeps; ++it) {
+    for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x)
+      L.t[v] = L.e[v] + om * L.dinv[v] * (L.r[v] - v_apply(L, L.e, v));
+    __syncthreads();
+    for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x) L.e[v] = L.t[v];
+    __syncthreads();
+  }
+}
+
+int grid1(int64_t n, int t = 256) {
+  int64_t g = (n + t - 1) / t;
+  if (g < 1) g = 1;
+  if (g > (1 << 30)) g = 1 << 30;
+  return (int)g;
+}
+
+#define MG_NT_DISPATCH(nt, CALL) \
+  switch (nt) {                  \
+    case 2: CALL(2); break;      \
+    case 3: CALL(3); break;      \
+    case 4: CALL(4); break;      \
+    case 5: CALL(5); break;      \
+    default: return (int)hipErrorInvalidValue; \
+  }
+
+int vcycle(MgData& M, int l, const CgState* st, hipStream_t s) {
+  MgLevel& L = M.lev[l];
+  const double om = M.omega_v;
+  if (l == M.nlev - 1) {
+    hipLaunchKernelGGL(v_coarsest_kernel, dim3(1), dim3(1024), 0, s, L, om, M.coarse_sweeps, st);
+    return (int)hipGetLastError();
+  }
+  const int g = grid1(L.n);
+  hipLaunchKernelGGL(v_first_kernel, dim3(g), dim3(256), 0, s, L, om, st);
+  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, L.e, L.t, om, st);
+  std::swap(L.e, L.t);
+  MgLevel& Cc = M.lev[l + 1];
+  hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(Cc.n)), dim3(256), 0, s, L, Cc, st);
+  int e = vcycle(M, l + 1, st, s);
+  if (e) return e;
+  hipLaunchKernelGGL(v_prolong_kernel, dim3(g), dim3(256), 0, s, L, Cc, st);
+  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, L.e, L.t, om, st);
+  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, L.t, L.e, om, st);
+  return (int)hipGetLastError();
+}
+
+}  // namespace
+
+int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
+                     CgVectors& v, int power_steps, hipStream_t s) {
+  hipLaunchKernelGGL(mg_vmask_kernel, dim3(grid1(M.nv)), dim3(256), 0, s, M, T);
+#define CELL(NT) hipLaunchKernelGGL(mg_cell_coarse_kernel<NT>, dim3(grid1(mesh.nc)), dim3(256), 0, s, mesh, M, T, Ate);
+  MG_NT_DISPATCH(T.nt, CELL)
+#undef CELL
+  hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, mesh, M);
+  for (int l = 1; l + 1 < M.nlev; ++l) {
+    hipLaunchKernelGGL(mg_cmask_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
+    hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
+  }
+  // omega = cfac / lambda_max(Dinv S), power iteration on free dofs; the
+  // CG vectors x, r serve as scratch before the solve initialises them
+  const int nb = v.nblocks;
+  double* pa = v.part;
+  double* pb = v.part + nb;
+#define PINIT(NT) hipLaunchKernelGGL(pow_init_kernel<NT>, dim3(nb), dim3(64 * NT), 0, s, T, v.x, pa);
+  MG_NT_DISPATCH(T.nt, PINIT)
+#undef PINIT
+  double* xa = v.x;
+  double* xb = v.r;
+  for (int k = 0; k < power_steps; ++k) {
+#define PSTEP(NT) \
+    hipLaunchKernelGGL(pow_step_kernel<NT>, dim3(nb), dim3(64 * NT), 0, s, P, T, xa, xb, pa, pb, nb);
+    MG_NT_DISPATCH(T.nt, PSTEP)
+#undef PSTEP
+    std::swap(xa, xb);
+    std::swap(pa, pb);
+  }
+  hipLaunchKernelGGL(pow_final_kernel, dim3(1), dim3(256), 0, s, pa, nb, v.st, M.omega_factor);
+  return (int)hipGetLastError();
+}
+
+int launch_mg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, hipStream_t s) {
+#define INIT(NT) hipLaunchKernelGGL(mg_init_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v, M);
+  MG_NT_DISPATCH(T.nt, INIT)
+#undef INIT
+  return (int)hipGetLastError();
+}
+
+int launch_mg_update(const TraceSystem& T, CgVectors& v, MgData& M, int i, hipStream_t s) {
+#define UPD(NT) hipLaunchKernelGGL(mg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, i);
+  MG_NT_DISPATCH(T.nt, UPD)
+#undef UPD
+  return (int)hipGetLastError();
+}
+
+// z = V-cycle(r), starting from e0 = omega Dinv r already in M.e0; writes the
+// (r, z) partials of iteration i+1.
+int launch_mg_precond(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, int i,
+                      hipStream_t s) {
+  const int gr = grid1(T.N, 256);
+  const int grows = gr > 4096 ? 4096 : gr;
+#define RES(NT) hipLaunchKernelGGL(mg_resid0_kernel<NT>, dim3(grows), dim3(256), 0, s, P, T, v, M);
+  MG_NT_DISPATCH(T.nt, RES)
+#undef RES
+#define RST(NT) hipLaunchKernelGGL(mg_restrict0_kernel<NT>, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, v, M);
+  MG_NT_DISPATCH(T.nt, RST)
+#undef RST
+  int e = vcycle(M, 1, v.st, s);
+  if (e) return e;
+#define PRO(NT) hipLaunchKernelGGL(mg_prolong0_kernel<NT>, dim3(grid1(T.nf)), dim3(256), 0, s, T, v, M);
+  MG_NT_DISPATCH(T.nt, PRO)
+#undef PRO
+#define SMO(NT) hipLaunchKernelGGL(mg_smooth0_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v, M, i);
+  MG_NT_DISPATCH(T.nt, SMO)
+#undef SMO
+  return (int)hipGetLastError();
+}
+
+}  // namespace hdg

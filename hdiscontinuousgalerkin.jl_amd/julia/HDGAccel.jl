@@ -34,6 +34,7 @@ struct Options
     rtol::Float64
     maxit::Int32
     schedule::Int32
+    precond::Int32
 end
 
 struct Source
@@ -91,7 +92,7 @@ order(Wh) = getorder(Wh.fe.func_basis)
 quad_degree(Wh) = length(Wh.qr_face_weigths)   # DefaultQuad: GL with quad_degree points
 
 function setup!(ctx::Context, Wh; tau = 1.0, precision = 0, rtol = 1e-12, maxit = 0)
-    o = Ref(Options(order(Wh), quad_degree(Wh), tau, precision, rtol, maxit, 0))
+    o = Ref(Options(order(Wh), quad_degree(Wh), tau, precision, rtol, maxit, 0, 0))
     check(ctx, ccall((:hdg_setup, libhdg), Cint, (Ptr{Cvoid}, Ptr{Options}), ctx.h, o))
 end
 

@@ -322,3 +322,56 @@ def test_full_size_c2_properties(hdg, O, ctx):
         assert ctx.sizes.nnz == (5 * (m.faces.shape[0] - nb) + 3 * nb) * 9
     # squared L2 error: halving h divides it by 2^(2(k+1)) = 64
     assert 40.0 < errs[0] / errs[1] < 90.0, errs
+
+
+# ------------------------------------------------------- multigrid path
+@pytest.mark.parametrize("k,n", [(1, 32), (2, 32), (3, 16), (4, 16), (2, 64)])
+def test_multigrid_matches_block_jacobi_and_direct(hdg, O, ctx, k, n):
+    """The multigrid-preconditioned CG (structured meshes) converges in a
+    mesh-independent number of iterations and gives the same trace solution
+    as the block-Jacobi PCG and the oracle's direct solve."""
+    m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=3)
+    om = O.rectangle_mesh(n, n, jitter=0.2, seed=3)
+    sols = {}
+    for pc in (hdg.PC_BLOCK_JACOBI, hdg.PC_MULTIGRID):
+        ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=QD[k], rtol=1e-13, precond=pc)
+        ctx.set_source()
+        P = O.problem(k=k, qdeg=QD[k])
+        d, v = O.dirichlet_trace(P, om)
+        ctx.set_dirichlet(d, v)
+        ctx.assemble_condense()
+        sols[pc] = ctx.solve()
+    (ub, ib), (um, im) = sols[hdg.PC_BLOCK_JACOBI], sols[hdg.PC_MULTIGRID]
+    assert im.iters < 60 < ib.iters or n < 64
+    assert im.relres <= 1e-13
+    r = O.solve_pipeline(O.problem(k=k, qdeg=QD[k]), om) if n <= 32 else None
+    if r is not None:
+        assert rel(um, r["uhat"]) < 1e-10
+    assert rel(um, ub) < 1e-9
+
+
+def test_multigrid_iterations_mesh_independent(hdg, ctx):
+    its = []
+    for n in (64, 128, 256):
+        m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=20261202)
+        ctx.set_mesh(m)
+        ctx.setup(2, quad_degree=3, rtol=1e-12, precond=hdg.PC_MULTIGRID)
+        ctx.set_source()
+        ctx.set_dirichlet(m.boundary_dofs(3), np.zeros(len(m.boundary_dofs(3))))
+        info = ctx.run()
+        its.append(info.iters)
+        assert info.true_relres < 1e-9
+    assert max(its) < 50, its
+
+
+def test_multigrid_rejects_unstructured(hdg, ctx):
+    m = hdg.parse_mesh_triangle(os.path.join(GOLDEN, "figure.1"))
+    ctx.set_mesh(m)
+    with pytest.raises(hdg.HDGError, match="UNSUPPORTED"):
+        ctx.setup(1, precond=hdg.PC_MULTIGRID)
+    ctx.setup(1)  # auto falls back to block-Jacobi
+    ctx.set_source()
+    ctx.set_dirichlet(m.boundary_dofs(2), np.zeros(len(m.boundary_dofs(2))))
+    info = ctx.run()
+    assert info.relres <= 1e-12
