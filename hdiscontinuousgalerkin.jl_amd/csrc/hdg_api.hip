@@ -147,6 +147,7 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->mg.cellA);
   dfree(c, c->mg.e0);
   dfree(c, c->mg.res0);
+  dfree(c, c->mg.sp);
   c->has_setup = c->assembled = c->solved = c->recovered = false;
 }
 
@@ -164,6 +165,7 @@ void free_mesh(hdg_ctx* c) {
   dfree(c, c->mg.fv);
   dfree(c, c->mg.vf_ptr);
   dfree(c, c->mg.vf_idx);
+  dfree(c, c->mg.fq4);
   for (int l = 0; l < kMaxLevels; ++l) {
     MgLevel& L = c->mg.lev[l];
     dfree(c, L.a);
@@ -232,6 +234,29 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
   if (!c->structured) return HDG_OK;
   M.nx = nx;
   M.ny = ny;
+  // the four vertices around every face (stencil of S P0)
+  std::vector<int> fq4(4 * nf);
+  for (int64_t f = 0; f < nf; ++f) {
+    const int a = fv[2 * f], b = fv[2 * f + 1];
+    int opp[2];
+    for (int side = 0; side < 2; ++side) {
+      const int64_t cc = faces[f + (2 + side) * nf] - 1;
+      opp[side] = -1;
+      if (cc < 0) continue;
+      for (int w = 0; w < 3; ++w) {
+        const int nw = (int)cells[6 * cc + w] - 1;
+        if (nw != a && nw != b) opp[side] = nw;
+      }
+    }
+    if (opp[0] < 0) return fail(c, HDG_ERR_ARG, "face %lld has no first cell", (long long)f + 1);
+    fq4[4 * f] = a;
+    fq4[4 * f + 1] = b;
+    fq4[4 * f + 2] = opp[0];
+    fq4[4 * f + 3] = opp[1] < 0 ? opp[0] : opp[1];
+  }
+  TRY(dalloc(c, &M.fq4, 4 * nf));
+  HIPCHK(c, hipMemcpyAsync(M.fq4, fq4.data(), sizeof(int) * fq4.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   int l = 1, lx = nx, ly = ny;
   for (;;) {
     MgLevel& L = M.lev[l];
@@ -250,6 +275,14 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     ly /= 2;
   }
   M.nlev = l;
+  // smallest suffix of levels whose nodes fit the one-workgroup budget
+  M.tail = M.nlev;
+  int64_t acc = 0;
+  for (int t = M.nlev - 1; t >= 1; --t) {
+    acc += M.lev[t].n;
+    if (acc > kTailNodes) break;
+    M.tail = t;
+  }
   return HDG_OK;
 }
 
@@ -569,6 +602,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     TRY(dalloc(c, &c->mg.cellA, 9 * nc));
     TRY(dalloc(c, &c->mg.e0, N));
     TRY(dalloc(c, &c->mg.res0, N));
+    TRY(dalloc(c, &c->mg.sp, 4 * N));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->src = SrcParams{};

@@ -106,6 +106,9 @@ struct MgLevel {
   double *r = nullptr, *e = nullptr, *t = nullptr;
 };
 constexpr int kMaxLevels = 16;
+// node budget of the levels solved inside one workgroup: three vectors of
+// this many doubles live in LDS (36 KB)
+constexpr int kTailNodes = 1536;
 struct MgData {
   int nlev = 0;                  // levels 1..nlev-1 are vertex grids
   int nx = 0, ny = 0;
@@ -117,6 +120,12 @@ struct MgData {
   double* cellA = nullptr;       // [9*nc] per-cell P^T (-Ate) P
   double* e0 = nullptr;          // trace-level correction
   double* res0 = nullptr;        // trace-level residual
+  int* fq4 = nullptr;            // [4*nf] the 4 vertices of the two cells of a face:
+                                 // lower, higher end point, opposite vertex of cell 1, of cell 2
+                                 // (= that of cell 1 on boundary faces)
+  double* sp = nullptr;          // [4*N] S P0 restricted to the rows: sp[j*N + r] couples
+                                 // row r with vertex fq4[4*face(r) + j]
+  int tail = kMaxLevels;         // levels tail..nlev-1 run in one workgroup (LDS)
   double omega_v = 0.8;          // vertex-level Jacobi weight
   double omega_factor = 1.5;     // trace-level weight = factor / lambda_max(Dinv S)
   int coarse_sweeps = 20;

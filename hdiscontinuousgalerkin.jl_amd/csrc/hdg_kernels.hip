@@ -71,12 +71,15 @@ struct Src {
       }
       case 2: {
         double sxp[4], syq[4];
+#pragma unroll
         for (int p = 0; p < 4; ++p) {
           sxp[p] = sin((p + 1) * kPi * x);
           syq[p] = sin((p + 1) * kPi * y);
         }
         double v = 0.0;
+#pragma unroll
         for (int p = 0; p < 4; ++p)
+#pragma unroll
           for (int q = 0; q < 4; ++q) v += S.coef[p * 4 + q] * sxp[p] * syq[q];
         return v;
       }

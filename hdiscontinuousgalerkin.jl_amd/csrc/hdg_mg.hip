@@ -234,6 +234,45 @@ __global__ void mg_cmask_kernel(MgLevel F, MgLevel Cc) {
   Cc.mask[C] = F.mask[(int64_t)(2 * J) * (F.nx + 1) + 2 * I];
 }
 
+// S P0 restricted to each row: sp[j N + r] = sum over the free gathered
+// columns (g, b < 2) of S[r, (g, b)] P0[(g, b), fq4[4 f + j]], where
+// P0[(g, 0), lo] = P0[(g, 0), hi] = 1/2 and P0[(g, 1), hi] = -P0[(g, 1), lo]
+// = 1/(2 sqrt 3).  Every face of the two cells of face f has its end points
+// among fq4[4f .. 4f+3].
+template <int NT>
+__global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T, MgData M) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= T.N) return;
+  const int64_t f = r / NT;
+  int q[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) q[j] = M.fq4[4 * f + j];
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
+  if (!T.dmask[r]) {
+    for (int sl = 0; sl < kSlots; ++sl) {
+      const int64_t g = P.bcol[f * kSlots + sl];
+      const int lo = M.fv[2 * g], hi = M.fv[2 * g + 1];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (T.dmask[g * NT + b]) continue;
+        const double sv = -T.vals[(int64_t)(sl * NT + b) * T.N + r];
+        const double cl = b == 0 ? 0.5 : -kS3, ch = b == 0 ? 0.5 : kS3;
+        bool dl = false, dh = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bool ml = !dl && q[j] == lo, mh = !dh && q[j] == hi;
+          w[j] = ml ? fma(sv, cl, w[j]) : w[j];
+          w[j] = mh ? fma(sv, ch, w[j]) : w[j];
+          dl = dl || ml;
+          dh = dh || mh;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) M.sp[(int64_t)j * T.N + r] = w[j];
+}
+
 // ---------------------------------------------- power iteration for omega
 __device__ inline double hash01(uint64_t i) {
   uint64_t z = i * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL;
@@ -412,32 +451,33 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(CgVectors v, MgData M
   L.r[vv] = s;
 }
 
-// e0 += P0 e1 on the modes 0 and 1 of every free face.
+// z = e0 + P0 e1 + omega Dinv (res0 - (S P0) e1): the post-smoothing step.
+// S (e0 + P0 e1) = S e0 + (S P0) e1 and res0 = r - S e0, so the trace matrix
+// is not read again; partial (r, z) for the next iteration.
 template <int NT>
-__global__ __launch_bounds__(256) void mg_prolong0_kernel(TraceSystem T, CgVectors v, MgData M) {
-  if (v.st->done) return;
-  const MgLevel L = M.lev[1];
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= T.nf) return;
-  const double ea = L.e[M.fv[2 * f]], eb = L.e[M.fv[2 * f + 1]];
-  if (!T.dmask[f * NT]) M.e0[f * NT] += 0.5 * (ea + eb);
-  if (!T.dmask[f * NT + 1]) M.e0[f * NT + 1] += kS3 * (eb - ea);
-}
-
-// z = e0 + omega Dinv (r - S e0); partial (r, z) for the next iteration.
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(DevPattern P, TraceSystem T, CgVectors v, MgData M,
-                                                            int i) {
+__global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceSystem T, CgVectors v, MgData M, int i) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
   if (v.st->done) return;
   const double om = v.st->omega;
+  const double* __restrict__ e1 = M.lev[1].e;
   double prz = 0.0;
   for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
     const int64_t r = r0 + threadIdx.x;
-    double rr = 0.0;
-    if (r < T.N) rr = T.dmask[r] ? 0.0 : v.r[r] - s_row<NT>(P, T, M.e0, r);
+    double rr = 0.0, pe = 0.0;
+    if (r < T.N && !T.dmask[r]) {
+      const int64_t f = r / NT;
+      double s = M.res0[r];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s = fma(-M.sp[(int64_t)j * T.N + r], e1[M.fq4[4 * f + j]], s);
+      rr = s;
+      const int a = (int)(r - f * NT);
+      if (a < 2) {
+        const double el = e1[M.fv[2 * f]], eh = e1[M.fv[2 * f + 1]];
+        pe = a == 0 ? 0.5 * (el + eh) : kS3 * (eh - el);
+      }
+    }
     rs[threadIdx.x] = rr;
     __syncthreads();
     if (r < T.N) {
@@ -445,7 +485,7 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(DevPattern P, Trace
       double z = 0.0;
 #pragma unroll
       for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
-      z = fma(om, z, M.e0[r]);
+      z = fma(om, z, M.e0[r] + pe);
       v.z[r] = z;
       prz = fma(v.r[r], z, prz);
     }
@@ -456,24 +496,41 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(DevPattern P, Trace
 }
 
 // ------------------------------------------------------- vertex levels
-// e = om dinv r
-__global__ __launch_bounds__(256) void v_first_kernel(MgLevel L, double om, const CgState* st) {
+// (A x)_v on a level, with x given as a function of (index, i, j), so that
+// fused kernels can form x on the fly.
+template <class X>
+__device__ inline double stencil_apply(const MgLevel& L, int64_t v, int i, int j, const X& xv) {
+  const int64_t row = L.nx + 1;
+  double s = L.a[v] * xv(v, i, j);
+  if (i < L.nx) s = fma(L.a[L.n + v], xv(v + 1, i + 1, j), s);
+  if (i > 0) s = fma(L.a[2 * L.n + v], xv(v - 1, i - 1, j), s);
+  if (j < L.ny) s = fma(L.a[3 * L.n + v], xv(v + row, i, j + 1), s);
+  if (j > 0) s = fma(L.a[4 * L.n + v], xv(v - row, i, j - 1), s);
+  if (i < L.nx && j > 0) s = fma(L.a[5 * L.n + v], xv(v + 1 - row, i + 1, j - 1), s);
+  if (i > 0 && j < L.ny) s = fma(L.a[6 * L.n + v], xv(v - 1 + row, i - 1, j + 1), s);
+  return s;
+}
+
+// linear interpolation of the coarse vector ec at fine node (x, y)
+__device__ inline double prolong_at(const MgLevel& C, const double* ec, int x, int y) {
+  const int64_t cs = C.nx + 1;
+  const int hx = x >> 1, hy = y >> 1;
+  const int64_t c0 = (int64_t)hy * cs + hx;
+  if (!(x & 1) && !(y & 1)) return ec[c0];
+  if (!(y & 1)) return 0.5 * (ec[c0] + ec[c0 + 1]);
+  if (!(x & 1)) return 0.5 * (ec[c0] + ec[c0 + cs]);
+  return 0.5 * (ec[c0 + 1] + ec[c0 + cs]);
+}
+
+// two pre-smoothing sweeps from zero in one pass: e1 = om dinv r is formed at
+// the neighbours on the fly, e = e1 + om dinv (r - A e1)
+__global__ __launch_bounds__(256) void v_pre_kernel(MgLevel L, double om, const CgState* st) {
   if (st->done) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= L.n) return;
-  L.e[v] = om * L.dinv[v] * L.r[v];
-}
-
-__device__ inline double v_apply(const MgLevel& L, const double* x, int64_t v) {
   const int i = (int)(v % (L.nx + 1)), j = (int)(v / (L.nx + 1));
-  double s = L.a[v] * x[v];
-  if (i + 1 <= L.nx) s = fma(L.a[1 * L.n + v], x[v + 1], s);
-  if (i - 1 >= 0) s = fma(L.a[2 * L.n + v], x[v - 1], s);
-  if (j + 1 <= L.ny) s = fma(L.a[3 * L.n + v], x[v + L.nx + 1], s);
-  if (j - 1 >= 0) s = fma(L.a[4 * L.n + v], x[v - L.nx - 1], s);
-  if (i + 1 <= L.nx && j - 1 >= 0) s = fma(L.a[5 * L.n + v], x[v + 1 - L.nx - 1], s);
-  if (i - 1 >= 0 && j + 1 <= L.ny) s = fma(L.a[6 * L.n + v], x[v - 1 + L.nx + 1], s);
-  return s;
+  auto e1 = [&](int64_t u, int, int) { return om * L.dinv[u] * L.r[u]; };
+  L.e[v] = e1(v, i, j) + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, e1));
 }
 
 // out = in + om dinv (r - A in)
@@ -482,55 +539,159 @@ __global__ __launch_bounds__(256) void v_smooth_kernel(MgLevel L, const double* 
   if (st->done) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= L.n) return;
-  out[v] = in[v] + om * L.dinv[v] * (L.r[v] - v_apply(L, in, v));
+  const int i = (int)(v % (L.nx + 1)), j = (int)(v / (L.nx + 1));
+  auto xv = [&](int64_t u, int, int) { return in[u]; };
+  out[v] = in[v] + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, xv));
 }
 
-// coarse r = R (r - A e) over the fine support of each coarse node
+// fine residual r - A e at (fx, fy); 0 outside the grid and on fixed nodes
+__device__ inline double fine_res(const MgLevel& F, int fx, int fy) {
+  if (fx < 0 || fx > F.nx || fy < 0 || fy > F.ny) return 0.0;
+  const int64_t fi = (int64_t)fy * (F.nx + 1) + fx;
+  if (F.mask[fi]) return 0.0;
+  auto ev = [&](int64_t u, int, int) { return F.e[u]; };
+  return F.r[fi] - stencil_apply(F, fi, fx, fy, ev);
+}
+
+// coarse r = R (r - A e): full weight at the fine node (2I, 2J), half at its
+// six stencil neighbours (the transpose of prolong_at)
 __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, const CgState* st) {
   if (st->done) return;
   const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (C >= Cc.n) return;
   double s = 0.0;
   if (!Cc.mask[C]) {
-    const int I = (int)(C % (Cc.nx + 1)), J = (int)(C / (Cc.nx + 1));
-    const int sx[7] = {0, 1, -1, 0, 0, 1, -1}, sy[7] = {0, 0, 0, 1, -1, -1, 1};
-    for (int t = 0; t < 7; ++t) {
-      const int fx = 2 * I + sx[t], fy = 2 * J + sy[t];
-      if (fx < 0 || fx > F.nx || fy < 0 || fy > F.ny) continue;
-      const int64_t fi = (int64_t)fy * (F.nx + 1) + fx;
-      if (F.mask[fi]) continue;
-      s += (t == 0 ? 1.0 : 0.5) * (F.r[fi] - v_apply(F, F.e, fi));
-    }
+    const int x = 2 * (int)(C % (Cc.nx + 1)), y = 2 * (int)(C / (Cc.nx + 1));
+    const double nb = fine_res(F, x + 1, y) + fine_res(F, x - 1, y) + fine_res(F, x, y + 1) +
+                      fine_res(F, x, y - 1) + fine_res(F, x + 1, y - 1) + fine_res(F, x - 1, y + 1);
+    s = fma(0.5, nb, fine_res(F, x, y));
   }
   Cc.r[C] = s;
 }
 
-// fine e += P coarse e
-__global__ __launch_bounds__(256) void v_prolong_kernel(MgLevel F, MgLevel Cc, const CgState* st) {
+// prolongation and the first post-smoothing sweep in one pass:
+// e' = e + P ec on free nodes (formed at the neighbours on the fly),
+// t = e' + om dinv (r - A e')
+__global__ __launch_bounds__(256) void v_post_kernel(MgLevel F, MgLevel Cc, double om, const CgState* st) {
   if (st->done) return;
-  const int64_t fi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (fi >= F.n || F.mask[fi]) return;
-  const int x = (int)(fi % (F.nx + 1)), y = (int)(fi / (F.nx + 1));
-  const int cs = Cc.nx + 1;
-  double add;
-  if (!(x & 1) && !(y & 1)) add = Cc.e[(int64_t)(y / 2) * cs + x / 2];
-  else if ((x & 1) && !(y & 1)) add = 0.5 * (Cc.e[(int64_t)(y / 2) * cs + (x - 1) / 2] + Cc.e[(int64_t)(y / 2) * cs + (x + 1) / 2]);
-  else if (!(x & 1) && (y & 1)) add = 0.5 * (Cc.e[(int64_t)((y - 1) / 2) * cs + x / 2] + Cc.e[(int64_t)((y + 1) / 2) * cs + x / 2]);
-  else add = 0.5 * (Cc.e[(int64_t)((y - 1) / 2) * cs + (x + 1) / 2] + Cc.e[(int64_t)((y + 1) / 2) * cs + (x - 1) / 2]);
-  F.e[fi] += add;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= F.n) return;
+  const int i = (int)(v % (F.nx + 1)), j = (int)(v / (F.nx + 1));
+  auto ep = [&](int64_t u, int x, int y) { return F.mask[u] ? F.e[u] : F.e[u] + prolong_at(Cc, Cc.e, x, y); };
+  F.t[v] = ep(v, i, j) + om * F.dinv[v] * (F.r[v] - stencil_apply(F, v, i, j, ep));
 }
 
-// Coarsest level: `sweeps` damped Jacobi sweeps from zero in one block.
+// Coarsest level too large for v_tail_kernel: `sweeps` damped Jacobi sweeps
+// from zero in one block, vectors in global memory.
 __global__ __launch_bounds__(1024) void v_coarsest_kernel(MgLevel L, double om, int sweeps, const CgState* st) {
   if (st->done) return;
   for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x) L.e[v] = om * L.dinv[v] * L.r[v];
   __syncthreads();
   for (int it = 1; it < sweeps; ++it) {
-    for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x)
-      L.t[v] = L.e[v] + om * L.dinv[v] * (L.r[v] - v_apply(L, L.e, v));
+    for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x) {
+      const int i = (int)(v % (L.nx + 1)), j = (int)(v / (L.nx + 1));
+      auto xv = [&](int64_t u, int, int) { return L.e[u]; };
+      L.t[v] = L.e[v] + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, xv));
+    }
     __syncthreads();
     for (int64_t v = threadIdx.x; v < L.n; v += blockDim.x) L.e[v] = L.t[v];
     __syncthreads();
+  }
+}
+
+// one Jacobi sweep of level L inside the workgroup: out = in + om dinv (r - A in)
+__device__ inline void tail_sweep(const MgLevel& L, const double* in, double* out, const double* r, double om) {
+  for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) {
+    const int i = v % (L.nx + 1), j = v / (L.nx + 1);
+    auto xv = [&](int64_t u, int, int) { return in[u]; };
+    out[v] = in[v] + om * L.dinv[v] * (r[v] - stencil_apply(L, v, i, j, xv));
+  }
+}
+
+// Levels lt..nlev-1 of the V-cycle in one workgroup, every vector in LDS
+// (the level sizes sum to at most kTailNodes): reads lev[lt].r, writes
+// lev[lt].e.  Same sweeps as the per-level kernels: two pre-smoothing
+// sweeps, restriction of the residual, coarse_sweeps sweeps on the coarsest
+// level, prolongation and two post-smoothing sweeps.
+__global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const CgState* st) {
+  __shared__ double pool[3 * kTailNodes];
+  if (st->done) return;
+  const double om = M.omega_v;
+  double* const E = pool;                    // scratch / residual
+  double* const Tt = pool + kTailNodes;      // iterate
+  double* const R = pool + 2 * kTailNodes;   // right-hand sides
+  {
+    const MgLevel L = M.lev[lt];
+    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) R[v] = L.r[v];
+  }
+  __syncthreads();
+  int o = 0;
+  for (int l = lt; l < M.nlev - 1; ++l) {
+    const MgLevel L = M.lev[l];
+    const MgLevel Cc = M.lev[l + 1];
+    double *e = E + o, *t = Tt + o, *r = R + o;
+    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) e[v] = om * L.dinv[v] * r[v];
+    __syncthreads();
+    tail_sweep(L, e, t, r, om);
+    __syncthreads();
+    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) {
+      const int i = v % (L.nx + 1), j = v / (L.nx + 1);
+      auto xv = [&](int64_t u, int, int) { return t[u]; };
+      e[v] = L.mask[v] ? 0.0 : r[v] - stencil_apply(L, v, i, j, xv);
+    }
+    __syncthreads();
+    double* rc = R + o + (int)L.n;
+    for (int C = threadIdx.x; C < (int)Cc.n; C += blockDim.x) {
+      double s = 0.0;
+      if (!Cc.mask[C]) {
+        const int x = 2 * (C % (Cc.nx + 1)), y = 2 * (C / (Cc.nx + 1));
+        auto res = [&](int fx, int fy) {
+          return (fx < 0 || fx > L.nx || fy < 0 || fy > L.ny) ? 0.0 : e[fy * (L.nx + 1) + fx];
+        };
+        const double nb = res(x + 1, y) + res(x - 1, y) + res(x, y + 1) + res(x, y - 1) + res(x + 1, y - 1) +
+                          res(x - 1, y + 1);
+        s = fma(0.5, nb, res(x, y));
+      }
+      rc[C] = s;
+    }
+    __syncthreads();
+    o += (int)L.n;
+  }
+  // coarsest level
+  const double* ec;
+  {
+    const MgLevel L = M.lev[M.nlev - 1];
+    double *e = E + o, *t = Tt + o;
+    const double* r = R + o;
+    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) t[v] = om * L.dinv[v] * r[v];
+    __syncthreads();
+    for (int it = 1; it < M.coarse_sweeps; ++it) {
+      tail_sweep(L, t, e, r, om);
+      __syncthreads();
+      double* sw = e;
+      e = t;
+      t = sw;
+    }
+    ec = t;
+  }
+  for (int l = M.nlev - 2; l >= lt; --l) {
+    const MgLevel L = M.lev[l];
+    const MgLevel Cc = M.lev[l + 1];
+    o -= (int)L.n;
+    double *e = E + o, *t = Tt + o;
+    const double* r = R + o;
+    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x)
+      if (!L.mask[v]) t[v] += prolong_at(Cc, ec, v % (L.nx + 1), v / (L.nx + 1));
+    __syncthreads();
+    tail_sweep(L, t, e, r, om);
+    __syncthreads();
+    tail_sweep(L, e, t, r, om);
+    __syncthreads();
+    ec = t;
+  }
+  {
+    const MgLevel L = M.lev[lt];
+    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) L.e[v] = ec[v];
   }
 }
 
@@ -550,24 +711,26 @@ int grid1(int64_t n, int t = 256) {
     default: return (int)hipErrorInvalidValue; \
   }
 
-int vcycle(MgData& M, int l, const CgState* st, hipStream_t s) {
-  MgLevel& L = M.lev[l];
+// V-cycle from level l down; the result is left in lev[l].e.
+int vcycle(const MgData& M, int l, const CgState* st, hipStream_t s) {
+  if (l >= M.tail) {
+    hipLaunchKernelGGL(v_tail_kernel, dim3(1), dim3(1024), 0, s, M, l, st);
+    return (int)hipGetLastError();
+  }
+  const MgLevel& L = M.lev[l];
   const double om = M.omega_v;
   if (l == M.nlev - 1) {
     hipLaunchKernelGGL(v_coarsest_kernel, dim3(1), dim3(1024), 0, s, L, om, M.coarse_sweeps, st);
     return (int)hipGetLastError();
   }
+  const MgLevel& Cc = M.lev[l + 1];
   const int g = grid1(L.n);
-  hipLaunchKernelGGL(v_first_kernel, dim3(g), dim3(256), 0, s, L, om, st);
-  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, L.e, L.t, om, st);
-  std::swap(L.e, L.t);
-  MgLevel& Cc = M.lev[l + 1];
+  hipLaunchKernelGGL(v_pre_kernel, dim3(g), dim3(256), 0, s, L, om, st);
   hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(Cc.n)), dim3(256), 0, s, L, Cc, st);
   int e = vcycle(M, l + 1, st, s);
   if (e) return e;
-  hipLaunchKernelGGL(v_prolong_kernel, dim3(g), dim3(256), 0, s, L, Cc, st);
-  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, L.e, L.t, om, st);
-  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, L.t, L.e, om, st);
+  hipLaunchKernelGGL(v_post_kernel, dim3(g), dim3(256), 0, s, L, Cc, om, st);
+  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, (const double*)L.t, L.e, om, st);
   return (int)hipGetLastError();
 }
 
@@ -580,6 +743,9 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPa
   MG_NT_DISPATCH(T.nt, CELL)
 #undef CELL
   hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, mesh, M);
+#define SPK(NT) hipLaunchKernelGGL(mg_sp_kernel<NT>, dim3(grid1(T.N)), dim3(256), 0, s, P, T, M);
+  MG_NT_DISPATCH(T.nt, SPK)
+#undef SPK
   for (int l = 1; l + 1 < M.nlev; ++l) {
     hipLaunchKernelGGL(mg_cmask_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
     hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
@@ -634,10 +800,7 @@ int launch_mg_precond(const DevPattern& P, const TraceSystem& T, CgVectors& v, M
 #undef RST
   int e = vcycle(M, 1, v.st, s);
   if (e) return e;
-#define PRO(NT) hipLaunchKernelGGL(mg_prolong0_kernel<NT>, dim3(grid1(T.nf)), dim3(256), 0, s, T, v, M);
-  MG_NT_DISPATCH(T.nt, PRO)
-#undef PRO
-#define SMO(NT) hipLaunchKernelGGL(mg_smooth0_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v, M, i);
+#define SMO(NT) hipLaunchKernelGGL(mg_smooth0_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, i);
   MG_NT_DISPATCH(T.nt, SMO)
 #undef SMO
   return (int)hipGetLastError();
