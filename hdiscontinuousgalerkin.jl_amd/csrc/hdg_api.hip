@@ -166,6 +166,7 @@ void free_mesh(hdg_ctx* c) {
   dfree(c, c->mg.vf_ptr);
   dfree(c, c->mg.vf_idx);
   dfree(c, c->mg.fq4);
+  dfree(c, c->mg.tailpack);
   for (int l = 0; l < kMaxLevels; ++l) {
     MgLevel& L = c->mg.lev[l];
     dfree(c, L.a);
@@ -270,7 +271,7 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     TRY(dalloc(c, &L.e, L.n));
     TRY(dalloc(c, &L.t, L.n));
     ++l;
-    if (lx % 2 || ly % 2 || std::min(lx, ly) < 8 || l >= kMaxLevels) break;
+    if (lx % 2 || ly % 2 || std::min(lx, ly) < 4 || l >= kMaxLevels) break;
     lx /= 2;
     ly /= 2;
   }
@@ -282,6 +283,11 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     acc += M.lev[t].n;
     if (acc > kTailNodes) break;
     M.tail = t;
+  }
+  if (M.tail < M.nlev) {
+    int64_t tn = 0;
+    for (int t = M.tail; t < M.nlev; ++t) tn += M.lev[t].n;
+    TRY(dalloc(c, &M.tailpack, 8 * tn));
   }
   return HDG_OK;
 }

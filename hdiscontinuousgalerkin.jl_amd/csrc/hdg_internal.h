@@ -106,8 +106,9 @@ struct MgLevel {
   double *r = nullptr, *e = nullptr, *t = nullptr;
 };
 constexpr int kMaxLevels = 16;
-// node budget of the levels solved inside one workgroup: three vectors of
-// this many doubles live in LDS (36 KB)
+// node budget of the levels solved inside one workgroup: their stencils,
+// diagonal inverses and three vectors live in LDS (11 doubles per node,
+// 132 KB of the 160 KB of a CU)
 constexpr int kTailNodes = 1536;
 struct MgData {
   int nlev = 0;                  // levels 1..nlev-1 are vertex grids
@@ -126,6 +127,8 @@ struct MgData {
   double* sp = nullptr;          // [4*N] S P0 restricted to the rows: sp[j*N + r] couples
                                  // row r with vertex fq4[4*face(r) + j]
   int tail = kMaxLevels;         // levels tail..nlev-1 run in one workgroup (LDS)
+  double* tailpack = nullptr;    // stencils (7 n) and dinv (n) of levels tail.. back to back,
+                                 // the LDS image the tail kernel loads
   double omega_v = 0.8;          // vertex-level Jacobi weight
   double omega_factor = 1.5;     // trace-level weight = factor / lambda_max(Dinv S)
   int coarse_sweeps = 20;

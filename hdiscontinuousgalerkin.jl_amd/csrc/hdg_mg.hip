@@ -225,6 +225,17 @@ __global__ void mg_rap_kernel(MgLevel F, MgLevel Cc) {
   Cc.dinv[C] = cm ? 0.0 : 1.0 / st[0];
 }
 
+// LDS image of the tail levels: per level its stencils (7 n) then dinv (n).
+__global__ void mg_tailpack_kernel(MgData M) {
+  int o = 0;
+  for (int l = M.tail; l < M.nlev; ++l) {
+    const int n = (int)M.lev[l].n;
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < 8 * n; u += gridDim.x * blockDim.x)
+      M.tailpack[o + u] = u < 7 * n ? M.lev[l].a[u] : M.lev[l].dinv[u - 7 * n];
+    o += 8 * n;
+  }
+}
+
 // Coarse masks (a coarse node is fixed when its fine node (2I, 2J) is),
 // written before mg_rap_kernel reads the masks of neighbouring nodes.
 __global__ void mg_cmask_kernel(MgLevel F, MgLevel Cc) {
@@ -554,19 +565,25 @@ __device__ inline double fine_res(const MgLevel& F, int fx, int fy) {
 }
 
 // coarse r = R (r - A e): full weight at the fine node (2I, 2J), half at its
-// six stencil neighbours (the transpose of prolong_at)
+// six stencil neighbours (the transpose of prolong_at).  Eight lanes per
+// coarse node: lane t < 7 forms the fine residual at stencil offset t, and a
+// fixed shuffle tree sums them.
 __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, const CgState* st) {
   if (st->done) return;
-  const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (C >= Cc.n) return;
-  double s = 0.0;
-  if (!Cc.mask[C]) {
-    const int x = 2 * (int)(C % (Cc.nx + 1)), y = 2 * (int)(C / (Cc.nx + 1));
-    const double nb = fine_res(F, x + 1, y) + fine_res(F, x - 1, y) + fine_res(F, x, y + 1) +
-                      fine_res(F, x, y - 1) + fine_res(F, x + 1, y - 1) + fine_res(F, x - 1, y + 1);
-    s = fma(0.5, nb, fine_res(F, x, y));
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t C = g >> 3;
+  const int t = (int)(g & 7);
+  double val = 0.0;
+  if (C < Cc.n && t < 7 && !Cc.mask[C]) {
+    const int x = 2 * (int)(C % (Cc.nx + 1)) + (t == 1 || t == 5 ? 1 : (t == 2 || t == 6 ? -1 : 0));
+    const int y = 2 * (int)(C / (Cc.nx + 1)) + (t == 3 || t == 6 ? 1 : (t == 4 || t == 5 ? -1 : 0));
+    val = fine_res(F, x, y);
+    if (t) val *= 0.5;
   }
-  Cc.r[C] = s;
+  val += __shfl_xor(val, 1, 8);
+  val += __shfl_xor(val, 2, 8);
+  val += __shfl_xor(val, 4, 8);
+  if (t == 0 && C < Cc.n) Cc.r[C] = val;
 }
 
 // prolongation and the first post-smoothing sweep in one pass:
@@ -599,51 +616,88 @@ __global__ __launch_bounds__(1024) void v_coarsest_kernel(MgLevel L, double om, 
   }
 }
 
-// one Jacobi sweep of level L inside the workgroup: out = in + om dinv (r - A in)
-__device__ inline void tail_sweep(const MgLevel& L, const double* in, double* out, const double* r, double om) {
-  for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) {
+// Level of the one-workgroup tail: stencils, diagonal inverse and vectors
+// all in LDS.  A fixed node has dinv = 0 (its stencil row is the identity).
+struct TailLevel {
+  int nx, ny, n;
+  const double* a;     // a[s*n + v]
+  const double* dinv;
+};
+
+__device__ inline double tail_apply(const TailLevel& L, const double* x, int v, int i, int j) {
+  const int row = L.nx + 1;
+  double s = L.a[v] * x[v];
+  if (i < L.nx) s = fma(L.a[L.n + v], x[v + 1], s);
+  if (i > 0) s = fma(L.a[2 * L.n + v], x[v - 1], s);
+  if (j < L.ny) s = fma(L.a[3 * L.n + v], x[v + row], s);
+  if (j > 0) s = fma(L.a[4 * L.n + v], x[v - row], s);
+  if (i < L.nx && j > 0) s = fma(L.a[5 * L.n + v], x[v + 1 - row], s);
+  if (i > 0 && j < L.ny) s = fma(L.a[6 * L.n + v], x[v - 1 + row], s);
+  return s;
+}
+
+// one Jacobi sweep: out = in + om dinv (r - A in)
+__device__ inline void tail_sweep(const TailLevel& L, const double* in, double* out, const double* r, double om) {
+  for (int v = threadIdx.x; v < L.n; v += blockDim.x) {
     const int i = v % (L.nx + 1), j = v / (L.nx + 1);
-    auto xv = [&](int64_t u, int, int) { return in[u]; };
-    out[v] = in[v] + om * L.dinv[v] * (r[v] - stencil_apply(L, v, i, j, xv));
+    out[v] = in[v] + om * L.dinv[v] * (r[v] - tail_apply(L, in, v, i, j));
   }
 }
 
-// Levels lt..nlev-1 of the V-cycle in one workgroup, every vector in LDS
-// (the level sizes sum to at most kTailNodes): reads lev[lt].r, writes
-// lev[lt].e.  Same sweeps as the per-level kernels: two pre-smoothing
-// sweeps, restriction of the residual, coarse_sweeps sweeps on the coarsest
-// level, prolongation and two post-smoothing sweeps.
+// Levels lt..nlev-1 of the V-cycle in one workgroup (their sizes sum to at
+// most kTailNodes), everything in LDS: reads lev[lt].r, writes lev[lt].e.
+// Same sweeps as the per-level kernels: two pre-smoothing sweeps,
+// restriction of the residual, coarse_sweeps sweeps on the coarsest level,
+// prolongation and two post-smoothing sweeps.
 __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const CgState* st) {
-  __shared__ double pool[3 * kTailNodes];
+  __shared__ double pool[11 * kTailNodes];
   if (st->done) return;
   const double om = M.omega_v;
   double* const E = pool;                    // scratch / residual
   double* const Tt = pool + kTailNodes;      // iterate
   double* const R = pool + 2 * kTailNodes;   // right-hand sides
+  double* const SA = pool + 3 * kTailNodes;  // stencils (7 n) and dinv (n) per level
+  {
+    int tn = 0;
+    for (int l = lt; l < M.nlev; ++l) tn += (int)M.lev[l].n;
+    const double* __restrict__ src = M.tailpack;
+#pragma unroll 8
+    for (int u = threadIdx.x; u < 8 * tn; u += blockDim.x) SA[u] = src[u];
+  }
   {
     const MgLevel L = M.lev[lt];
     for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) R[v] = L.r[v];
   }
   __syncthreads();
+  // level l's stencil block starts at 8 * (nodes of levels lt..l-1); its
+  // vectors at the same node offset o
+  auto lvl = [&](int l, int o) {
+    TailLevel L;
+    L.nx = M.lev[l].nx;
+    L.ny = M.lev[l].ny;
+    L.n = (int)M.lev[l].n;
+    L.a = SA + 8 * o;
+    L.dinv = SA + 8 * o + 7 * L.n;
+    return L;
+  };
   int o = 0;
   for (int l = lt; l < M.nlev - 1; ++l) {
-    const MgLevel L = M.lev[l];
-    const MgLevel Cc = M.lev[l + 1];
+    const TailLevel L = lvl(l, o);
+    const TailLevel Cc = lvl(l + 1, o + L.n);
     double *e = E + o, *t = Tt + o, *r = R + o;
-    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) e[v] = om * L.dinv[v] * r[v];
+    for (int v = threadIdx.x; v < L.n; v += blockDim.x) e[v] = om * L.dinv[v] * r[v];
     __syncthreads();
     tail_sweep(L, e, t, r, om);
     __syncthreads();
-    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) {
+    for (int v = threadIdx.x; v < L.n; v += blockDim.x) {
       const int i = v % (L.nx + 1), j = v / (L.nx + 1);
-      auto xv = [&](int64_t u, int, int) { return t[u]; };
-      e[v] = L.mask[v] ? 0.0 : r[v] - stencil_apply(L, v, i, j, xv);
+      e[v] = L.dinv[v] == 0.0 ? 0.0 : r[v] - tail_apply(L, t, v, i, j);
     }
     __syncthreads();
-    double* rc = R + o + (int)L.n;
-    for (int C = threadIdx.x; C < (int)Cc.n; C += blockDim.x) {
+    double* rc = R + o + L.n;
+    for (int C = threadIdx.x; C < Cc.n; C += blockDim.x) {
       double s = 0.0;
-      if (!Cc.mask[C]) {
+      if (Cc.dinv[C] != 0.0) {
         const int x = 2 * (C % (Cc.nx + 1)), y = 2 * (C / (Cc.nx + 1));
         auto res = [&](int fx, int fy) {
           return (fx < 0 || fx > L.nx || fy < 0 || fy > L.ny) ? 0.0 : e[fy * (L.nx + 1) + fx];
@@ -655,15 +709,15 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
       rc[C] = s;
     }
     __syncthreads();
-    o += (int)L.n;
+    o += L.n;
   }
   // coarsest level
-  const double* ec;
+  double* ec;
   {
-    const MgLevel L = M.lev[M.nlev - 1];
+    const TailLevel L = lvl(M.nlev - 1, o);
     double *e = E + o, *t = Tt + o;
     const double* r = R + o;
-    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) t[v] = om * L.dinv[v] * r[v];
+    for (int v = threadIdx.x; v < L.n; v += blockDim.x) t[v] = om * L.dinv[v] * r[v];
     __syncthreads();
     for (int it = 1; it < M.coarse_sweeps; ++it) {
       tail_sweep(L, t, e, r, om);
@@ -675,13 +729,24 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
     ec = t;
   }
   for (int l = M.nlev - 2; l >= lt; --l) {
-    const MgLevel L = M.lev[l];
-    const MgLevel Cc = M.lev[l + 1];
-    o -= (int)L.n;
+    const int oc = o;
+    o -= (int)M.lev[l].n;
+    const TailLevel L = lvl(l, o);
+    const TailLevel Cc = lvl(l + 1, oc);
     double *e = E + o, *t = Tt + o;
     const double* r = R + o;
-    for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x)
-      if (!L.mask[v]) t[v] += prolong_at(Cc, ec, v % (L.nx + 1), v / (L.nx + 1));
+    const int cs = Cc.nx + 1;
+    for (int v = threadIdx.x; v < L.n; v += blockDim.x) {
+      if (L.dinv[v] == 0.0) continue;
+      const int x = v % (L.nx + 1), y = v / (L.nx + 1);
+      const int c0 = (y >> 1) * cs + (x >> 1);
+      double add;
+      if (!(x & 1) && !(y & 1)) add = ec[c0];
+      else if (!(y & 1)) add = 0.5 * (ec[c0] + ec[c0 + 1]);
+      else if (!(x & 1)) add = 0.5 * (ec[c0] + ec[c0 + cs]);
+      else add = 0.5 * (ec[c0 + 1] + ec[c0 + cs]);
+      t[v] += add;
+    }
     __syncthreads();
     tail_sweep(L, t, e, r, om);
     __syncthreads();
@@ -726,7 +791,7 @@ int vcycle(const MgData& M, int l, const CgState* st, hipStream_t s) {
   const MgLevel& Cc = M.lev[l + 1];
   const int g = grid1(L.n);
   hipLaunchKernelGGL(v_pre_kernel, dim3(g), dim3(256), 0, s, L, om, st);
-  hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(Cc.n)), dim3(256), 0, s, L, Cc, st);
+  hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(8 * Cc.n)), dim3(256), 0, s, L, Cc, st);
   int e = vcycle(M, l + 1, st, s);
   if (e) return e;
   hipLaunchKernelGGL(v_post_kernel, dim3(g), dim3(256), 0, s, L, Cc, om, st);
@@ -750,6 +815,7 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPa
     hipLaunchKernelGGL(mg_cmask_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
     hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
   }
+  if (M.tail < M.nlev) hipLaunchKernelGGL(mg_tailpack_kernel, dim3(16), dim3(256), 0, s, M);
   // omega = cfac / lambda_max(Dinv S), power iteration on free dofs; the
   // CG vectors x, r serve as scratch before the solve initialises them
   const int nb = v.nblocks;
