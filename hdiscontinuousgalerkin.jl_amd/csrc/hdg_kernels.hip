@@ -320,36 +320,38 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
       bool bad = !(g.detJ > 0.0);
       bool used = !(ls < M);
       int mycol = 0;
+      double mypiv = 1.0;
       // Gauss-Jordan on the first M columns with partial pivoting among
-      // the rows not yet used as pivots (implicit row exchanges).
+      // the rows not yet used as pivots (implicit row exchanges).  Pivot rows
+      // stay unscaled (each row is divided by its own pivot at the end), so
+      // an elimination step is one fma per column.  The pivot search reduces
+      // a 32-bit key: the upper word of |a_ij| with its low log2(SEG) bits
+      // replaced by the lane (ties and near-ties go to the lower lane).
+      constexpr int LB = SEG == 16 ? 4 : (SEG == 32 ? 5 : 6);
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        double cand = used ? -1.0 : fabs(row[j]);
-        int bl = ls;
+        int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~(unsigned)(SEG - 1)) |
+                                    (unsigned)(SEG - 1 - ls));
 #pragma unroll
-        for (int off = SEG / 2; off > 0; off >>= 1) {
-          const double oc = __shfl_xor(cand, off, SEG);
-          const int ob = __shfl_xor(bl, off, SEG);
-          if (oc > cand || (oc == cand && ob < bl)) {
-            cand = oc;
-            bl = ob;
-          }
-        }
+        for (int off = SEG / 2; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off, SEG));
+        const int bl = SEG - 1 - (key & ((1 << LB) - 1));
         const int src = sbase + bl;
         const double pv = seg_bcast<SEG>(row[j], src);
         bad |= !(fabs(pv) > thr);
-        const double inv = 1.0 / pv;
         const bool me = (ls == bl);
-        const double l = row[j] * inv;
+        const double f = me ? 0.0 : row[j] / pv;
 #pragma unroll
-        for (int c = j + 1; c < D::NC; ++c) {
-          const double pc = seg_bcast<SEG>(row[c], src);
-          row[c] = me ? pc * inv : fma(-l, pc, row[c]);
-        }
+        for (int c = j + 1; c < D::NC; ++c) row[c] = fma(-f, seg_bcast<SEG>(row[c], src), row[c]);
         if (me) {
           used = true;
           mycol = j;
+          mypiv = pv;
         }
+      }
+      if (ls < M) {
+        const double ip = 1.0 / mypiv;
+#pragma unroll
+        for (int c = M; c < D::NC; ++c) row[c] *= ip;
       }
       if (valid) {
         if (bad && ls == 0) atomicOr(a.err, g.detJ > 0.0 ? 2 : 1);
