@@ -132,6 +132,7 @@ struct MgData {
   double omega_v = 0.8;          // vertex-level Jacobi weight
   double omega_factor = 1.5;     // trace-level weight = factor / lambda_max(Dinv S)
   int coarse_sweeps = 20;
+  int vsweeps = 2;               // pre- and post-smoothing sweeps on the vertex levels (1 or 2)
 };
 
 // launchers (return hipError_t as int)

@@ -555,11 +555,17 @@ __global__ __launch_bounds__(256) void v_smooth_kernel(MgLevel L, const double* 
   out[v] = in[v] + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, xv));
 }
 
-// fine residual r - A e at (fx, fy); 0 outside the grid and on fixed nodes
-__device__ inline double fine_res(const MgLevel& F, int fx, int fy) {
+// fine residual r - A e at (fx, fy); 0 outside the grid and on fixed nodes.
+// With one_sweep the smoothed iterate is the single sweep from zero,
+// e = om dinv r, formed on the fly.
+__device__ inline double fine_res(const MgLevel& F, int fx, int fy, bool one_sweep, double om) {
   if (fx < 0 || fx > F.nx || fy < 0 || fy > F.ny) return 0.0;
   const int64_t fi = (int64_t)fy * (F.nx + 1) + fx;
   if (F.mask[fi]) return 0.0;
+  if (one_sweep) {
+    auto e1 = [&](int64_t u, int, int) { return om * F.dinv[u] * F.r[u]; };
+    return F.r[fi] - stencil_apply(F, fi, fx, fy, e1);
+  }
   auto ev = [&](int64_t u, int, int) { return F.e[u]; };
   return F.r[fi] - stencil_apply(F, fi, fx, fy, ev);
 }
@@ -568,7 +574,8 @@ __device__ inline double fine_res(const MgLevel& F, int fx, int fy) {
 // six stencil neighbours (the transpose of prolong_at).  Eight lanes per
 // coarse node: lane t < 7 forms the fine residual at stencil offset t, and a
 // fixed shuffle tree sums them.
-__global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, const CgState* st) {
+__global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, int one_sweep, double om,
+                                                         const CgState* st) {
   if (st->done) return;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t C = g >> 3;
@@ -577,7 +584,7 @@ __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, 
   if (C < Cc.n && t < 7 && !Cc.mask[C]) {
     const int x = 2 * (int)(C % (Cc.nx + 1)) + (t == 1 || t == 5 ? 1 : (t == 2 || t == 6 ? -1 : 0));
     const int y = 2 * (int)(C / (Cc.nx + 1)) + (t == 3 || t == 6 ? 1 : (t == 4 || t == 5 ? -1 : 0));
-    val = fine_res(F, x, y);
+    val = fine_res(F, x, y, one_sweep != 0, om);
     if (t) val *= 0.5;
   }
   val += __shfl_xor(val, 1, 8);
@@ -588,14 +595,24 @@ __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, 
 
 // prolongation and the first post-smoothing sweep in one pass:
 // e' = e + P ec on free nodes (formed at the neighbours on the fly),
-// t = e' + om dinv (r - A e')
-__global__ __launch_bounds__(256) void v_post_kernel(MgLevel F, MgLevel Cc, double om, const CgState* st) {
+// out = e' + om dinv (r - A e').  With one_sweep the pre-smoothed iterate is
+// e = om dinv r (formed on the fly) and out is the level's final e.
+__global__ __launch_bounds__(256) void v_post_kernel(MgLevel F, MgLevel Cc, int one_sweep, double om,
+                                                     const CgState* st) {
   if (st->done) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= F.n) return;
   const int i = (int)(v % (F.nx + 1)), j = (int)(v / (F.nx + 1));
-  auto ep = [&](int64_t u, int x, int y) { return F.mask[u] ? F.e[u] : F.e[u] + prolong_at(Cc, Cc.e, x, y); };
-  F.t[v] = ep(v, i, j) + om * F.dinv[v] * (F.r[v] - stencil_apply(F, v, i, j, ep));
+  if (one_sweep) {
+    auto ep = [&](int64_t u, int x, int y) {
+      const double e = om * F.dinv[u] * F.r[u];
+      return F.mask[u] ? e : e + prolong_at(Cc, Cc.e, x, y);
+    };
+    F.e[v] = ep(v, i, j) + om * F.dinv[v] * (F.r[v] - stencil_apply(F, v, i, j, ep));
+  } else {
+    auto ep = [&](int64_t u, int x, int y) { return F.mask[u] ? F.e[u] : F.e[u] + prolong_at(Cc, Cc.e, x, y); };
+    F.t[v] = ep(v, i, j) + om * F.dinv[v] * (F.r[v] - stencil_apply(F, v, i, j, ep));
+  }
 }
 
 // Coarsest level too large for v_tail_kernel: `sweeps` damped Jacobi sweeps
@@ -685,9 +702,13 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
     const TailLevel L = lvl(l, o);
     const TailLevel Cc = lvl(l + 1, o + L.n);
     double *e = E + o, *t = Tt + o, *r = R + o;
-    for (int v = threadIdx.x; v < L.n; v += blockDim.x) e[v] = om * L.dinv[v] * r[v];
-    __syncthreads();
-    tail_sweep(L, e, t, r, om);
+    if (M.vsweeps > 1) {
+      for (int v = threadIdx.x; v < L.n; v += blockDim.x) e[v] = om * L.dinv[v] * r[v];
+      __syncthreads();
+      tail_sweep(L, e, t, r, om);
+    } else {
+      for (int v = threadIdx.x; v < L.n; v += blockDim.x) t[v] = om * L.dinv[v] * r[v];
+    }
     __syncthreads();
     for (int v = threadIdx.x; v < L.n; v += blockDim.x) {
       const int i = v % (L.nx + 1), j = v / (L.nx + 1);
@@ -748,11 +769,17 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
       t[v] += add;
     }
     __syncthreads();
-    tail_sweep(L, t, e, r, om);
-    __syncthreads();
-    tail_sweep(L, e, t, r, om);
-    __syncthreads();
-    ec = t;
+    if (M.vsweeps > 1) {
+      tail_sweep(L, t, e, r, om);
+      __syncthreads();
+      tail_sweep(L, e, t, r, om);
+      __syncthreads();
+      ec = t;
+    } else {
+      tail_sweep(L, t, e, r, om);
+      __syncthreads();
+      ec = e;
+    }
   }
   {
     const MgLevel L = M.lev[lt];
@@ -790,12 +817,13 @@ int vcycle(const MgData& M, int l, const CgState* st, hipStream_t s) {
   }
   const MgLevel& Cc = M.lev[l + 1];
   const int g = grid1(L.n);
-  hipLaunchKernelGGL(v_pre_kernel, dim3(g), dim3(256), 0, s, L, om, st);
-  hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(8 * Cc.n)), dim3(256), 0, s, L, Cc, st);
+  const int one = M.vsweeps > 1 ? 0 : 1;
+  if (!one) hipLaunchKernelGGL(v_pre_kernel, dim3(g), dim3(256), 0, s, L, om, st);
+  hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(8 * Cc.n)), dim3(256), 0, s, L, Cc, one, om, st);
   int e = vcycle(M, l + 1, st, s);
   if (e) return e;
-  hipLaunchKernelGGL(v_post_kernel, dim3(g), dim3(256), 0, s, L, Cc, om, st);
-  hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, (const double*)L.t, L.e, om, st);
+  hipLaunchKernelGGL(v_post_kernel, dim3(g), dim3(256), 0, s, L, Cc, one, om, st);
+  if (!one) hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, (const double*)L.t, L.e, om, st);
   return (int)hipGetLastError();
 }
 

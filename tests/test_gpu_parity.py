@@ -351,6 +351,27 @@ def test_multigrid_matches_block_jacobi_and_direct(hdg, O, ctx, k, n):
     assert rel(um, ub) < 1e-9
 
 
+@pytest.mark.parametrize("k,nx,ny", [(2, 24, 40), (1, 33, 17), (3, 48, 32), (2, 65, 63)])
+def test_multigrid_rectangular_and_odd_grids(hdg, ctx, k, nx, ny):
+    """Hierarchies that stop at an odd grid (24x40 -> 3x5), a single odd
+    vertex level (33x17), and a coarsest level too large for the one-workgroup
+    tail (65x63: Jacobi sweeps on the vertex grid) all converge to the
+    block-Jacobi PCG's solution."""
+    m = hdg.rectangle_mesh((nx, ny), jitter=0.2, seed=11)
+    sols = {}
+    for pc in (hdg.PC_BLOCK_JACOBI, hdg.PC_MULTIGRID):
+        ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=QD[k], rtol=1e-13, precond=pc)
+        ctx.set_source()
+        ctx.set_dirichlet(m.boundary_dofs(k + 1), np.zeros(len(m.boundary_dofs(k + 1))))
+        ctx.assemble_condense()
+        sols[pc] = ctx.solve()
+    (ub, ib), (um, im) = sols[hdg.PC_BLOCK_JACOBI], sols[hdg.PC_MULTIGRID]
+    assert im.relres <= 1e-13 and ib.relres <= 1e-13
+    assert im.iters < ib.iters
+    assert rel(um, ub) < 1e-9
+
+
 def test_multigrid_iterations_mesh_independent(hdg, ctx):
     its = []
     for n in (64, 128, 256):
