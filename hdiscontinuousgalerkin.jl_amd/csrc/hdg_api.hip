@@ -60,6 +60,12 @@ struct hdg_ctx {
   double* d_scal = nullptr; // [0] errornorm, [1] true residual^2
   double* d_part2 = nullptr;
 
+  // mixed precision (HDG_MIXED): the multigrid preconditioner's trace-level
+  // data in float: trace matrix, Jacobi blocks, S P0, e0, res0, power-step
+  // scratch
+  float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr;
+  float *f_e0 = nullptr, *f_res0 = nullptr, *f_xa = nullptr, *f_xb = nullptr;
+
   bool assembled = false, solved = false, recovered = false;
 };
 
@@ -148,6 +154,13 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->mg.e0);
   dfree(c, c->mg.res0);
   dfree(c, c->mg.sp);
+  dfree(c, c->f_vals);
+  dfree(c, c->f_Dinv);
+  dfree(c, c->f_sp);
+  dfree(c, c->f_e0);
+  dfree(c, c->f_res0);
+  dfree(c, c->f_xa);
+  dfree(c, c->f_xb);
   c->has_setup = c->assembled = c->solved = c->recovered = false;
 }
 
@@ -336,6 +349,21 @@ int assemble_impl(hdg_ctx* c) {
 bool use_mg(const hdg_ctx* c) {
   return c->structured && c->opt.precond != HDG_PC_BLOCK_JACOBI;
 }
+bool mixed(const hdg_ctx* c) { return use_mg(c) && c->opt.precision == HDG_MIXED; }
+
+// views of the trace system and Krylov vectors for the CG kernels
+TraceOp<double> op64(const hdg_ctx* c) {
+  return TraceOp<double>{c->T.nt, c->T.nf, c->T.N, c->T.vals, c->T.Dinv, c->T.dmask};
+}
+TraceOp<float> op32(const hdg_ctx* c) {
+  return TraceOp<float>{c->T.nt, c->T.nf, c->T.N, c->f_vals, c->f_Dinv, c->T.dmask};
+}
+Krylov<double> kv64(const hdg_ctx* c) {
+  const CgVectors& v = c->cg;
+  return Krylov<double>{v.r, v.z, v.p0, v.p1, v.q, v.x, v.part, v.st, v.nblocks};
+}
+MgVecs<double> mv64(const hdg_ctx* c) { return MgVecs<double>{c->mg.e0, c->mg.res0, c->mg.sp}; }
+MgVecs<float> mv32(const hdg_ctx* c) { return MgVecs<float>{c->f_e0, c->f_res0, c->f_sp}; }
 
 // PCG iterations enqueued between convergence checks
 int chunk_of(const hdg_ctx* c) { return use_mg(c) ? 4 : 64; }
@@ -343,10 +371,13 @@ int chunk_of(const hdg_ctx* c) { return use_mg(c) ? 4 : 64; }
 int enqueue_chunk(hdg_ctx* c, int first, double rtol2, int maxit) {
   const int n = chunk_of(c);
   for (int i = first; i < first + n; ++i) {
-    LAUNCHCHK(c, launch_cg_spmv(c->pat, c->T, c->cg, i, rtol2, maxit, 0, c->stream));
-    if (use_mg(c)) {
-      LAUNCHCHK(c, launch_mg_update(c->T, c->cg, c->mg, i, c->stream));
-      LAUNCHCHK(c, launch_mg_precond(c->pat, c->T, c->cg, c->mg, i, c->stream));
+    LAUNCHCHK(c, launch_cg_spmv(c->pat, op64(c), kv64(c), i, rtol2, maxit, 0, c->stream));
+    if (mixed(c)) {
+      LAUNCHCHK(c, launch_mg_update(op32(c), kv64(c), mv32(c), i, c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), i, c->stream));
+    } else if (use_mg(c)) {
+      LAUNCHCHK(c, launch_mg_update(op64(c), kv64(c), mv64(c), i, c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), i, c->stream));
     } else {
       LAUNCHCHK(c, launch_cg_update(c->T, c->cg, i, 0, c->stream));
     }
@@ -363,10 +394,22 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   const double rtol = c->opt.rtol > 0 ? c->opt.rtol : 1e-12;
   const int maxit = c->opt.maxit > 0 ? c->opt.maxit : 200000;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  if (use_mg(c)) {
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, c->cg, 10, c->stream));
-    LAUNCHCHK(c, launch_mg_init(c->pat, c->T, c->cg, c->mg, c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, c->T, c->cg, c->mg, 0, c->stream));
+  if (mixed(c)) {
+    const int64_t N = c->T.N;
+    LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
+    LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, c->f_sp, c->stream));
+    LAUNCHCHK(c, launch_mg_power(c->pat, op32(c), c->f_xa, c->f_xb, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
+                                 10, c->stream));
+    LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), 0, c->stream));
+  } else if (use_mg(c)) {
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
+    // the CG vectors x, r serve as scratch before the solve initialises them
+    LAUNCHCHK(c, launch_mg_power(c->pat, op64(c), c->cg.x, c->cg.r, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
+                                 10, c->stream));
+    LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), 0, c->stream));
   } else {
     LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   }
@@ -554,6 +597,9 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     return fail(c, HDG_ERR_ARG, "unknown preconditioner %d", opt->precond);
   if (opt->precond == HDG_PC_MULTIGRID && !c->structured)
     return fail(c, HDG_ERR_UNSUPPORTED, "the multigrid preconditioner needs a structured rectangle_mesh");
+  if (opt->precision == HDG_MIXED && (!c->structured || opt->precond == HDG_PC_BLOCK_JACOBI))
+    return fail(c, HDG_ERR_UNSUPPORTED,
+                "mixed precision runs on the multigrid path: it needs a structured rectangle_mesh");
   TRY(select_device(c));
   hdg_options o = *opt;
   if (o.quad_degree <= 0) o.quad_degree = o.k + 1;
@@ -612,6 +658,15 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     TRY(dalloc(c, &c->mg.e0, N));
     TRY(dalloc(c, &c->mg.res0, N));
     TRY(dalloc(c, &c->mg.sp, 4 * N));
+  }
+  if (c->structured && o.precision == HDG_MIXED) {
+    TRY(dalloc(c, &c->f_vals, (size_t)kSlots * c->nt * N));
+    TRY(dalloc(c, &c->f_Dinv, (size_t)c->nt * N));
+    TRY(dalloc(c, &c->f_sp, 4 * N));
+    TRY(dalloc(c, &c->f_e0, N));
+    TRY(dalloc(c, &c->f_res0, N));
+    TRY(dalloc(c, &c->f_xa, N));
+    TRY(dalloc(c, &c->f_xb, N));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->src = SrcParams{};
@@ -924,7 +979,7 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   const int64_t ncopy = (int64_t)kSlots * c->nt * c->T.N;  // the size of the trace matrix values
   if (which == 3) TRY(dalloc(c, &scratch, ncopy));
   auto once = [&]() -> int {
-    if (which == 0) return launch_cg_spmv(c->pat, c->T, c->cg, 2, 0.0, 0, 1, c->stream);
+    if (which == 0) return launch_cg_spmv(c->pat, op64(c), kv64(c), 2, 0.0, 0, 1, c->stream);
     if (which == 1) return launch_cg_update(c->T, c->cg, 2, 1, c->stream);
     if (which == 3) return launch_copy(c->T.vals, scratch, ncopy, c->stream);
     return launch_condense(c->k, a, false, c->stream);

@@ -95,6 +95,32 @@ struct CgVectors {
   int nblocks;
 };
 
+// Views for the CG kernels.  TraceOp<W> is the trace operator stored as W:
+// FP64 for the CG product, FP64 or float for the multigrid preconditioner
+// (HDG_MIXED).  Krylov vectors, the iterate and every reduction are FP64.
+template <class W>
+struct TraceOp {
+  int nt;
+  int64_t nf, N;
+  const W* vals;   // face-ELL, vals[(s*nt + b)*N + row]
+  const W* Dinv;   // Dinv[b*N + row]
+  const unsigned char* dmask;
+};
+template <class V>
+struct Krylov {
+  V *r, *z, *p0, *p1, *q;
+  double* x;
+  double* part;    // partial-sum slots of nblocks doubles: 0 (p,q), 1-2 (r,z) by parity, 3 (r,r)
+  CgState* st;
+  int nblocks;
+};
+template <class W>
+struct MgVecs {
+  W* e0;           // trace-level correction
+  W* res0;         // trace-level residual
+  const W* sp;     // S P0 rows (4 per row)
+};
+
 // Multigrid hierarchy on the vertex grids of a structured mesh (level 0 is
 // the trace space, levels 1.. are vertex grids with 7-point stencils).
 struct MgLevel {
@@ -144,8 +170,8 @@ int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);
 int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s);
 // force = 1 runs the kernel regardless of the convergence state and leaves
 // x, r untouched (timing only; the CG state is not usable afterwards)
-int launch_cg_spmv(const DevPattern& P, const TraceSystem& T, CgVectors& v, int i, double rtol2, int maxit,
-                   int force, hipStream_t s);
+int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i, double rtol2,
+                   int maxit, int force, hipStream_t s);
 int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipStream_t s);
 int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s);
 // copies n doubles (n even) with a plain streaming kernel
@@ -158,11 +184,23 @@ int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev&
                      const double* qu, int uex_kind, const double* par3, double* part,
                      double* result, hipStream_t s);
 int cg_blocks(int64_t N, int nt);
+// vertex hierarchy and S P0 (FP64, and a float copy of S P0 when sp32 is set)
 int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
-                     CgVectors& v, int power_steps, hipStream_t s);
-int launch_mg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, hipStream_t s);
-int launch_mg_update(const TraceSystem& T, CgVectors& v, MgData& M, int i, hipStream_t s);
-int launch_mg_precond(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, int i,
-                      hipStream_t s);
+                     float* sp32, hipStream_t s);
+// omega = factor / lambda_max(Dinv S) by power steps on the operator as stored
+template <class V>
+int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
+                    const MgData& M, int steps, hipStream_t s);
+// r = b~, x = p = 0, e0 = omega Dinv r (preconditioner storage W)
+template <class W>
+int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
+                   const MgVecs<W>& mv, hipStream_t s);
+template <class W>
+int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, int i, hipStream_t s);
+template <class W>
+int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
+                      const MgVecs<W>& mv, int i, hipStream_t s);
+// float copy of n doubles
+int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
 
 }  // namespace hdg

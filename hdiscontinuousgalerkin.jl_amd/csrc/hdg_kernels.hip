@@ -554,8 +554,10 @@ __device__ inline double sum_partials(const double* p, int n, double* sh) {
   return t;
 }
 
+// CG iteration i, first half: convergence test on the (r, r) partials of
+// update(i-1), beta, p = z + beta p_old, q = S p, partial (p, q).
 template <int NT>
-__global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceSystem T, CgVectors v,
+__global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceOp<double> T, Krylov<double> v,
                                                              int i, double rtol2, int maxit, int force) {
   constexpr int NW = kCgThreads / 64;
   __shared__ double sh[NW];
@@ -717,6 +719,13 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
     st->it = 0;
     st->pq = 0.0;
   }
+}
+
+// float copy of a double array
+__global__ __launch_bounds__(256) void to_float_kernel(const double* __restrict__ src, float* __restrict__ dst,
+                                                       int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
 }
 
 // Plain streaming copy (HBM traffic calibration for hardware counters).
@@ -897,11 +906,11 @@ int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipS
   return (int)hipGetLastError();
 }
 
-int launch_cg_spmv(const DevPattern& P, const TraceSystem& T, CgVectors& v, int i, double rtol2, int maxit,
-                   int force, hipStream_t s) {
-#define SPMV(NT)                                                                                  \
-  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, i, rtol2, \
-                     maxit, force);
+int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i, double rtol2,
+                   int maxit, int force, hipStream_t s) {
+#define SPMV(NT)                                                                                   \
+  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, i, rtol2, maxit, \
+                     force);
   HDG_NT_DISPATCH(T.nt, SPMV)
 #undef SPMV
   return (int)hipGetLastError();
@@ -912,6 +921,11 @@ int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipSt
   hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, i, force);
   HDG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
+  return (int)hipGetLastError();
+}
+
+int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(to_float_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n);
   return (int)hipGetLastError();
 }
 

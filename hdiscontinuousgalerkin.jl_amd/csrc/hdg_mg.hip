@@ -70,16 +70,18 @@ __device__ inline double sum_partials_mg(const double* p, int n, double* sh) {
   return t;
 }
 
-// (S x)_row for S = -K on free rows, from the face-ELL matrix.
-template <int NT>
-__device__ inline double s_row(const DevPattern& P, const TraceSystem& T, const double* x, int64_t r) {
+// (S x)_row for S = -K on free rows, from the face-ELL matrix stored as V
+// (accumulated in FP64).
+template <int NT, class V>
+__device__ inline double s_row(const DevPattern& P, const TraceOp<V>& T, const V* x, int64_t r) {
   const int64_t f = r / NT;
   double acc = 0.0;
 #pragma unroll
   for (int s = 0; s < kSlots; ++s) {
     const int64_t g = (int64_t)__ldg(P.bcol + f * kSlots + s) * NT;
 #pragma unroll
-    for (int b = 0; b < NT; ++b) acc = fma(__ldg(T.vals + (int64_t)(s * NT + b) * T.N + r), x[g + b], acc);
+    for (int b = 0; b < NT; ++b)
+      acc = fma((double)__ldg(T.vals + (int64_t)(s * NT + b) * T.N + r), (double)x[g + b], acc);
   }
   return T.dmask[r] ? 0.0 : -acc;
 }
@@ -251,7 +253,7 @@ __global__ void mg_cmask_kernel(MgLevel F, MgLevel Cc) {
 // = 1/(2 sqrt 3).  Every face of the two cells of face f has its end points
 // among fq4[4f .. 4f+3].
 template <int NT>
-__global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T, MgData M) {
+__global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T, MgData M, float* sp32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= T.N) return;
   const int64_t f = r / NT;
@@ -281,7 +283,10 @@ __global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T,
     }
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) M.sp[(int64_t)j * T.N + r] = w[j];
+  for (int j = 0; j < 4; ++j) {
+    M.sp[(int64_t)j * T.N + r] = w[j];
+    if (sp32) sp32[(int64_t)j * T.N + r] = (float)w[j];
+  }
 }
 
 // ---------------------------------------------- power iteration for omega
@@ -293,24 +298,23 @@ __device__ inline double hash01(uint64_t i) {
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceSystem T, double* x, double* part) {
+template <int NT, class V>
+__global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceOp<V> T, V* x, double* part) {
   __shared__ double sh[NT];
   double s2 = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x) {
-    const double v = T.dmask[r] ? 0.0 : 0.5 + hash01((uint64_t)r);
+    const V v = (V)(T.dmask[r] ? 0.0 : 0.5 + hash01((uint64_t)r));
     x[r] = v;
-    s2 = fma(v, v, s2);
+    s2 = fma((double)v, (double)v, s2);
   }
   const double b = block_sum_mg<NT>(s2, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
 // y = Dinv S (x / |x|), |x|^2 from the previous step's partials.
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void pow_step_kernel(DevPattern P, TraceSystem T, const double* x,
-                                                           double* y, const double* part_in, double* part_out,
-                                                           int nb) {
+template <int NT, class V>
+__global__ __launch_bounds__(64 * NT) void pow_step_kernel(DevPattern P, TraceOp<V> T, const V* x, V* y,
+                                                           const double* part_in, double* part_out, int nb) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
@@ -319,15 +323,16 @@ __global__ __launch_bounds__(64 * NT) void pow_step_kernel(DevPattern P, TraceSy
   double s2 = 0.0;
   for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
     const int64_t r = r0 + threadIdx.x;
-    rs[threadIdx.x] = r < T.N ? inv * s_row<NT>(P, T, x, r) : 0.0;
+    rs[threadIdx.x] = r < T.N ? inv * s_row<NT, V>(P, T, x, r) : 0.0;
     __syncthreads();
     if (r < T.N) {
       const int lb = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
-      y[r] = z;
-      s2 = fma(z, z, s2);
+      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      const V zv = (V)z;
+      y[r] = zv;
+      s2 = fma((double)zv, (double)zv, s2);
     }
     __syncthreads();
   }
@@ -343,9 +348,10 @@ __global__ __launch_bounds__(256) void pow_final_kernel(const double* part, int 
 
 // ------------------------------------------------------- level 0 (trace)
 // update(i) with the multigrid preconditioner: x += alpha p; r -= alpha q;
-// e0 = omega Dinv r (first smoothing step); partial (r, r).
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceSystem T, CgVectors v, MgData M, int i) {
+// e0 = omega Dinv r (first smoothing step, preconditioner storage W);
+// partial (r, r).
+template <int NT, class W>
+__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov<double> v, MgVecs<W> mv, int i) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
@@ -379,8 +385,8 @@ __global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceSystem T, CgVec
       const int lb = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
-      M.e0[r] = om * z;
+      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      mv.e0[r] = (W)(om * z);
     }
     __syncthreads();
   }
@@ -388,9 +394,11 @@ __global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceSystem T, CgVec
   if (threadIdx.x == 0) v.part[(size_t)nb * 3 + blockIdx.x] = b;
 }
 
-// init (update(0)) for the multigrid path: r0 = b~, x = p = 0, e0 = omega Dinv r0.
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSystem T, CgVectors v, MgData M) {
+// init (update(0)) for the multigrid path: r0 = b~ = -(rhs - K uD), x = p = 0,
+// e0 = omega Dinv r0.
+template <int NT, class W>
+__global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSystem T64, TraceOp<W> T,
+                                                          Krylov<double> v, MgVecs<W> mv) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
@@ -400,14 +408,14 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
     const int64_t r = r0 + threadIdx.x;
     double b = 0.0;
     if (r < T.N) {
-      if (!T.dmask[r]) {
+      if (!T64.dmask[r]) {
         const int64_t f = r / NT;
-        double s = T.rhs[r];
+        double s = T64.rhs[r];
 #pragma unroll
         for (int sl = 0; sl < kSlots; ++sl) {
           const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-          for (int c = 0; c < NT; ++c) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
+          for (int c = 0; c < NT; ++c) s -= T64.vals[(int64_t)(sl * NT + c) * T64.N + r] * T64.uD[g + c];
         }
         b = -s;
       }
@@ -423,8 +431,8 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
       const int lb = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int c = 0; c < NT; ++c) z = fma(T.Dinv[(int64_t)c * T.N + r], rs[lb + c], z);
-      M.e0[r] = om * z;
+      for (int c = 0; c < NT; ++c) z = fma((double)T.Dinv[(int64_t)c * T.N + r], rs[lb + c], z);
+      mv.e0[r] = (W)(om * z);
     }
     __syncthreads();
   }
@@ -438,16 +446,17 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
 }
 
 // res0 = r - S e0.
-template <int NT>
-__global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceSystem T, CgVectors v, MgData M) {
+template <int NT, class W>
+__global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceOp<W> T, Krylov<double> v,
+                                                        MgVecs<W> mv) {
   if (v.st->done) return;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x)
-    M.res0[r] = T.dmask[r] ? 0.0 : v.r[r] - s_row<NT>(P, T, M.e0, r);
+    mv.res0[r] = T.dmask[r] ? (W)0 : (W)(v.r[r] - s_row<NT, W>(P, T, mv.e0, r));
 }
 
 // r1 = P0^T res0, vertex by vertex over the incident faces.
-template <int NT>
-__global__ __launch_bounds__(256) void mg_restrict0_kernel(CgVectors v, MgData M) {
+template <int NT, class W>
+__global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgData M, MgVecs<W> mv) {
   if (v.st->done) return;
   const MgLevel L = M.lev[1];
   const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -457,7 +466,7 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(CgVectors v, MgData M
     for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
       const int64_t f = M.vf_idx[k];
       const bool lo = M.fv[2 * f] == vv;
-      s += 0.5 * M.res0[f * NT] + (lo ? -kS3 : kS3) * M.res0[f * NT + 1];
+      s += 0.5 * (double)mv.res0[f * NT] + (lo ? -kS3 : kS3) * (double)mv.res0[f * NT + 1];
     }
   L.r[vv] = s;
 }
@@ -465,8 +474,9 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(CgVectors v, MgData M
 // z = e0 + P0 e1 + omega Dinv (res0 - (S P0) e1): the post-smoothing step.
 // S (e0 + P0 e1) = S e0 + (S P0) e1 and res0 = r - S e0, so the trace matrix
 // is not read again; partial (r, z) for the next iteration.
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceSystem T, CgVectors v, MgData M, int i) {
+template <int NT, class W>
+__global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylov<double> v, MgData M,
+                                                             MgVecs<W> mv, int i) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
@@ -479,9 +489,9 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceSystem T, CgVe
     double rr = 0.0, pe = 0.0;
     if (r < T.N && !T.dmask[r]) {
       const int64_t f = r / NT;
-      double s = M.res0[r];
+      double s = (double)mv.res0[r];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) s = fma(-M.sp[(int64_t)j * T.N + r], e1[M.fq4[4 * f + j]], s);
+      for (int j = 0; j < 4; ++j) s = fma(-(double)mv.sp[(int64_t)j * T.N + r], e1[M.fq4[4 * f + j]], s);
       rr = s;
       const int a = (int)(r - f * NT);
       if (a < 2) {
@@ -495,10 +505,10 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceSystem T, CgVe
       const int lb = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
-      z = fma(om, z, M.e0[r] + pe);
-      v.z[r] = z;
-      prz = fma(v.r[r], z, prz);
+      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      const double zv = fma(om, z, (double)mv.e0[r] + pe);
+      v.z[r] = zv;
+      prz = fma(v.r[r], zv, prz);
     }
     __syncthreads();
   }
@@ -830,13 +840,13 @@ int vcycle(const MgData& M, int l, const CgState* st, hipStream_t s) {
 }  // namespace
 
 int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
-                     CgVectors& v, int power_steps, hipStream_t s) {
+                     float* sp32, hipStream_t s) {
   hipLaunchKernelGGL(mg_vmask_kernel, dim3(grid1(M.nv)), dim3(256), 0, s, M, T);
 #define CELL(NT) hipLaunchKernelGGL(mg_cell_coarse_kernel<NT>, dim3(grid1(mesh.nc)), dim3(256), 0, s, mesh, M, T, Ate);
   MG_NT_DISPATCH(T.nt, CELL)
 #undef CELL
   hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, mesh, M);
-#define SPK(NT) hipLaunchKernelGGL(mg_sp_kernel<NT>, dim3(grid1(T.N)), dim3(256), 0, s, P, T, M);
+#define SPK(NT) hipLaunchKernelGGL(mg_sp_kernel<NT>, dim3(grid1(T.N)), dim3(256), 0, s, P, T, M, sp32);
   MG_NT_DISPATCH(T.nt, SPK)
 #undef SPK
   for (int l = 1; l + 1 < M.nlev; ++l) {
@@ -844,60 +854,81 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPa
     hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
   }
   if (M.tail < M.nlev) hipLaunchKernelGGL(mg_tailpack_kernel, dim3(16), dim3(256), 0, s, M);
-  // omega = cfac / lambda_max(Dinv S), power iteration on free dofs; the
-  // CG vectors x, r serve as scratch before the solve initialises them
-  const int nb = v.nblocks;
-  double* pa = v.part;
-  double* pb = v.part + nb;
-#define PINIT(NT) hipLaunchKernelGGL(pow_init_kernel<NT>, dim3(nb), dim3(64 * NT), 0, s, T, v.x, pa);
+  return (int)hipGetLastError();
+}
+
+// omega = factor / lambda_max(Dinv S) by power steps on free dofs; xa, xb
+// are scratch vectors of N entries, part holds 2 nb partials.
+template <class V>
+int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
+                    const MgData& M, int steps, hipStream_t s) {
+  double* pa = part;
+  double* pb = part + nb;
+#define PINIT(NT) hipLaunchKernelGGL((pow_init_kernel<NT, V>), dim3(nb), dim3(64 * NT), 0, s, T, xa, pa);
   MG_NT_DISPATCH(T.nt, PINIT)
 #undef PINIT
-  double* xa = v.x;
-  double* xb = v.r;
-  for (int k = 0; k < power_steps; ++k) {
+  for (int k = 0; k < steps; ++k) {
 #define PSTEP(NT) \
-    hipLaunchKernelGGL(pow_step_kernel<NT>, dim3(nb), dim3(64 * NT), 0, s, P, T, xa, xb, pa, pb, nb);
+    hipLaunchKernelGGL((pow_step_kernel<NT, V>), dim3(nb), dim3(64 * NT), 0, s, P, T, (const V*)xa, xb, pa, pb, nb);
     MG_NT_DISPATCH(T.nt, PSTEP)
 #undef PSTEP
     std::swap(xa, xb);
     std::swap(pa, pb);
   }
-  hipLaunchKernelGGL(pow_final_kernel, dim3(1), dim3(256), 0, s, pa, nb, v.st, M.omega_factor);
+  hipLaunchKernelGGL(pow_final_kernel, dim3(1), dim3(256), 0, s, pa, nb, st, M.omega_factor);
   return (int)hipGetLastError();
 }
 
-int launch_mg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, hipStream_t s) {
-#define INIT(NT) hipLaunchKernelGGL(mg_init_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v, M);
+template <class W>
+int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
+                   const MgVecs<W>& mv, hipStream_t s) {
+#define INIT(NT) hipLaunchKernelGGL((mg_init_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, P, T64, T, v, mv);
   MG_NT_DISPATCH(T.nt, INIT)
 #undef INIT
   return (int)hipGetLastError();
 }
 
-int launch_mg_update(const TraceSystem& T, CgVectors& v, MgData& M, int i, hipStream_t s) {
-#define UPD(NT) hipLaunchKernelGGL(mg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, i);
+template <class W>
+int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, int i, hipStream_t s) {
+#define UPD(NT) hipLaunchKernelGGL((mg_update_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, mv, i);
   MG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
   return (int)hipGetLastError();
 }
 
-// z = V-cycle(r), starting from e0 = omega Dinv r already in M.e0; writes the
-// (r, z) partials of iteration i+1.
-int launch_mg_precond(const DevPattern& P, const TraceSystem& T, CgVectors& v, MgData& M, int i,
-                      hipStream_t s) {
+// z = V-cycle(r), starting from e0 = omega Dinv r already in mv.e0; writes
+// the (r, z) partials of iteration i+1.
+template <class W>
+int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
+                      const MgVecs<W>& mv, int i, hipStream_t s) {
   const int gr = grid1(T.N, 256);
   const int grows = gr > 4096 ? 4096 : gr;
-#define RES(NT) hipLaunchKernelGGL(mg_resid0_kernel<NT>, dim3(grows), dim3(256), 0, s, P, T, v, M);
+#define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES
-#define RST(NT) hipLaunchKernelGGL(mg_restrict0_kernel<NT>, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, v, M);
+#define RST(NT) \
+  hipLaunchKernelGGL((mg_restrict0_kernel<NT, W>), dim3(grid1(M.lev[1].n)), dim3(256), 0, s, v, M, mv);
   MG_NT_DISPATCH(T.nt, RST)
 #undef RST
   int e = vcycle(M, 1, v.st, s);
   if (e) return e;
-#define SMO(NT) hipLaunchKernelGGL(mg_smooth0_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, i);
+#define SMO(NT) \
+  hipLaunchKernelGGL((mg_smooth0_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, mv, i);
   MG_NT_DISPATCH(T.nt, SMO)
 #undef SMO
   return (int)hipGetLastError();
 }
+
+#define MG_INSTANTIATE(W)                                                                                      \
+  template int launch_mg_power<W>(const DevPattern&, const TraceOp<W>&, W*, W*, double*, int, CgState*,          \
+                                  const MgData&, int, hipStream_t);                                             \
+  template int launch_mg_init<W>(const DevPattern&, const TraceSystem&, const TraceOp<W>&, const Krylov<double>&, \
+                                 const MgVecs<W>&, hipStream_t);                                                \
+  template int launch_mg_update<W>(const TraceOp<W>&, const Krylov<double>&, const MgVecs<W>&, int, hipStream_t); \
+  template int launch_mg_precond<W>(const DevPattern&, const TraceOp<W>&, const Krylov<double>&, const MgData&,  \
+                                    const MgVecs<W>&, int, hipStream_t);
+MG_INSTANTIATE(double)
+MG_INSTANTIATE(float)
+#undef MG_INSTANTIATE
 
 }  // namespace hdg

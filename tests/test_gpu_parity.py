@@ -396,3 +396,43 @@ def test_multigrid_rejects_unstructured(hdg, ctx):
     ctx.set_dirichlet(m.boundary_dofs(2), np.zeros(len(m.boundary_dofs(2))))
     info = ctx.run()
     assert info.relres <= 1e-12
+
+
+# ------------------------------------------------------- mixed precision
+@pytest.mark.parametrize("k,n", [(1, 32), (2, 32), (3, 16), (2, 128), (3, 64)])
+def test_mixed_precision_matches_fp64(hdg, O, ctx, k, n):
+    """HDG_MIXED (FP64 CG on the FP64 trace matrix, multigrid preconditioner
+    on float copies of the trace matrix, Jacobi blocks and S P0) reaches
+    the FP64 solve's accuracy: the same stopping test on the FP64 residual
+    recursion, a recomputed residual at the same FP64 floor, the FP64
+    solution to 1e-10 relative and the oracle's direct solve on small
+    meshes."""
+    m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=5)
+    sols = {}
+    for prec in (hdg.HDG_FP64, hdg.HDG_MIXED):
+        ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=QD[k], rtol=1e-12, precision=prec, precond=hdg.PC_MULTIGRID)
+        ctx.set_source()
+        ctx.set_dirichlet(m.boundary_dofs(k + 1), np.zeros(len(m.boundary_dofs(k + 1))))
+        ctx.assemble_condense()
+        sols[prec] = ctx.solve()
+    (u64, i64), (u32, i32) = sols[hdg.HDG_FP64], sols[hdg.HDG_MIXED]
+    assert i32.relres <= 1e-12
+    assert i32.true_relres < 4 * max(i64.true_relres, 1e-12)
+    assert i32.iters <= i64.iters + 3
+    assert rel(u32, u64) < 1e-10
+    if n <= 32:
+        om = O.rectangle_mesh(n, n, jitter=0.2, seed=5)
+        r = O.solve_pipeline(O.problem(k=k, qdeg=QD[k]), om)
+        assert rel(u32, r["uhat"]) < 1e-10
+
+
+def test_mixed_precision_needs_multigrid_path(hdg, ctx):
+    m = hdg.parse_mesh_triangle(os.path.join(GOLDEN, "figure.1"))
+    ctx.set_mesh(m)
+    with pytest.raises(hdg.HDGError, match="UNSUPPORTED"):
+        ctx.setup(1, precision=hdg.HDG_MIXED)
+    m = hdg.rectangle_mesh((8, 8))
+    ctx.set_mesh(m)
+    with pytest.raises(hdg.HDGError, match="UNSUPPORTED"):
+        ctx.setup(1, precision=hdg.HDG_MIXED, precond=hdg.PC_BLOCK_JACOBI)
