@@ -1,17 +1,22 @@
-"""Benchmark: the plain FP64 HDG pipeline on one MI355X.
+"""Benchmark: the plain FP64 HDG pipeline on one MI355X, with the
+mixed-precision form measured beside it on the same workload.
 
-Workload (the GPU configuration the metric is quoted on): -Δu = f on the
-unit square, 512 x 512 cells split into 524,288 triangles, P2 (u 6, q 12,
+Default workload (the GPU configuration the metric is quoted on): -Δu = f on
+the unit square, 512 x 512 cells split into 524,288 triangles, P2 (u 6, q 12,
 trace 3 per edge), tau = 1, interior vertices jittered by U(-0.2h, 0.2h)
 from seed 20261202, f = 2 pi^2 sin(pi x) sin(pi y), u = 0 on the boundary.
+`--workload p3-1024` runs 1024 x 1024 cells (2,097,152 triangles) at P3
+(quadrature degree 6) instead.
 One step = assemble + condense + trace solve (multigrid-preconditioned CG
 to a relative residual of 1e-12) + recovery, with mesh, tables and
 Dirichlet data already resident in HBM.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the PCG
-trace SpMV), timed with HIP events on the library's stream; `cpu_baseline`
-times the oracle (CPU restatement of the reference pipeline) on a bounded
-sample of the same workload.
+Prints one JSON line.  `roofline` is for the dominant kernel (the CG trace
+SpMV), timed with HIP events on the library's stream; `mixed_precision`
+repeats the timed steps with HDG_MIXED (FP64 CG, float multigrid
+preconditioner) and compares its solution with the FP64 one;
+`cpu_baseline` times the oracle (CPU restatement of the reference pipeline)
+on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -28,12 +33,15 @@ sys.path.insert(0, REPO)
 
 METRIC = "elements/s (assemble+condense+solve), p50/p99 latency vs FP64 plain, same L2"
 HBM_PEAK_GBS = 8000.0
-NX = NY = 512
-K_ORDER, QDEG = 2, 3
 SEED = 20261202
+# name: (cells per side, k, quad_degree, description)
+WORKLOADS = {
+    "p2-512": (512, 2, 3, "HDG Poisson 512x512 cells (524288 triangles) P2"),
+    "p3-1024": (1024, 3, 6, "HDG Poisson 1024x1024 cells (2097152 triangles) P3, quad_degree 6"),
+}
 
 
-def spmv_bytes(ctx, mesh):
+def spmv_bytes(ctx):
     """Algorithmic bytes of one trace-SpMV launch (FP64): matrix values
     8 nnz, block column ids 4 per stored block, the gathered input vectors z
     and p_old read once (16 N), p_new and q written (16 N), the Dirichlet
@@ -45,18 +53,18 @@ def spmv_bytes(ctx, mesh):
     return 8 * s.nnz + 4 * nblocks + 16 * N + 16 * N + N
 
 
-def measured_traffic():
+def measured_traffic(workload):
     """HBM bytes per SpMV launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
     calibration), or None if that summary is absent."""
-    path = os.path.join(REPO, "profiles", "cg_spmv_c2_traffic.json")
+    path = os.path.join(REPO, "profiles", f"cg_spmv_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         return json.load(fh).get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(mesh_np, iters_cpu, seconds_cap=30.0):
+def cpu_baseline(mesh_np, k, qdeg, iters_cpu, seconds_cap=30.0):
     """Oracle on the same mesh: full build + condense + assembly, then its
     block-Jacobi PCG timed on a bounded prefix and scaled to the iteration
     count that solver needs to converge on this mesh (`iters_cpu`, measured
@@ -66,33 +74,48 @@ def cpu_baseline(mesh_np, iters_cpu, seconds_cap=30.0):
     import oracle as O
     ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     om = O.Mesh(mesh_np.nodes, mesh_np.cells, mesh_np.faces, mesh_np.facesets, mesh_np.nodesets)
-    P = O.problem(k=K_ORDER, qdeg=QDEG)
+    P = O.problem(k=k, qdeg=qdeg)
     t0 = time.perf_counter()
     Ate, bte, Ke, be = O.condense(P, om, ncores)
     t1 = time.perf_counter()
-    K, rhs = O.assemble(om, K_ORDER + 1, Ate, bte)
+    K, rhs = O.assemble(om, k + 1, Ate, bte)
     d, v = O.dirichlet_trace(P, om)
     t2 = time.perf_counter()
     budget = max(seconds_cap - (t2 - t0), 5.0)
     # time 50 iterations to size the bounded run, then the bounded run
     ts = time.perf_counter()
-    O.pcg_solve(K, rhs, K_ORDER + 1, d, v, rtol=1e-12, maxit=50, nthreads=ncores)
+    O.pcg_solve(K, rhs, k + 1, d, v, rtol=1e-12, maxit=50, nthreads=ncores)
     per_it = (time.perf_counter() - ts) / 50
     n_it = int(max(50, min(iters_cpu, budget / max(per_it, 1e-9))))
     ts = time.perf_counter()
-    x, it, _ = O.pcg_solve(K, rhs, K_ORDER + 1, d, v, rtol=1e-12, maxit=n_it, nthreads=ncores)
+    x, it, _ = O.pcg_solve(K, rhs, k + 1, d, v, rtol=1e-12, maxit=n_it, nthreads=ncores)
     t_it = (time.perf_counter() - ts) / max(it, 1)
     t3 = time.perf_counter()
-    O.recover(om, K_ORDER, Ke, be, x)
+    O.recover(om, k, Ke, be, x)
     t4 = time.perf_counter()
     total = (t1 - t0) + (t2 - t1) + t_it * iters_cpu + (t4 - t3)
     return {
         "value": om.ncells / total, "unit": "elements/s", "cores": ncores, "kind": "port",
-        "sample": (f"oracle on the full 512x512 P2 mesh: condense+assembly timed in full "
+        "sample": (f"oracle on the full mesh: condense+assembly timed in full "
                    f"({t2 - t0:.2f} s), {it} block-Jacobi PCG iterations timed ({t_it * 1e3:.2f} ms/iter) "
                    f"and scaled to the {iters_cpu} iterations that solver needs here to reach 1e-12, "
                    f"recovery timed in full"),
     }
+
+
+def timed_steps(ctx, steps, warmup, sync):
+    """Runs warmup + steps of hdg_run; returns (elapsed s, per-step latencies, infos)."""
+    for _ in range(warmup):
+        ctx.run()
+    sync()
+    lat, infos = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        infos.append(ctx.run())
+        lat.append(time.perf_counter() - ts)
+    sync()
+    return time.perf_counter() - t0, lat, infos
 
 
 def main():
@@ -100,7 +123,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="p2-512")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--precond", choices=("auto", "bj", "mg"), default="auto",
                     help="trace-solve preconditioner: auto (multigrid on structured meshes), "
                          "block-Jacobi, multigrid")
@@ -115,41 +140,34 @@ def main():
     import torch
     from __graft_entry__ import _import_package
     H = _import_package()
+    n, k, qdeg, desc = WORKLOADS[args.workload]
 
-    mesh = H.rectangle_mesh((NX, NY), jitter=0.2, seed=SEED)
+    mesh = H.rectangle_mesh((n, n), jitter=0.2, seed=SEED)
     ctx = H.Context(0)
     ctx.set_mesh(mesh)
     pc = {"auto": H.PC_AUTO, "bj": H.PC_BLOCK_JACOBI, "mg": H.PC_MULTIGRID}[args.precond]
-    ctx.setup(K_ORDER, quad_degree=QDEG, tau=1.0, rtol=1e-12, precond=pc)
     pc_name = "block-Jacobi PCG" if args.precond == "bj" else "multigrid-preconditioned CG"
-    ctx.set_source()
-    nt = K_ORDER + 1
+    nt = k + 1
     dofs = mesh.boundary_dofs(nt)
-    ctx.set_dirichlet(dofs, np.zeros(len(dofs)))
 
-    for _ in range(args.warmup):
-        ctx.run()
-    torch.cuda.synchronize()
-    lat = []
-    infos = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        infos.append(ctx.run())
-        lat.append(time.perf_counter() - ts)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    def configure(precision, precond=pc):
+        ctx.setup(k, quad_degree=qdeg, tau=1.0, rtol=1e-12, precond=precond, precision=precision)
+        ctx.set_source()
+        ctx.set_dirichlet(dofs, np.zeros(len(dofs)))
+
+    configure(H.HDG_FP64)
+    elapsed, lat, infos = timed_steps(ctx, args.steps, args.warmup, torch.cuda.synchronize)
     ncells = mesh.cells.shape[0]
     l2sq = ctx.errornorm()
     iters = infos[-1].iters
 
-    # dominant kernel: one PCG trace-SpMV launch, HIP events on ctx's stream
+    # dominant kernel: one CG trace-SpMV launch, HIP events on ctx's stream
     spmv_us = ctx.time_kernel(0, reps=200)
-    upd_us = ctx.time_kernel(1, reps=200)
-    ab = spmv_bytes(ctx, mesh)
+    ab = spmv_bytes(ctx)
     achieved = ab / (spmv_us * 1e-6) / 1e9
     copy_us = ctx.time_kernel(3, reps=3) if args.calibrate else None
-    traffic = measured_traffic()
+    traffic = measured_traffic(args.workload)
+    nnz, ndofs = int(ctx.sizes.nnz), int(ctx.sizes.n_dofs)
 
     out = {
         "metric": METRIC,
@@ -164,11 +182,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "HDG Poisson 512x512 cells (524288 triangles) P2, jitter 0.2h "
-                               f"seed 20261202, tau=1, FP64, {pc_name} rtol 1e-12",
+        "config": {"workload": f"{desc}, jitter 0.2h seed {SEED}, tau=1, FP64, {pc_name} rtol 1e-12",
                    "preconditioner": pc_name,
-                   "cells": ncells, "k": K_ORDER, "quad_degree": QDEG, "trace_dofs": int(ctx.sizes.n_dofs),
-                   "nnz": int(ctx.sizes.nnz)},
+                   "cells": ncells, "k": k, "quad_degree": qdeg, "trace_dofs": ndofs, "nnz": nnz},
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "pcg_iterations": int(iters),
@@ -177,21 +193,40 @@ def main():
         "errornorm_sq": float(l2sq),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "cg_spmv_kernel<3>", "avg_us": spmv_us, "bytes_per_launch": int(ab),
-                     "update_kernel_avg_us": upd_us},
+                     "kernel": f"cg_spmv_kernel<{nt}>", "avg_us": spmv_us, "bytes_per_launch": int(ab)},
     }
     if copy_us is not None:
-        copy_bytes = 2 * 8 * 5 * ctx.sizes.nt * ctx.sizes.n_dofs
+        copy_bytes = 2 * 8 * 5 * nt * ndofs
         out["roofline"]["copy_kernel"] = {"avg_us": copy_us, "bytes": copy_bytes,
                                           "GBps": copy_bytes / (copy_us * 1e-6) / 1e9}
+
+    if not args.no_mixed and args.precond != "bj":
+        ctx.assemble_condense()
+        u64, _ = ctx.solve()
+        configure(H.HDG_MIXED)
+        m_elapsed, m_lat, m_infos = timed_steps(ctx, args.steps, args.warmup, torch.cuda.synchronize)
+        m_l2 = float(np.sqrt(ctx.errornorm()))
+        ctx.assemble_condense()
+        u32, _ = ctx.solve()
+        out["mixed_precision"] = {
+            "form": "FP64 CG on the FP64 trace matrix, multigrid preconditioner in float",
+            "value": ncells * args.steps / m_elapsed,
+            "ms_per_step": 1e3 * m_elapsed / args.steps,
+            "p50_ms": 1e3 * float(np.percentile(m_lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(m_lat, 99)),
+            "speedup_vs_fp64": elapsed / m_elapsed,
+            "pcg_iterations": int(m_infos[-1].iters),
+            "true_relres": float(m_infos[-1].true_relres),
+            "l2_error": m_l2,
+            "max_rel_diff_vs_fp64": float(np.abs(u32 - u64).max() / np.abs(u64).max()),
+        }
+
     if not args.no_cpu_baseline:
         # iterations the oracle's own solver (block-Jacobi PCG) needs on this
         # mesh, counted by the same algorithm on the GPU
-        ctx.setup(K_ORDER, quad_degree=QDEG, tau=1.0, rtol=1e-12, precond=H.PC_BLOCK_JACOBI)
-        ctx.set_source()
-        ctx.set_dirichlet(dofs, np.zeros(len(dofs)))
+        configure(H.HDG_FP64, H.PC_BLOCK_JACOBI)
         iters_bj = int(ctx.run().iters)
-        out["cpu_baseline"] = cpu_baseline(mesh, iters_bj)
+        out["cpu_baseline"] = cpu_baseline(mesh, k, qdeg, iters_bj)
     ctx.close()
     print(json.dumps(out), flush=True)
 
