@@ -53,7 +53,10 @@ typedef enum {
 
 typedef struct hdg_ctx hdg_ctx;
 
-/* Precision modes of the trace solve. */
+/* Precision modes of the trace solve.  HDG_MIXED keeps condensation, the
+   trace matrix, the CG recurrences and the iterate in FP64 and runs the
+   multigrid preconditioner's trace-level data in float (structured meshes
+   only: HDG_ERR_UNSUPPORTED otherwise). */
 enum { HDG_FP64 = 0, HDG_MIXED = 1 };
 
 /* Preconditioners of the trace solve.  HDG_PC_AUTO picks multigrid when the
@@ -103,7 +106,7 @@ typedef struct {
 
 typedef struct {
   int32_t iters;       /* PCG iterations */
-  int32_t refinements; /* outer FP64 refinement steps (HDG_MIXED) */
+  int32_t refinements; /* 0: the FP64 CG recurrence itself carries the accuracy */
   double relres;       /* final recursive ||r|| / ||b|| */
   double true_relres;  /* ||b - S x|| / ||b|| recomputed in FP64 */
   double seconds;      /* device time of the solve */
