@@ -28,7 +28,9 @@ EXPORTS = (
     "hdg_rectangle_mesh", "hdg_set_mesh", "hdg_setup", "hdg_set_source", "hdg_set_dirichlet",
     "hdg_query_sizes", "hdg_assemble_condense", "hdg_get_system", "hdg_get_local_ops",
     "hdg_get_condensed", "hdg_get_element_blocks", "hdg_solve", "hdg_set_trace", "hdg_recover",
-    "hdg_errornorm", "hdg_run", "hdg_dubiner", "hdg_quadrature", "hdg_time_kernel",
+    "hdg_errornorm", "hdg_run", "hdg_dubiner", "hdg_quadrature", "hdg_time_kernel", "hdg_get_trace",
+    "hdg_server_create", "hdg_server_submit", "hdg_server_wait", "hdg_server_trace", "hdg_server_footprint",
+    "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error",
 )
 
 
@@ -70,6 +72,25 @@ class Sizes(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int32), ("refinements", C.c_int32), ("relres", C.c_double),
                 ("true_relres", C.c_double), ("seconds", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("jitter", C.c_double), ("seed", C.c_uint64),
+                ("opt", Options), ("src", Source), ("uex_kind", C.c_int32), ("want_trace", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("iters", C.c_int32), ("relres", C.c_double),
+                ("true_relres", C.c_double), ("errornorm", C.c_double), ("n_cells", C.c_int64),
+                ("n_dofs", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double), ("device_bytes", C.c_size_t), ("slot", C.c_int32)]
+
+
+class ServerSummary(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("failed", C.c_int64), ("cells", C.c_int64),
+                ("p50_latency", C.c_double), ("p99_latency", C.c_double), ("mean_latency", C.c_double),
+                ("mean_queue_wait", C.c_double), ("busy_seconds", C.c_double),
+                ("cells_per_second", C.c_double), ("peak_reserved", C.c_size_t)]
 
 
 _lib = None
@@ -118,6 +139,15 @@ def _declare(L):
         "hdg_dubiner": (C.c_int, [C.c_int, C.c_double, C.c_double, _d, _d]),
         "hdg_quadrature": (C.c_int, [C.c_int, C.c_int, C.c_int, _d, _d]),
         "hdg_time_kernel": (C.c_int, [_ctx, C.c_int, C.c_int, _d]),
+        "hdg_get_trace": (C.c_int, [_ctx, _d]),
+        "hdg_server_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int]),
+        "hdg_server_submit": (C.c_int, [C.c_void_p, _P(Request), _i64]),
+        "hdg_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, _P(Result)]),
+        "hdg_server_trace": (C.c_int, [C.c_void_p, C.c_int64, _d]),
+        "hdg_server_footprint": (C.c_size_t, [_P(Request)]),
+        "hdg_server_stats": (C.c_int, [C.c_void_p, _P(ServerSummary)]),
+        "hdg_server_destroy": (None, [C.c_void_p]),
+        "hdg_server_last_error": (C.c_char_p, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

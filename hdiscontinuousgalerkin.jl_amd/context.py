@@ -141,6 +141,12 @@ class Context:
         self._chk(lib().hdg_solve(self._h, ptr(uh), C.byref(info)))
         return uh, info
 
+    def trace(self):
+        """The trace solution of the last solve / run (hdg_get_trace)."""
+        u = np.zeros(self.sizes.n_dofs)
+        self._chk(lib().hdg_get_trace(self._h, ptr(u)))
+        return u
+
     def set_trace(self, uhat):
         u = np.ascontiguousarray(uhat, dtype=np.float64)
         self._chk(lib().hdg_set_trace(self._h, ptr(u)))

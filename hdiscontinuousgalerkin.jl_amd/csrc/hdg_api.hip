@@ -277,16 +277,18 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     L.nx = lx;
     L.ny = ly;
     L.n = (int64_t)(lx + 1) * (ly + 1);
-    TRY(dalloc(c, &L.a, 7 * L.n));
+    // RAP across a one-cell-wide coarse column or row adds the (+1,+1) pair
+    L.npts = l == 1 ? 7 : ((M.lev[l - 1].npts > 7 || M.lev[l - 1].nx % 2 || M.lev[l - 1].ny % 2) ? kStencil : 7);
+    TRY(dalloc(c, &L.a, kStencil * L.n));
     TRY(dalloc(c, &L.dinv, L.n));
     TRY(dalloc(c, &L.mask, L.n));
     TRY(dalloc(c, &L.r, L.n));
     TRY(dalloc(c, &L.e, L.n));
     TRY(dalloc(c, &L.t, L.n));
     ++l;
-    if (lx % 2 || ly % 2 || std::min(lx, ly) < 4 || l >= kMaxLevels) break;
-    lx /= 2;
-    ly /= 2;
+    if (std::max(lx, ly) <= 2 || l >= kMaxLevels) break;
+    lx = (lx + 1) / 2;  // coarsening with a ragged end on odd sides
+    ly = (ly + 1) / 2;
   }
   M.nlev = l;
   // smallest suffix of levels whose nodes fit the one-workgroup budget
@@ -300,7 +302,7 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
   if (M.tail < M.nlev) {
     int64_t tn = 0;
     for (int t = M.tail; t < M.nlev; ++t) tn += M.lev[t].n;
-    TRY(dalloc(c, &M.tailpack, 8 * tn));
+    TRY(dalloc(c, &M.tailpack, (kStencil + 1) * tn));
   }
   return HDG_OK;
 }
@@ -768,6 +770,15 @@ int hdg_set_trace(hdg_ctx* c, const double* uhat) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->solved = true;
   c->recovered = false;
+  return HDG_OK;
+}
+
+int hdg_get_trace(hdg_ctx* c, double* uhat) {
+  if (!c || !uhat) return HDG_ERR_ARG;
+  if (!c->solved) return fail(c, HDG_ERR_STATE, "no trace solution: call hdg_solve or hdg_run");
+  TRY(select_device(c));
+  HIPCHK(c, hipMemcpyAsync(uhat, c->d_uhat, sizeof(double) * c->T.N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return HDG_OK;
 }
 

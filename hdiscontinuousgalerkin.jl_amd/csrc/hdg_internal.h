@@ -125,17 +125,20 @@ struct MgVecs {
 // the trace space, levels 1.. are vertex grids with 7-point stencils).
 struct MgLevel {
   int nx = 0, ny = 0;
+  int npts = 7;                  // 7, or kStencil once a ragged coarsening is above this level
   int64_t n = 0;                 // (nx+1)(ny+1) nodes
-  double* a = nullptr;           // stencils, a[s*n + v]
+  double* a = nullptr;           // stencils, a[s*n + v], s < kStencil
   double* dinv = nullptr;        // 1 / a[0], 0 on fixed nodes
   unsigned char* mask = nullptr; // fixed (Dirichlet) nodes
   double *r = nullptr, *e = nullptr, *t = nullptr;
 };
 constexpr int kMaxLevels = 16;
+// points of the vertex-level stencils (slots listed in hdg_mg.hip)
+constexpr int kStencil = 9;
 // node budget of the levels solved inside one workgroup: their stencils,
-// diagonal inverses and three vectors live in LDS (11 doubles per node,
-// 132 KB of the 160 KB of a CU)
-constexpr int kTailNodes = 1536;
+// diagonal inverses and three vectors live in LDS (13 doubles per node,
+// 152 KB of the 160 KB of a CU)
+constexpr int kTailNodes = 1500;
 struct MgData {
   int nlev = 0;                  // levels 1..nlev-1 are vertex grids
   int nx = 0, ny = 0;

@@ -7,14 +7,20 @@
 // (v_hi - v_lo) / (2 sqrt 3), the face parameter running from the lower to
 // the higher node id), and the coarse operator is the Galerkin product
 // P^T S P, assembled cell by cell from the condensed blocks.  Levels 2.. are
-// the vertex grids of a rectangle_mesh coarsened by two, with linear
-// interpolation on the nested triangulation and Galerkin (RAP) operators;
-// every level is a 7-point stencil.  One symmetric V-cycle (damped Jacobi,
+// the vertex grids of a rectangle_mesh coarsened by two (a side with n
+// cells gets (n + 1) / 2: coarse node I sits on fine node min(2I, n), so an
+// odd side keeps its last fine column as a one-cell-wide coarse column),
+// with linear interpolation on the nested triangulation and Galerkin (RAP)
+// operators; every level is a 7-point stencil.  One symmetric V-cycle (damped Jacobi,
 // fixed sweep counts, Jacobi sweeps on the coarsest grid) is a fixed SPD
 // operator, so it preconditions CG.
 //
 // Stencil slots: 0 (0,0), 1 (+1,0), 2 (-1,0), 3 (0,+1), 4 (0,-1), 5 (+1,-1),
-// 6 (-1,+1); node (i, j) of a level with nx cells per row is i + j (nx+1).
+// 6 (-1,+1), 7 (+1,+1), 8 (-1,-1); node (i, j) of a level with nx cells per
+// row is i + j (nx+1).  The vertex operator of the mesh and the Galerkin
+// operators of even coarsening use slots 0-6; a one-cell-wide coarse column
+// or row is not nested in the coarse triangulation, and RAP across it also
+// couples the (+1,+1) neighbours (9 points close under RAP).
 #include <hip/hip_runtime.h>
 
 #include "hdg_internal.h"
@@ -24,8 +30,8 @@ namespace hdg {
 namespace {
 
 constexpr double kS3 = 0.28867513459481288225;  // 1 / (2 sqrt 3)
-__constant__ int kOffX[7] = {0, 1, -1, 0, 0, 1, -1};
-__constant__ int kOffY[7] = {0, 0, 0, 1, -1, -1, 1};
+__constant__ int kOffX[kStencil] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+__constant__ int kOffY[kStencil] = {0, 0, 0, 1, -1, -1, 1, 1, -1};
 
 __device__ inline int slot_of(int dx, int dy) {
   if (dx == 0 && dy == 0) return 0;
@@ -33,7 +39,44 @@ __device__ inline int slot_of(int dx, int dy) {
   if (dx == 0) return dy == 1 ? 3 : (dy == -1 ? 4 : -1);
   if (dx == 1 && dy == -1) return 5;
   if (dx == -1 && dy == 1) return 6;
+  if (dx == 1 && dy == 1) return 7;
+  if (dx == -1 && dy == -1) return 8;
   return -1;
+}
+
+// Coarse parents of fine index x on a side of n cells: one (x even, or the
+// last node of an odd side) or two neighbours at weight 1/2 (odd x < n).
+struct Par1 {
+  int a, b;  // a == b: single parent
+};
+__device__ inline Par1 parents1(int x, int n) {
+  if (!(x & 1)) return Par1{x >> 1, x >> 1};
+  if (x == n) return Par1{(n + 1) >> 1, (n + 1) >> 1};
+  return Par1{(x - 1) >> 1, (x + 1) >> 1};
+}
+// fine index of coarse index I on a side of n fine cells
+__device__ inline int fine_of(int I, int n) { return min(2 * I, n); }
+
+// Linear interpolation of the coarse vector ec (row length cnx + 1) at fine
+// node (x, y) of a level with fnx x fny cells: along x, along y, or along the
+// cell diagonal ((x+1)/2, (y-1)/2) -- ((x-1)/2, (y+1)/2) when both are odd.
+__device__ inline double prolong_at(int fnx, int fny, int cnx, const double* ec, int x, int y) {
+  const Par1 px = parents1(x, fnx), py = parents1(y, fny);
+  const int64_t cs = cnx + 1;
+  if (px.a == px.b && py.a == py.b) return ec[py.a * cs + px.a];
+  if (py.a == py.b) return 0.5 * (ec[py.a * cs + px.a] + ec[py.a * cs + px.b]);
+  if (px.a == px.b) return 0.5 * (ec[py.a * cs + px.a] + ec[py.b * cs + px.a]);
+  return 0.5 * (ec[py.a * cs + px.b] + ec[py.b * cs + px.a]);
+}
+
+// weight of coarse node (I, J) in prolong_at at fine node (x, y)
+__device__ inline double pweight(int fnx, int fny, int x, int y, int I, int J) {
+  const Par1 px = parents1(x, fnx), py = parents1(y, fny);
+  const bool sx = px.a == px.b, sy = py.a == py.b;
+  if (sx && sy) return (px.a == I && py.a == J) ? 1.0 : 0.0;
+  if (sy) return (py.a == J && (px.a == I || px.b == I)) ? 0.5 : 0.0;
+  if (sx) return (px.a == I && (py.a == J || py.b == J)) ? 0.5 : 0.0;
+  return ((px.b == I && py.a == J) || (px.a == I && py.b == J)) ? 0.5 : 0.0;
 }
 
 __device__ inline double wave_sum_mg(double v) {
@@ -161,7 +204,7 @@ __global__ void mg_assemble_l1_kernel(DevMesh mesh, MgData M) {
   if (v >= L.n) return;
   const int nx = M.nx, ny = M.ny;
   const int i = (int)(v % (nx + 1)), j = (int)(v / (nx + 1));
-  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  double st[kStencil] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (!L.mask[v]) {
     // (cell grid position, upper?, local index of v)
     const int ca[6] = {i, i - 1, i, i - 1, i - 1, i};
@@ -182,69 +225,73 @@ __global__ void mg_assemble_l1_kernel(DevMesh mesh, MgData M) {
       }
     }
   }
-  for (int s = 0; s < 7; ++s) L.a[s * L.n + v] = L.mask[v] ? (s == 0 ? 1.0 : 0.0) : st[s];
+  for (int s = 0; s < kStencil; ++s) L.a[s * L.n + v] = L.mask[v] ? (s == 0 ? 1.0 : 0.0) : st[s];
   L.dinv[v] = L.mask[v] ? 0.0 : 1.0 / st[0];
 }
 
-// Galerkin coarse stencil R A P on the next level (coarse node = fine (2I,2J)).
+// Galerkin coarse stencil R A P on the next level: R = P^T with the weights
+// of prolong_at.
 __global__ void mg_rap_kernel(MgLevel F, MgLevel Cc) {
   const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (C >= Cc.n) return;
-  const int cnx = Cc.nx, cny = Cc.ny, fnx = F.nx, fny = F.ny;
+  const int cnx = Cc.nx, fnx = F.nx, fny = F.ny;
   const int I = (int)(C % (cnx + 1)), J = (int)(C / (cnx + 1));
   const unsigned char cm = Cc.mask[C];  // written by mg_cmask_kernel
-  double st[7] = {0, 0, 0, 0, 0, 0, 0};
+  double st[kStencil] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (!cm) {
+    const int X = fine_of(I, fnx), Y = fine_of(J, fny);
     const int sx[7] = {0, 1, -1, 0, 0, 1, -1}, sy[7] = {0, 0, 0, 1, -1, -1, 1};
     for (int t = 0; t < 7; ++t) {  // fine support of C
-      const int fx = 2 * I + sx[t], fy = 2 * J + sy[t];
+      const int fx = X + sx[t], fy = Y + sy[t];
       if (fx < 0 || fx > fnx || fy < 0 || fy > fny) continue;
       const int64_t fi = (int64_t)fy * (fnx + 1) + fx;
       if (F.mask[fi]) continue;
-      const double w1 = t == 0 ? 1.0 : 0.5;
-      for (int s = 0; s < 7; ++s) {
+      const double w1 = pweight(fnx, fny, fx, fy, I, J);
+      if (w1 == 0.0) continue;
+      for (int s = 0; s < F.npts; ++s) {
         const double av = F.a[s * F.n + fi];
         if (av == 0.0) continue;
         const int gx = fx + kOffX[s], gy = fy + kOffY[s];
         if (gx < 0 || gx > fnx || gy < 0 || gy > fny) continue;
         if (F.mask[(int64_t)gy * (fnx + 1) + gx]) continue;
-        // coarse parents of fine node g
-        int px[2], py[2], np = 0;
-        double w2;
-        if (!(gx & 1) && !(gy & 1)) { px[0] = gx / 2; py[0] = gy / 2; np = 1; w2 = 1.0; }
-        else if ((gx & 1) && !(gy & 1)) { px[0] = (gx - 1) / 2; py[0] = gy / 2; px[1] = (gx + 1) / 2; py[1] = gy / 2; np = 2; w2 = 0.5; }
-        else if (!(gx & 1) && (gy & 1)) { px[0] = gx / 2; py[0] = (gy - 1) / 2; px[1] = gx / 2; py[1] = (gy + 1) / 2; np = 2; w2 = 0.5; }
-        else { px[0] = (gx + 1) / 2; py[0] = (gy - 1) / 2; px[1] = (gx - 1) / 2; py[1] = (gy + 1) / 2; np = 2; w2 = 0.5; }
+        // coarse parents of fine node g, as prolong_at weighs them
+        const Par1 px = parents1(gx, fnx), py = parents1(gy, fny);
+        int qx[2], qy[2], np = 2;
+        double w2 = 0.5;
+        if (px.a == px.b && py.a == py.b) { qx[0] = px.a; qy[0] = py.a; np = 1; w2 = 1.0; }
+        else if (py.a == py.b) { qx[0] = px.a; qx[1] = px.b; qy[0] = qy[1] = py.a; }
+        else if (px.a == px.b) { qx[0] = qx[1] = px.a; qy[0] = py.a; qy[1] = py.b; }
+        else { qx[0] = px.b; qy[0] = py.a; qx[1] = px.a; qy[1] = py.b; }
         for (int q = 0; q < np; ++q) {
-          if (Cc.mask[(int64_t)py[q] * (cnx + 1) + px[q]]) continue;  // zero column of P
-          const int sl = slot_of(px[q] - I, py[q] - J);
+          if (Cc.mask[(int64_t)qy[q] * (cnx + 1) + qx[q]]) continue;  // zero column of P
+          const int sl = slot_of(qx[q] - I, qy[q] - J);
           if (sl >= 0) st[sl] = fma(w1 * av, w2, st[sl]);
         }
       }
     }
   }
-  for (int s = 0; s < 7; ++s) Cc.a[s * Cc.n + C] = cm ? (s == 0 ? 1.0 : 0.0) : st[s];
+  for (int s = 0; s < kStencil; ++s) Cc.a[s * Cc.n + C] = cm ? (s == 0 ? 1.0 : 0.0) : st[s];
   Cc.dinv[C] = cm ? 0.0 : 1.0 / st[0];
 }
 
-// LDS image of the tail levels: per level its stencils (7 n) then dinv (n).
+// LDS image of the tail levels: per level its stencils (kStencil n) then dinv (n).
 __global__ void mg_tailpack_kernel(MgData M) {
   int o = 0;
   for (int l = M.tail; l < M.nlev; ++l) {
     const int n = (int)M.lev[l].n;
-    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < 8 * n; u += gridDim.x * blockDim.x)
-      M.tailpack[o + u] = u < 7 * n ? M.lev[l].a[u] : M.lev[l].dinv[u - 7 * n];
-    o += 8 * n;
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < (kStencil + 1) * n; u += gridDim.x * blockDim.x)
+      M.tailpack[o + u] = u < kStencil * n ? M.lev[l].a[u] : M.lev[l].dinv[u - kStencil * n];
+    o += (kStencil + 1) * n;
   }
 }
 
-// Coarse masks (a coarse node is fixed when its fine node (2I, 2J) is),
-// written before mg_rap_kernel reads the masks of neighbouring nodes.
+// Coarse masks (a coarse node is fixed when its fine node is), written
+// before mg_rap_kernel reads the masks of neighbouring nodes.
 __global__ void mg_cmask_kernel(MgLevel F, MgLevel Cc) {
   const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (C >= Cc.n) return;
   const int I = (int)(C % (Cc.nx + 1)), J = (int)(C / (Cc.nx + 1));
-  Cc.mask[C] = F.mask[(int64_t)(2 * J) * (F.nx + 1) + 2 * I];
+  Cc.mask[C] = F.mask[(int64_t)fine_of(J, F.ny) * (F.nx + 1) + fine_of(I, F.nx)];
 }
 
 // S P0 restricted to each row: sp[j N + r] = sum over the free gathered
@@ -529,18 +576,11 @@ __device__ inline double stencil_apply(const MgLevel& L, int64_t v, int i, int j
   if (j > 0) s = fma(L.a[4 * L.n + v], xv(v - row, i, j - 1), s);
   if (i < L.nx && j > 0) s = fma(L.a[5 * L.n + v], xv(v + 1 - row, i + 1, j - 1), s);
   if (i > 0 && j < L.ny) s = fma(L.a[6 * L.n + v], xv(v - 1 + row, i - 1, j + 1), s);
+  if (L.npts > 7) {
+    if (i < L.nx && j < L.ny) s = fma(L.a[7 * L.n + v], xv(v + 1 + row, i + 1, j + 1), s);
+    if (i > 0 && j > 0) s = fma(L.a[8 * L.n + v], xv(v - 1 - row, i - 1, j - 1), s);
+  }
   return s;
-}
-
-// linear interpolation of the coarse vector ec at fine node (x, y)
-__device__ inline double prolong_at(const MgLevel& C, const double* ec, int x, int y) {
-  const int64_t cs = C.nx + 1;
-  const int hx = x >> 1, hy = y >> 1;
-  const int64_t c0 = (int64_t)hy * cs + hx;
-  if (!(x & 1) && !(y & 1)) return ec[c0];
-  if (!(y & 1)) return 0.5 * (ec[c0] + ec[c0 + 1]);
-  if (!(x & 1)) return 0.5 * (ec[c0] + ec[c0 + cs]);
-  return 0.5 * (ec[c0 + 1] + ec[c0 + cs]);
 }
 
 // two pre-smoothing sweeps from zero in one pass: e1 = om dinv r is formed at
@@ -580,8 +620,8 @@ __device__ inline double fine_res(const MgLevel& F, int fx, int fy, bool one_swe
   return F.r[fi] - stencil_apply(F, fi, fx, fy, ev);
 }
 
-// coarse r = R (r - A e): full weight at the fine node (2I, 2J), half at its
-// six stencil neighbours (the transpose of prolong_at).  Eight lanes per
+// coarse r = R (r - A e), R = P^T: the fine node of C and its six stencil
+// neighbours, weighted as prolong_at weighs C there.  Eight lanes per
 // coarse node: lane t < 7 forms the fine residual at stencil offset t, and a
 // fixed shuffle tree sums them.
 __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, int one_sweep, double om,
@@ -592,10 +632,11 @@ __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, 
   const int t = (int)(g & 7);
   double val = 0.0;
   if (C < Cc.n && t < 7 && !Cc.mask[C]) {
-    const int x = 2 * (int)(C % (Cc.nx + 1)) + (t == 1 || t == 5 ? 1 : (t == 2 || t == 6 ? -1 : 0));
-    const int y = 2 * (int)(C / (Cc.nx + 1)) + (t == 3 || t == 6 ? 1 : (t == 4 || t == 5 ? -1 : 0));
-    val = fine_res(F, x, y, one_sweep != 0, om);
-    if (t) val *= 0.5;
+    const int I = (int)(C % (Cc.nx + 1)), J = (int)(C / (Cc.nx + 1));
+    const int x = fine_of(I, F.nx) + (t == 1 || t == 5 ? 1 : (t == 2 || t == 6 ? -1 : 0));
+    const int y = fine_of(J, F.ny) + (t == 3 || t == 6 ? 1 : (t == 4 || t == 5 ? -1 : 0));
+    const double w = (x < 0 || x > F.nx || y < 0 || y > F.ny) ? 0.0 : pweight(F.nx, F.ny, x, y, I, J);
+    if (w != 0.0) val = w * fine_res(F, x, y, one_sweep != 0, om);
   }
   val += __shfl_xor(val, 1, 8);
   val += __shfl_xor(val, 2, 8);
@@ -616,11 +657,13 @@ __global__ __launch_bounds__(256) void v_post_kernel(MgLevel F, MgLevel Cc, int 
   if (one_sweep) {
     auto ep = [&](int64_t u, int x, int y) {
       const double e = om * F.dinv[u] * F.r[u];
-      return F.mask[u] ? e : e + prolong_at(Cc, Cc.e, x, y);
+      return F.mask[u] ? e : e + prolong_at(F.nx, F.ny, Cc.nx, Cc.e, x, y);
     };
     F.e[v] = ep(v, i, j) + om * F.dinv[v] * (F.r[v] - stencil_apply(F, v, i, j, ep));
   } else {
-    auto ep = [&](int64_t u, int x, int y) { return F.mask[u] ? F.e[u] : F.e[u] + prolong_at(Cc, Cc.e, x, y); };
+    auto ep = [&](int64_t u, int x, int y) {
+      return F.mask[u] ? F.e[u] : F.e[u] + prolong_at(F.nx, F.ny, Cc.nx, Cc.e, x, y);
+    };
     F.t[v] = ep(v, i, j) + om * F.dinv[v] * (F.r[v] - stencil_apply(F, v, i, j, ep));
   }
 }
@@ -646,7 +689,7 @@ __global__ __launch_bounds__(1024) void v_coarsest_kernel(MgLevel L, double om, 
 // Level of the one-workgroup tail: stencils, diagonal inverse and vectors
 // all in LDS.  A fixed node has dinv = 0 (its stencil row is the identity).
 struct TailLevel {
-  int nx, ny, n;
+  int nx, ny, n, npts;
   const double* a;     // a[s*n + v]
   const double* dinv;
 };
@@ -660,6 +703,10 @@ __device__ inline double tail_apply(const TailLevel& L, const double* x, int v, 
   if (j > 0) s = fma(L.a[4 * L.n + v], x[v - row], s);
   if (i < L.nx && j > 0) s = fma(L.a[5 * L.n + v], x[v + 1 - row], s);
   if (i > 0 && j < L.ny) s = fma(L.a[6 * L.n + v], x[v - 1 + row], s);
+  if (L.npts > 7) {
+    if (i < L.nx && j < L.ny) s = fma(L.a[7 * L.n + v], x[v + 1 + row], s);
+    if (i > 0 && j > 0) s = fma(L.a[8 * L.n + v], x[v - 1 - row], s);
+  }
   return s;
 }
 
@@ -677,34 +724,35 @@ __device__ inline void tail_sweep(const TailLevel& L, const double* in, double* 
 // restriction of the residual, coarse_sweeps sweeps on the coarsest level,
 // prolongation and two post-smoothing sweeps.
 __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const CgState* st) {
-  __shared__ double pool[11 * kTailNodes];
+  __shared__ double pool[(kStencil + 4) * kTailNodes];
   if (st->done) return;
   const double om = M.omega_v;
   double* const E = pool;                    // scratch / residual
   double* const Tt = pool + kTailNodes;      // iterate
   double* const R = pool + 2 * kTailNodes;   // right-hand sides
-  double* const SA = pool + 3 * kTailNodes;  // stencils (7 n) and dinv (n) per level
+  double* const SA = pool + 3 * kTailNodes;  // stencils (kStencil n) and dinv (n) per level
   {
     int tn = 0;
     for (int l = lt; l < M.nlev; ++l) tn += (int)M.lev[l].n;
     const double* __restrict__ src = M.tailpack;
 #pragma unroll 8
-    for (int u = threadIdx.x; u < 8 * tn; u += blockDim.x) SA[u] = src[u];
+    for (int u = threadIdx.x; u < (kStencil + 1) * tn; u += blockDim.x) SA[u] = src[u];
   }
   {
     const MgLevel L = M.lev[lt];
     for (int v = threadIdx.x; v < (int)L.n; v += blockDim.x) R[v] = L.r[v];
   }
   __syncthreads();
-  // level l's stencil block starts at 8 * (nodes of levels lt..l-1); its
+  // level l's stencil block starts at (kStencil + 1) * (nodes of levels lt..l-1); its
   // vectors at the same node offset o
   auto lvl = [&](int l, int o) {
     TailLevel L;
     L.nx = M.lev[l].nx;
     L.ny = M.lev[l].ny;
     L.n = (int)M.lev[l].n;
-    L.a = SA + 8 * o;
-    L.dinv = SA + 8 * o + 7 * L.n;
+    L.npts = M.lev[l].npts;
+    L.a = SA + (kStencil + 1) * o;
+    L.dinv = SA + (kStencil + 1) * o + kStencil * L.n;
     return L;
   };
   int o = 0;
@@ -729,13 +777,15 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
     for (int C = threadIdx.x; C < Cc.n; C += blockDim.x) {
       double s = 0.0;
       if (Cc.dinv[C] != 0.0) {
-        const int x = 2 * (C % (Cc.nx + 1)), y = 2 * (C / (Cc.nx + 1));
+        const int I = C % (Cc.nx + 1), J = C / (Cc.nx + 1);
+        const int x = fine_of(I, L.nx), y = fine_of(J, L.ny);
         auto res = [&](int fx, int fy) {
-          return (fx < 0 || fx > L.nx || fy < 0 || fy > L.ny) ? 0.0 : e[fy * (L.nx + 1) + fx];
+          return (fx < 0 || fx > L.nx || fy < 0 || fy > L.ny)
+                     ? 0.0
+                     : pweight(L.nx, L.ny, fx, fy, I, J) * e[fy * (L.nx + 1) + fx];
         };
-        const double nb = res(x + 1, y) + res(x - 1, y) + res(x, y + 1) + res(x, y - 1) + res(x + 1, y - 1) +
-                          res(x - 1, y + 1);
-        s = fma(0.5, nb, res(x, y));
+        s = res(x, y) + res(x + 1, y) + res(x - 1, y) + res(x, y + 1) + res(x, y - 1) + res(x + 1, y - 1) +
+            res(x - 1, y + 1);
       }
       rc[C] = s;
     }
@@ -766,17 +816,9 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
     const TailLevel Cc = lvl(l + 1, oc);
     double *e = E + o, *t = Tt + o;
     const double* r = R + o;
-    const int cs = Cc.nx + 1;
     for (int v = threadIdx.x; v < L.n; v += blockDim.x) {
       if (L.dinv[v] == 0.0) continue;
-      const int x = v % (L.nx + 1), y = v / (L.nx + 1);
-      const int c0 = (y >> 1) * cs + (x >> 1);
-      double add;
-      if (!(x & 1) && !(y & 1)) add = ec[c0];
-      else if (!(y & 1)) add = 0.5 * (ec[c0] + ec[c0 + 1]);
-      else if (!(x & 1)) add = 0.5 * (ec[c0] + ec[c0 + cs]);
-      else add = 0.5 * (ec[c0 + 1] + ec[c0 + cs]);
-      t[v] += add;
+      t[v] += prolong_at(L.nx, L.ny, Cc.nx, ec, v % (L.nx + 1), v / (L.nx + 1));
     }
     __syncthreads();
     if (M.vsweeps > 1) {

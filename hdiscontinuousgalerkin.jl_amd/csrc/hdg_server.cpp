@@ -1,0 +1,369 @@
+// Serving loop of libhdg (include/hdg.h, hdg_server_*): a queue of Poisson
+// solves on rectangle meshes, worker slots that each own a context and its
+// HIP stream, admission by device-memory budget, per-request latency.
+// Host code only; every device operation goes through the C ABI.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hdg.h"
+
+namespace {
+
+constexpr int kLookahead = 8;  // queued requests a free slot may pick from
+
+struct Pending {
+  int64_t id;
+  hdg_request req;
+  double t_submit;
+  size_t foot;
+};
+
+struct Done {
+  hdg_result res;
+  std::vector<double> trace;
+};
+
+struct Slot {
+  hdg_ctx* ctx = nullptr;
+  size_t held = 0;  // device bytes the slot's context holds
+  // what the context currently holds
+  bool has_mesh = false, has_setup = false;
+  int64_t nx = 0, ny = 0;
+  double jitter = 0.0;
+  uint64_t seed = 0;
+  int8_t nt = 0;
+  hdg_options opt{};
+  std::vector<int64_t> bfaces;  // 0-based boundary faces of the mesh
+};
+
+bool same_options(const hdg_options& a, const hdg_options& b) {
+  return a.k == b.k && a.quad_degree == b.quad_degree && a.tau == b.tau && a.precision == b.precision &&
+         a.rtol == b.rtol && a.maxit == b.maxit && a.schedule == b.schedule && a.precond == b.precond;
+}
+
+}  // namespace
+
+struct hdg_server {
+  int device = 0;
+  size_t budget = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Pending> queue;
+  std::unordered_map<int64_t, Done> done;
+  std::unordered_map<int64_t, int> open;  // submitted, not yet waited
+  size_t reserved = 0, peak = 0;
+  bool stopping = false;
+  int64_t next_id = 1;
+  std::vector<Slot> slots;
+  std::vector<std::thread> threads;
+  std::vector<double> latency, qwait;
+  int64_t completed = 0, failed = 0, cells = 0;
+  double first_start = 1e300, last_done = 0.0;
+  std::chrono::steady_clock::time_point t0;
+  std::string err;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+// Runs one request in slot s; returns the result (device bytes filled in).
+hdg_result run_request(hdg_server* S, Slot& sl, const Pending& p, std::vector<double>* trace) {
+  hdg_result r{};
+  r.id = p.id;
+  r.t_submit = p.t_submit;
+  r.t_start = S->now();
+  const hdg_request& q = p.req;
+  int st = HDG_OK;
+  if (!sl.ctx) st = hdg_create(&sl.ctx, S->device, 0);
+  const bool mesh_hit = sl.has_mesh && sl.nx == q.nx && sl.ny == q.ny && sl.jitter == q.jitter &&
+                        sl.seed == q.seed;
+  if (st == HDG_OK && !mesh_hit) {
+    sl.has_mesh = sl.has_setup = false;
+    const int64_t nx = q.nx, ny = q.ny, nn = (nx + 1) * (ny + 1), nc = 2 * nx * ny;
+    const int64_t nf = hdg_rectangle_nfaces(nx, ny);
+    std::vector<double> xy(2 * nn);
+    std::vector<int64_t> cells(6 * nc), faces(4 * nf);
+    std::vector<int8_t> side(nf);
+    st = hdg_rectangle_mesh(nx, ny, 0.0, 0.0, 1.0, 1.0, q.jitter, q.seed, xy.data(), cells.data(), faces.data(),
+                            side.data());
+    if (st == HDG_OK) st = hdg_set_mesh(sl.ctx, nn, xy.data(), nc, cells.data(), nf, faces.data());
+    if (st == HDG_OK) {
+      sl.bfaces.clear();
+      for (int64_t f = 0; f < nf; ++f)
+        if (side[f]) sl.bfaces.push_back(f);
+      sl.has_mesh = true;
+      sl.nx = nx;
+      sl.ny = ny;
+      sl.jitter = q.jitter;
+      sl.seed = q.seed;
+    }
+  }
+  if (st == HDG_OK && !(sl.has_setup && same_options(sl.opt, q.opt))) {
+    sl.has_setup = false;
+    st = hdg_setup(sl.ctx, &q.opt);
+    if (st == HDG_OK) {
+      sl.has_setup = true;
+      sl.opt = q.opt;
+    }
+  }
+  hdg_sizes sz{};
+  if (st == HDG_OK) st = hdg_query_sizes(sl.ctx, &sz);
+  if (st == HDG_OK) st = hdg_set_source(sl.ctx, &q.src);
+  if (st == HDG_OK) {
+    // u = 0 on the boundary (every serving problem): all trace dofs of the
+    // boundary faces, 1-based
+    const int nt = sz.nt;
+    std::vector<int64_t> dofs;
+    dofs.reserve(sl.bfaces.size() * nt);
+    for (int64_t f : sl.bfaces)
+      for (int j = 0; j < nt; ++j) dofs.push_back(f * nt + j + 1);
+    std::vector<double> zero(dofs.size(), 0.0);
+    st = hdg_set_dirichlet(sl.ctx, (int64_t)dofs.size(), dofs.data(), zero.data());
+  }
+  hdg_solve_info info{};
+  if (st == HDG_OK) st = hdg_run(sl.ctx, &info);
+  if (st == HDG_OK && q.uex_kind >= 0) st = hdg_errornorm(sl.ctx, q.uex_kind, nullptr, &r.errornorm);
+  if (st == HDG_OK && q.want_trace && trace) {
+    trace->resize(sz.n_dofs);
+    st = hdg_get_trace(sl.ctx, trace->data());
+  }
+  size_t used = 0;
+  if (sl.ctx) hdg_memory_used(sl.ctx, &used);
+  r.status = st;
+  r.iters = info.iters;
+  r.relres = info.relres;
+  r.true_relres = info.true_relres;
+  r.n_cells = sz.n_cells;
+  r.n_dofs = sz.n_dofs;
+  r.device_bytes = used;
+  r.t_done = S->now();
+  return r;
+}
+
+void worker(hdg_server* S, int w) {
+  Slot& sl = S->slots[w];
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    if (S->stopping && S->queue.empty()) break;
+    // first of the next few queued requests whose footprint fits next to
+    // what the other slots hold
+    int pick = -1;
+    const int look = std::min<int>(kLookahead, (int)S->queue.size());
+    for (int j = 0; j < look; ++j) {
+      const size_t need = S->reserved - sl.held + S->queue[j].foot;
+      if (S->budget == 0 || need <= S->budget) {
+        pick = j;
+        break;
+      }
+    }
+    if (pick < 0) {
+      if (!S->queue.empty() && sl.held > 0) {
+        // release this idle slot's device data so that the queue can move
+        hdg_ctx* c = sl.ctx;
+        sl.ctx = nullptr;
+        sl.has_mesh = sl.has_setup = false;
+        S->reserved -= sl.held;
+        sl.held = 0;
+        lk.unlock();
+        hdg_destroy(c);
+        lk.lock();
+        S->cv_work.notify_all();
+        continue;
+      }
+      S->cv_work.wait(lk);
+      continue;
+    }
+    Pending p = S->queue[pick];
+    S->queue.erase(S->queue.begin() + pick);
+    S->reserved = S->reserved - sl.held + p.foot;
+    sl.held = p.foot;
+    S->peak = std::max(S->peak, S->reserved);
+    lk.unlock();
+
+    std::vector<double> trace;
+    hdg_result r = run_request(S, sl, p, &trace);
+    r.slot = w;
+
+    lk.lock();
+    // hold what the context really holds from now on
+    S->reserved = S->reserved - sl.held + r.device_bytes;
+    sl.held = r.device_bytes;
+    S->peak = std::max(S->peak, S->reserved);
+    if (r.status == HDG_OK) {
+      ++S->completed;
+      S->cells += r.n_cells;
+      S->latency.push_back(r.t_done - r.t_submit);
+      S->qwait.push_back(r.t_start - r.t_submit);
+      S->first_start = std::min(S->first_start, r.t_start);
+      S->last_done = std::max(S->last_done, r.t_done);
+    } else {
+      ++S->failed;
+      // a failed context may hold partial state: start the slot afresh
+      hdg_ctx* c = sl.ctx;
+      sl.ctx = nullptr;
+      sl.has_mesh = sl.has_setup = false;
+      S->reserved -= sl.held;
+      sl.held = 0;
+      lk.unlock();
+      hdg_destroy(c);
+      lk.lock();
+    }
+    Done& d = S->done[r.id];
+    d.res = r;
+    d.trace.swap(trace);
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+  }
+  lk.unlock();
+  if (sl.ctx) hdg_destroy(sl.ctx);
+  sl.ctx = nullptr;
+}
+
+double percentile(std::vector<double> v, double q) {
+  if (v.empty()) return 0.0;
+  std::sort(v.begin(), v.end());
+  const size_t i = (size_t)std::min<double>((double)v.size() - 1, std::ceil(q * v.size()) - 1);
+  return v[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t hdg_server_footprint(const hdg_request* q) {
+  if (!q || q->nx < 1 || q->ny < 1 || q->opt.k < 1 || q->opt.k > 4) return 0;
+  const double nx = (double)q->nx, ny = (double)q->ny;
+  const double nn = (nx + 1) * (ny + 1), nc = 2 * nx * ny, nf = nx + ny + 3 * nx * ny;
+  const int k = q->opt.k, nu = (k + 1) * (k + 2) / 2, m = 3 * nu, nt = k + 1, ntr = 3 * nt;
+  const double N = nf * nt;
+  // mesh, face pattern, vertex structures
+  double b = 16 * nn + 24 * nc + 8 * nf + 67 * nf + 32 * nf + 4 * nn;
+  // vertex hierarchy over ~4/3 nn nodes: 9-point stencil, inverse, three
+  // vectors (13 doubles) and a mask byte per node
+  b += (13.0 * 8 + 1) * nn * 4.0 / 3.0;
+  // per-cell operators, trace system, solver and multigrid trace-level data
+  b += 8 * nc * ((double)ntr * ntr + ntr + (double)m * ntr + 2 * m + 9);
+  b += N * (1 + 8 + 8 * 5 * nt + 8 + 8 * nt + 6 * 8 + 8 + 2 * 8 + 4 * 8);
+  if (q->src.kind == HDG_SRC_VALUES) b += 8 * nc * 16;
+  if (q->opt.precision == HDG_MIXED) b += N * (4 * 5 * nt + 4 * nt + 4 * 4 + 4 * 4);
+  // allocation granularity and small buffers
+  return (size_t)(b * 1.05) + (8u << 20);
+}
+
+int hdg_server_create(hdg_server** out, int device, size_t hbm_budget, int slots) {
+  if (!out || slots < 1 || slots > 64) return HDG_ERR_ARG;
+  hdg_server* S = new hdg_server();
+  S->device = device;
+  S->budget = hbm_budget;
+  S->t0 = std::chrono::steady_clock::now();
+  S->slots.resize(slots);
+  for (int w = 0; w < slots; ++w) S->threads.emplace_back(worker, S, w);
+  *out = S;
+  return HDG_OK;
+}
+
+int hdg_server_submit(hdg_server* S, const hdg_request* req, int64_t* id) {
+  if (!S || !req || !id) return HDG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (req->nx < 1 || req->ny < 1) {
+    S->err = "request mesh must have nx, ny >= 1";
+    return HDG_ERR_ARG;
+  }
+  const size_t foot = hdg_server_footprint(req);
+  if (foot == 0) {
+    S->err = "unsupported request (k must be 1..4)";
+    return HDG_ERR_UNSUPPORTED;
+  }
+  if (S->budget && foot > S->budget) {
+    S->err = "request footprint exceeds the server's HBM budget";
+    return HDG_ERR_HBM_BUDGET;
+  }
+  Pending p{S->next_id++, *req, S->now(), foot};
+  *id = p.id;
+  S->open[p.id] = 1;
+  S->queue.push_back(p);
+  S->cv_work.notify_one();
+  return HDG_OK;
+}
+
+int hdg_server_wait(hdg_server* S, int64_t id, double timeout_s, hdg_result* out) {
+  if (!S) return HDG_ERR_ARG;
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (!S->open.count(id) && !S->done.count(id)) {
+    S->err = "unknown request id";
+    return HDG_ERR_STATE;
+  }
+  auto ready = [&] { return S->done.count(id) != 0; };
+  if (timeout_s < 0) {
+    S->cv_done.wait(lk, ready);
+  } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+    S->err = "timed out waiting for the request";
+    return HDG_ERR_NOT_CONVERGED;
+  }
+  Done& d = S->done[id];
+  if (out) *out = d.res;
+  S->open.erase(id);
+  if (d.trace.empty()) S->done.erase(id);
+  return HDG_OK;
+}
+
+int hdg_server_trace(hdg_server* S, int64_t id, double* uhat) {
+  if (!S || !uhat) return HDG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->done.find(id);
+  if (it == S->done.end() || it->second.trace.empty()) {
+    S->err = "no kept trace for this request (want_trace = 0, not completed, or already taken)";
+    return HDG_ERR_STATE;
+  }
+  std::memcpy(uhat, it->second.trace.data(), sizeof(double) * it->second.trace.size());
+  S->done.erase(it);
+  return HDG_OK;
+}
+
+int hdg_server_stats(hdg_server* S, hdg_server_summary* o) {
+  if (!S || !o) return HDG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  std::memset(o, 0, sizeof(*o));
+  o->completed = S->completed;
+  o->failed = S->failed;
+  o->cells = S->cells;
+  o->p50_latency = percentile(S->latency, 0.50);
+  o->p99_latency = percentile(S->latency, 0.99);
+  double sl = 0.0, sw = 0.0;
+  for (double v : S->latency) sl += v;
+  for (double v : S->qwait) sw += v;
+  o->mean_latency = S->latency.empty() ? 0.0 : sl / S->latency.size();
+  o->mean_queue_wait = S->qwait.empty() ? 0.0 : sw / S->qwait.size();
+  o->busy_seconds = S->completed ? S->last_done - S->first_start : 0.0;
+  o->cells_per_second = o->busy_seconds > 0 ? S->cells / o->busy_seconds : 0.0;
+  o->peak_reserved = S->peak;
+  return HDG_OK;
+}
+
+void hdg_server_destroy(hdg_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stopping = true;
+  }
+  S->cv_work.notify_all();
+  for (auto& t : S->threads) t.join();
+  delete S;
+}
+
+const char* hdg_server_last_error(const hdg_server* S) { return S ? S->err.c_str() : "null server"; }
+
+}  // extern "C"

@@ -179,6 +179,9 @@ int hdg_solve(hdg_ctx* ctx, double* uhat_out, hdg_solve_info* info);
    result of the caller's own K \ b), so that hdg_recover applies get_uσ! to
    it.  Requires hdg_assemble_condense. */
 int hdg_set_trace(hdg_ctx* ctx, const double* uhat);
+/* The trace solution û of the last hdg_solve / hdg_run / hdg_set_trace
+   (the `K \ b` of poisson2D_HDG.jl:195), n_dofs doubles. */
+int hdg_get_trace(hdg_ctx* ctx, double* uhat);
 /* get_uσ! (poisson2D_HDG.jl:197-212): q (ncells x nq), u (ncells x nu),
    uhat_cell (ncells x nt x 3), all column-major; any may be NULL. */
 int hdg_recover(hdg_ctx* ctx, double* q, double* u, double* uhat_cell);
@@ -209,6 +212,70 @@ int hdg_quadrature(int dim, int order, int cap, double* pts, double* w);
    point for hardware traffic counters.  Timing runs overwrite the solver
    state; solve again before reading results. */
 int hdg_time_kernel(hdg_ctx* ctx, int which, int reps, double* avg_us);
+
+/* ---- serving: a request loop over one device ---------------------------- */
+/* No counterpart in the reference (it solves one problem per script run).
+   A server owns `slots` worker threads, each with its own context and HIP
+   stream, so solves in different slots overlap on the device.  A request is
+   admitted when its estimated device footprint fits the HBM budget next to
+   the footprints the other slots hold (continuous admission: a slot takes
+   the first of the next few queued requests that fits as soon as it is
+   free).  A slot keeps its mesh and set-up while consecutive requests share
+   them, so repeated meshes stay device-resident.  Results are bitwise the
+   same whatever else runs in the other slots (no atomics, fixed reduction
+   order). */
+typedef struct hdg_server hdg_server;
+
+typedef struct {
+  int64_t nx, ny;      /* rectangle_mesh(TriangleCell, (nx, ny), (0,0), (1,1)) */
+  double jitter;       /* interior nodes moved by up to jitter*h (0: none) */
+  uint64_t seed;       /* jitter stream */
+  hdg_options opt;     /* k, quad_degree, tau, precision, rtol, maxit, precond */
+  hdg_source src;      /* f (and kappa); f_qp must stay valid until completion */
+  int32_t uex_kind;    /* -1: no error norm; else HDG_UEX_* for hdg_errornorm */
+  int32_t want_trace;  /* 1: keep a host copy of the trace solution for hdg_server_trace */
+} hdg_request;
+
+typedef struct {
+  int64_t id;
+  int32_t status;      /* hdg_status of the solve */
+  int32_t iters;
+  double relres, true_relres;
+  double errornorm;    /* squared L2 error (uex_kind >= 0), else 0 */
+  int64_t n_cells, n_dofs;
+  double t_submit, t_start, t_done;  /* seconds since hdg_server_create */
+  size_t device_bytes; /* device memory the slot held for it */
+  int32_t slot;
+} hdg_result;
+
+typedef struct {
+  int64_t completed, failed;
+  int64_t cells;             /* cells of the completed requests */
+  double p50_latency, p99_latency, mean_latency;  /* seconds, submit -> done */
+  double mean_queue_wait;    /* seconds, submit -> start */
+  double busy_seconds;       /* first start -> last done */
+  double cells_per_second;   /* cells / busy_seconds */
+  size_t peak_reserved;      /* largest sum of admitted footprints */
+} hdg_server_summary;
+
+/* hbm_budget 0 = no cap; slots >= 1. */
+int hdg_server_create(hdg_server** srv, int device, size_t hbm_budget, int slots);
+/* Queues a request (copied); returns at once with its id. */
+int hdg_server_submit(hdg_server* srv, const hdg_request* req, int64_t* id);
+/* Blocks until request `id` completes (timeout_s < 0: no timeout), fills
+   `out` and forgets the request unless it kept a trace.  HDG_ERR_STATE on
+   an unknown id, HDG_ERR_NOT_CONVERGED on timeout. */
+int hdg_server_wait(hdg_server* srv, int64_t id, double timeout_s, hdg_result* out);
+/* Copies the kept trace solution of a completed request (n_dofs doubles)
+   and releases it. */
+int hdg_server_trace(hdg_server* srv, int64_t id, double* uhat);
+/* Device bytes a request would hold (mesh, per-cell operators, trace system,
+   solver and multigrid data). */
+size_t hdg_server_footprint(const hdg_request* req);
+int hdg_server_stats(hdg_server* srv, hdg_server_summary* out);
+/* Finishes the queued requests, joins the slots, frees everything. */
+void hdg_server_destroy(hdg_server* srv);
+const char* hdg_server_last_error(const hdg_server* srv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,98 @@
+"""GPU tests of the serving loop (hdg_server_*): results identical alone
+and in a mix of concurrent requests, HBM-budget admission, error isolation.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+QD = {1: 2, 2: 3, 3: 6, 4: 6}
+
+
+def _req(hdg, nx, ny, k, seed, want_trace=False, precision=0, ka=3.0, kb=5.0):
+    rng = np.random.default_rng(seed)
+    return hdg.make_request(nx=nx, ny=ny, k=k, jitter=0.2, seed=seed, src_kind=hdg._lib.SRC_SINE_SERIES,
+                            kappa=(ka, kb), coef=rng.standard_normal(16), want_trace=want_trace,
+                            precision=precision)
+
+
+def test_server_alone_equals_mix_and_context(hdg):
+    """A request gives the same trace solution, bit for bit, run alone in a
+    one-slot server, in the middle of 12 concurrent requests of other sizes,
+    orders and precisions in a 4-slot server, and through a plain Context."""
+    x = _req(hdg, 48, 40, 2, 9, want_trace=True)
+    with hdg.Server(slots=1) as s1:
+        r1 = s1.wait(s1.submit(x))
+        assert r1["status"] == 0
+        t_alone = s1.trace(r1["id"], r1["n_dofs"])
+    others = [(64, 64, 1), (33, 17, 3), (96, 80, 2), (20, 100, 4), (128, 128, 2), (50, 50, 1),
+              (40, 72, 3), (81, 81, 2), (16, 16, 4), (100, 60, 1), (64, 32, 3)]
+    with hdg.Server(slots=4) as s4:
+        ids = []
+        for i, (nx, ny, k) in enumerate(others):
+            ids.append(s4.submit(_req(hdg, nx, ny, k, 100 + i, precision=i % 2)))
+            if i == 5:
+                xid = s4.submit(x)
+        res = [s4.wait(i) for i in ids]
+        rx = s4.wait(xid)
+        t_mix = s4.trace(xid, rx["n_dofs"])
+        st = s4.stats()
+    assert all(r["status"] == 0 for r in res) and rx["status"] == 0
+    assert st["completed"] == len(others) + 1 and st["failed"] == 0
+    assert np.array_equal(t_alone, t_mix)
+    assert rx["iters"] == r1["iters"]
+    # the same problem through the Context API
+    m = hdg.rectangle_mesh((48, 40), jitter=0.2, seed=9)
+    ctx = hdg.Context(0)
+    ctx.set_mesh(m)
+    ctx.setup(2, quad_degree=3, rtol=1e-12)
+    rng = np.random.default_rng(9)
+    ctx.set_source(kind=hdg._lib.SRC_SINE_SERIES, kappa=(3.0, 5.0), coef=rng.standard_normal(16))
+    ctx.set_dirichlet(m.boundary_dofs(3), np.zeros(len(m.boundary_dofs(3))))
+    ctx.run()
+    t_ctx = ctx.trace()
+    ctx.close()
+    assert np.array_equal(t_alone, t_ctx)
+    # overlap happened: some requests started before others finished
+    iv = sorted((r["t_start"], r["t_done"]) for r in res + [rx])
+    assert any(iv[i + 1][0] < iv[i][1] for i in range(len(iv) - 1))
+
+
+def test_server_budget_admission(hdg):
+    """With a budget below two footprints, requests run one at a time."""
+    reqs = [_req(hdg, 96, 96, 2, 200 + i) for i in range(5)]
+    foot = max(hdg.footprint(r) for r in reqs)
+    budget = int(1.5 * foot)
+    with hdg.Server(slots=4, hbm_budget=budget) as s:
+        ids = [s.submit(r) for r in reqs]
+        res = [s.wait(i) for i in ids]
+        st = s.stats()
+    assert all(r["status"] == 0 for r in res)
+    iv = sorted((r["t_start"], r["t_done"]) for r in res)
+    assert all(iv[i + 1][0] >= iv[i][1] for i in range(len(iv) - 1))
+    assert st["peak_reserved"] <= budget * 1.1
+    assert all(r["device_bytes"] <= foot for r in res)
+
+
+def test_server_rejects_and_isolates_failures(hdg):
+    with hdg.Server(slots=2, hbm_budget=64 << 20) as s:
+        with pytest.raises(hdg.HDGError, match="HBM_BUDGET"):
+            s.submit(_req(hdg, 512, 512, 3, 1))
+        with pytest.raises(hdg.HDGError, match="UNSUPPORTED"):
+            s.submit(hdg.make_request(nx=8, ny=8, k=5))
+        # k=3 at quad_degree 4 (the reference default k+1): singular local matrix
+        bad = s.submit(hdg.make_request(nx=8, ny=8, k=3, quad_degree=4))
+        good = s.submit(hdg.make_request(nx=16, ny=16, k=1, uex_kind=0))
+        rb, rg = s.wait(bad), s.wait(good)
+    assert rb["status"] == 4
+    assert rg["status"] == 0 and 0 < rg["errornorm"] < 1e-3
+
+
+def test_footprint_matches_device_use(hdg):
+    """The admission estimate covers what a context really allocates."""
+    for (n, k, prec) in [(64, 1, 0), (128, 2, 0), (100, 3, 1), (40, 4, 0)]:
+        r = _req(hdg, n, n, k, 3, precision=prec)
+        with hdg.Server(slots=1) as s:
+            res = s.wait(s.submit(r))
+        assert res["status"] == 0
+        assert res["device_bytes"] <= hdg.footprint(r) <= 1.5 * res["device_bytes"] + (16 << 20)
