@@ -829,7 +829,12 @@ int hdg_run(hdg_ctx* c, hdg_solve_info* info) {
   TRY(select_device(c));
   TRY(assemble_impl(c));
   int r = solve_impl(c, info);
-  if (r != HDG_OK) return r;
+  if (r != HDG_OK) {
+    // a singular local matrix or a negative Jacobian is reported as such,
+    // not as the failed solve it causes
+    TRY(check_err_flag(c));
+    return r;
+  }
   TRY(recover_impl(c));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_err_flag(c);

@@ -24,7 +24,8 @@ def make_request(nx, ny, k, quad_degree=None, tau=1.0, precision=L.HDG_FP64, rto
     """An hdg_request; quad_degree defaults to {1:2, 2:3, 3:6, 4:6}."""
     r = L.Request()
     r.nx, r.ny, r.jitter, r.seed = int(nx), int(ny), float(jitter), int(seed)
-    r.opt = L.Options(k=int(k), quad_degree=int(quad_degree or QDEG_DEFAULT[int(k)]), tau=float(tau),
+    r.opt = L.Options(k=int(k), quad_degree=int(quad_degree or QDEG_DEFAULT.get(int(k), int(k) + 1)),
+                      tau=float(tau),
                       precision=int(precision), rtol=float(rtol), maxit=int(maxit), schedule=0,
                       precond=int(precond))
     r.src.kind = int(src_kind)
