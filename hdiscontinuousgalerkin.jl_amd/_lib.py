@@ -19,6 +19,7 @@ STATUS = {
 }
 HDG_FP64, HDG_MIXED = 0, 1
 PC_AUTO, PC_BLOCK_JACOBI, PC_MULTIGRID = 0, 1, 2
+SCHEDULE_EAGER = 1
 SRC_REFERENCE, SRC_KAPPA, SRC_SINE_SERIES, SRC_VALUES = 0, 1, 2, 3
 UEX_SINSIN, UEX_LINEAR = 0, 1
 

@@ -57,9 +57,9 @@ class Context:
         self.sizes = None
 
     def setup(self, k, quad_degree=None, tau=1.0, precision=L.HDG_FP64, rtol=1e-12, maxit=0,
-              precond=L.PC_AUTO):
+              precond=L.PC_AUTO, schedule=0):
         o = L.Options(k=int(k), quad_degree=int(quad_degree or 0), tau=float(tau),
-                      precision=int(precision), rtol=float(rtol), maxit=int(maxit), schedule=0,
+                      precision=int(precision), rtol=float(rtol), maxit=int(maxit), schedule=int(schedule),
                       precond=int(precond))
         self._chk(lib().hdg_setup(self._h, C.byref(o)))
         s = L.Sizes()

@@ -66,6 +66,10 @@ struct hdg_ctx {
   float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr;
   float *f_e0 = nullptr, *f_res0 = nullptr, *f_xa = nullptr, *f_xb = nullptr;
 
+  // captured chunks of CG iterations, one per read-back slot (valid until
+  // the next hdg_setup / hdg_set_mesh)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
   bool assembled = false, solved = false, recovered = false;
 };
 
@@ -101,9 +105,12 @@ int dalloc(hdg_ctx* c, T** p, size_t count) {
   if (c->budget && c->used + bytes > c->budget)
     return fail(c, HDG_ERR_HBM_BUDGET, "device allocation of %zu bytes exceeds the budget (%zu of %zu used)",
                 bytes, c->used, c->budget);
+  // stream-ordered allocation from the device's pool: unlike hipMalloc /
+  // hipFree it does not synchronise the device, so contexts working on
+  // other streams (the serving loop's slots) keep running
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess) return fail(c, HDG_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  hipError_t e = hipMallocAsync(&q, bytes, c->stream);
+  if (e != hipSuccess) return fail(c, HDG_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
   c->allocs[q] = bytes;
   c->used += bytes;
   *p = static_cast<T*>(q);
@@ -118,7 +125,7 @@ void dfree(hdg_ctx* c, T*& p) {
     c->used -= it->second;
     c->allocs.erase(it);
   }
-  (void)hipFree((void*)p);
+  (void)hipFreeAsync((void*)p, c->stream);
   p = nullptr;
 }
 
@@ -128,7 +135,16 @@ void dfree(hdg_ctx* c, T*& p) {
     if (r_ != HDG_OK) return r_; \
   } while (0)
 
+void drop_graphs(hdg_ctx* c) {
+  for (auto& g : c->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
 void free_problem(hdg_ctx* c) {
+  drop_graphs(c);
   dfree(c, c->d_tab);
   dfree(c, c->d_fqp);
   dfree(c, c->d_dmask);
@@ -370,20 +386,49 @@ MgVecs<float> mv32(const hdg_ctx* c) { return MgVecs<float>{c->f_e0, c->f_res0, 
 // PCG iterations enqueued between convergence checks
 int chunk_of(const hdg_ctx* c) { return use_mg(c) ? 4 : 64; }
 
-int enqueue_chunk(hdg_ctx* c, int first, double rtol2, int maxit) {
+// one chunk of CG iterations; the kernels take the iteration number from the
+// device state, so the same launches serve every chunk (and a graph of them)
+int enqueue_chunk(hdg_ctx* c, double rtol2, int maxit) {
   const int n = chunk_of(c);
-  for (int i = first; i < first + n; ++i) {
-    LAUNCHCHK(c, launch_cg_spmv(c->pat, op64(c), kv64(c), i, rtol2, maxit, 0, c->stream));
+  for (int it = 0; it < n; ++it) {
+    LAUNCHCHK(c, launch_cg_spmv(c->pat, op64(c), kv64(c), 0, rtol2, maxit, 0, c->stream));
     if (mixed(c)) {
-      LAUNCHCHK(c, launch_mg_update(op32(c), kv64(c), mv32(c), i, c->stream));
-      LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), i, c->stream));
+      LAUNCHCHK(c, launch_mg_update(op32(c), kv64(c), mv32(c), c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), c->stream));
     } else if (use_mg(c)) {
-      LAUNCHCHK(c, launch_mg_update(op64(c), kv64(c), mv64(c), i, c->stream));
-      LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), i, c->stream));
+      LAUNCHCHK(c, launch_mg_update(op64(c), kv64(c), mv64(c), c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), c->stream));
     } else {
-      LAUNCHCHK(c, launch_cg_update(c->T, c->cg, i, 0, c->stream));
+      LAUNCHCHK(c, launch_cg_update(c->T, c->cg, 0, 0, c->stream));
     }
   }
+  return HDG_OK;
+}
+
+// A chunk of iterations followed by the read-back of the CG state into
+// h_st[slot], eagerly or by replaying its captured graph (built once per
+// set-up: every launch argument is fixed until the next hdg_setup).
+int launch_chunk(hdg_ctx* c, int slot, double rtol2, int maxit) {
+  const bool eager = (c->opt.schedule & HDG_SCHEDULE_EAGER) != 0;
+  if (!eager && !c->gexec[slot]) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = enqueue_chunk(c, rtol2, maxit);
+    hipError_t e = hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (r != HDG_OK) return r;
+    HIPCHK(c, e);
+    HIPCHK(c, e2);
+    e = hipGraphInstantiate(&c->gexec[slot], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(c, e);
+  }
+  if (!eager) {
+    HIPCHK(c, hipGraphLaunch(c->gexec[slot], c->stream));
+    return HDG_OK;
+  }
+  TRY(enqueue_chunk(c, rtol2, maxit));
+  HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
   return HDG_OK;
 }
 
@@ -404,14 +449,14 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     LAUNCHCHK(c, launch_mg_power(c->pat, op32(c), c->f_xa, c->f_xb, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
                                  10, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), 0, c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), c->stream));
   } else if (use_mg(c)) {
     LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
     // the CG vectors x, r serve as scratch before the solve initialises them
     LAUNCHCHK(c, launch_mg_power(c->pat, op64(c), c->cg.x, c->cg.r, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
                                  10, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), 0, c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), c->stream));
   } else {
     LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   }
@@ -420,16 +465,12 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   hipEvent_t evs[2];
   HIPCHK(c, hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   HIPCHK(c, hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
-  int slot = 0, next = 1;
-  TRY(enqueue_chunk(c, next, rtol * rtol, maxit));
-  next += chunk_of(c);
-  HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+  int slot = 0;
+  TRY(launch_chunk(c, slot, rtol * rtol, maxit));
   HIPCHK(c, hipEventRecord(evs[slot], c->stream));
   for (;;) {
     const int nxt = slot ^ 1;
-    TRY(enqueue_chunk(c, next, rtol * rtol, maxit));
-    next += chunk_of(c);
-    HIPCHK(c, hipMemcpyAsync(&c->h_st[nxt], c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    TRY(launch_chunk(c, nxt, rtol * rtol, maxit));
     HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
     HIPCHK(c, hipEventSynchronize(evs[slot]));
     if (c->h_st[slot].done) break;
@@ -488,6 +529,15 @@ int hdg_create(hdg_ctx** out, int device, size_t hbm_budget) {
   c->budget = hbm_budget;
   int r = select_device(c);
   if (r) { delete c; return r; }
+  {
+    // freed blocks stay in the device pool for the next allocation instead
+    // of going back to the driver at every synchronisation
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipHostMalloc((void**)&c->h_st, 2 * sizeof(CgState), hipHostMallocDefault) != hipSuccess) {
@@ -498,7 +548,11 @@ int hdg_create(hdg_ctx** out, int device, size_t hbm_budget) {
     delete c;
     return HDG_ERR_HIP;
   }
-  (void)hipMemset(c->cg.st, 0, sizeof(CgState));
+  if (hipMemsetAsync(c->cg.st, 0, sizeof(CgState), c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    hdg_destroy(c);
+    return HDG_ERR_HIP;
+  }
   *out = c;
   return HDG_OK;
 }
@@ -514,7 +568,10 @@ void hdg_destroy(hdg_ctx* c) {
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
   delete c;
 }
 

@@ -75,7 +75,13 @@ struct CgState {
   int it;        // completed iterations when the solve stopped
   int maxit;
   int done;      // 0 running, 1 converged, 2 iteration cap, 3 breakdown
-  int pad1;
+  // iteration number handed from kernel to kernel, so that no launch takes
+  // it as an argument and a captured chunk of iterations can be replayed:
+  // each field is written by the lead thread of one kernel and read by the
+  // next kernel of the chain, never by the kernel that writes it
+  int i_spmv;    // for the next CG product (written by the update / smoother)
+  int i_upd;     // for the update (written by the product)
+  int i_pre;     // for the smoother (written by the multigrid update)
 };
 
 struct TraceSystem {
@@ -171,11 +177,13 @@ int launch_fill(const DevPattern& P, const double* Ate, const double* bte, Trace
                 hipStream_t s);
 int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);
 int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s);
-// force = 1 runs the kernel regardless of the convergence state and leaves
-// x, r untouched (timing only; the CG state is not usable afterwards)
-int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i, double rtol2,
-                   int maxit, int force, hipStream_t s);
-int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipStream_t s);
+// i_fixed = 0 takes the iteration number from the device state; force = 1
+// (timing only, with i_fixed > 0) runs the kernel regardless of the
+// convergence state and leaves x, r untouched (the CG state is not usable
+// afterwards)
+int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i_fixed,
+                   double rtol2, int maxit, int force, hipStream_t s);
+int launch_cg_update(const TraceSystem& T, CgVectors& v, int i_fixed, int force, hipStream_t s);
 int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream_t s);
 // copies n doubles (n even) with a plain streaming kernel
 int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
@@ -199,10 +207,10 @@ template <class W>
 int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
                    const MgVecs<W>& mv, hipStream_t s);
 template <class W>
-int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, int i, hipStream_t s);
+int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, hipStream_t s);
 template <class W>
 int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
-                      const MgVecs<W>& mv, int i, hipStream_t s);
+                      const MgVecs<W>& mv, hipStream_t s);
 // float copy of n doubles
 int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
 

@@ -558,11 +558,12 @@ __device__ inline double sum_partials(const double* p, int n, double* sh) {
 // update(i-1), beta, p = z + beta p_old, q = S p, partial (p, q).
 template <int NT>
 __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceOp<double> T, Krylov<double> v,
-                                                             int i, double rtol2, int maxit, int force) {
+                                                             int i_fixed, double rtol2, int maxit, int force) {
   constexpr int NW = kCgThreads / 64;
   __shared__ double sh[NW];
   CgState* st = v.st;
   if (!force && st->done) return;
+  const int i = i_fixed > 0 ? i_fixed : st->i_spmv;
   const int nb = v.nblocks;
   const double rz = sum_partials<NW>(v.part + (size_t)nb * (1 + (i & 1)), nb, sh);
   const double rr = sum_partials<NW>(v.part + (size_t)nb * 3, nb, sh);
@@ -584,6 +585,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
       st->it = i - 1;
       st->rr = rr;
       if (i == 1) st->bb = rr;
+      st->i_upd = i;
     }
   }
   const double beta = (i == 1) ? 0.0 : rz / st->rz[(i - 1) & 1];
@@ -615,12 +617,14 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
 }
 
 template <int NT>
-__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, int i, int force) {
+__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, int i_fixed, int force) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
   CgState* st = v.st;
   if (!force && st->done) return;
+  const int i = i_fixed > 0 ? i_fixed : st->i_upd;
+  if (!force && blockIdx.x == 0 && threadIdx.x == 0) st->i_spmv = i + 1;
   const int nb = v.nblocks;
   const double pq = sum_partials<NT>(v.part, nb, sh);
   const double rz = st->rz[i & 1];
@@ -718,6 +722,7 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
     st->done = 0;
     st->it = 0;
     st->pq = 0.0;
+    st->i_spmv = 1;
   }
 }
 
@@ -906,19 +911,19 @@ int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipS
   return (int)hipGetLastError();
 }
 
-int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i, double rtol2,
-                   int maxit, int force, hipStream_t s) {
-#define SPMV(NT)                                                                                   \
-  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, i, rtol2, maxit, \
-                     force);
+int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i_fixed,
+                   double rtol2, int maxit, int force, hipStream_t s) {
+#define SPMV(NT)                                                                                          \
+  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, i_fixed, rtol2, \
+                     maxit, force);
   HDG_NT_DISPATCH(T.nt, SPMV)
 #undef SPMV
   return (int)hipGetLastError();
 }
 
-int launch_cg_update(const TraceSystem& T, CgVectors& v, int i, int force, hipStream_t s) {
+int launch_cg_update(const TraceSystem& T, CgVectors& v, int i_fixed, int force, hipStream_t s) {
 #define UPD(NT) \
-  hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, i, force);
+  hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, i_fixed, force);
   HDG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
   return (int)hipGetLastError();

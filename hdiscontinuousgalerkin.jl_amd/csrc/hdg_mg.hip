@@ -398,12 +398,14 @@ __global__ __launch_bounds__(256) void pow_final_kernel(const double* part, int 
 // e0 = omega Dinv r (first smoothing step, preconditioner storage W);
 // partial (r, r).
 template <int NT, class W>
-__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov<double> v, MgVecs<W> mv, int i) {
+__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov<double> v, MgVecs<W> mv) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
   CgState* st = v.st;
   if (st->done) return;
+  const int i = st->i_upd;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->i_pre = i;
   const int nb = v.nblocks;
   const double pq = sum_partials_mg<NT>(v.part, nb, sh);
   const double rz = st->rz[i & 1];
@@ -489,6 +491,7 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
     v.st->done = 0;
     v.st->it = 0;
     v.st->pq = 0.0;
+    v.st->i_pre = 0;  // the first smoother call completes update(0)
   }
 }
 
@@ -523,11 +526,13 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgD
 // is not read again; partial (r, z) for the next iteration.
 template <int NT, class W>
 __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylov<double> v, MgData M,
-                                                             MgVecs<W> mv, int i) {
+                                                             MgVecs<W> mv) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
   if (v.st->done) return;
+  const int i = v.st->i_pre;
+  if (blockIdx.x == 0 && threadIdx.x == 0) v.st->i_spmv = i + 1;
   const double om = v.st->omega;
   const double* __restrict__ e1 = M.lev[1].e;
   double prz = 0.0;
@@ -931,18 +936,18 @@ int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>
 }
 
 template <class W>
-int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, int i, hipStream_t s) {
-#define UPD(NT) hipLaunchKernelGGL((mg_update_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, mv, i);
+int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, hipStream_t s) {
+#define UPD(NT) hipLaunchKernelGGL((mg_update_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, mv);
   MG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
   return (int)hipGetLastError();
 }
 
 // z = V-cycle(r), starting from e0 = omega Dinv r already in mv.e0; writes
-// the (r, z) partials of iteration i+1.
+// the (r, z) partials of the next iteration.
 template <class W>
 int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
-                      const MgVecs<W>& mv, int i, hipStream_t s) {
+                      const MgVecs<W>& mv, hipStream_t s) {
   const int gr = grid1(T.N, 256);
   const int grows = gr > 4096 ? 4096 : gr;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv);
@@ -955,7 +960,7 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
   int e = vcycle(M, 1, v.st, s);
   if (e) return e;
 #define SMO(NT) \
-  hipLaunchKernelGGL((mg_smooth0_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, mv, i);
+  hipLaunchKernelGGL((mg_smooth0_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, mv);
   MG_NT_DISPATCH(T.nt, SMO)
 #undef SMO
   return (int)hipGetLastError();
@@ -966,9 +971,9 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
                                   const MgData&, int, hipStream_t);                                             \
   template int launch_mg_init<W>(const DevPattern&, const TraceSystem&, const TraceOp<W>&, const Krylov<double>&, \
                                  const MgVecs<W>&, hipStream_t);                                                \
-  template int launch_mg_update<W>(const TraceOp<W>&, const Krylov<double>&, const MgVecs<W>&, int, hipStream_t); \
+  template int launch_mg_update<W>(const TraceOp<W>&, const Krylov<double>&, const MgVecs<W>&, hipStream_t);   \
   template int launch_mg_precond<W>(const DevPattern&, const TraceOp<W>&, const Krylov<double>&, const MgData&,  \
-                                    const MgVecs<W>&, int, hipStream_t);
+                                    const MgVecs<W>&, hipStream_t);
 MG_INSTANTIATE(double)
 MG_INSTANTIATE(float)
 #undef MG_INSTANTIATE

@@ -63,6 +63,11 @@ enum { HDG_FP64 = 0, HDG_MIXED = 1 };
    mesh is a structured rectangle_mesh and block-Jacobi otherwise. */
 enum { HDG_PC_AUTO = 0, HDG_PC_BLOCK_JACOBI = 1, HDG_PC_MULTIGRID = 2 };
 
+/* Schedule flags.  By default each chunk of CG iterations is captured once
+   per hdg_setup as a HIP graph and replayed; HDG_SCHEDULE_EAGER launches
+   every kernel from the host instead (same results, bit for bit). */
+enum { HDG_SCHEDULE_EAGER = 1 };
+
 /* Sources.  HDG_SRC_REFERENCE is the reference's f = 2 pi^2 sin(pi x) sin(pi y)
    (examples/poisson2D_HDG.jl:55).  HDG_SRC_VALUES takes f already evaluated
    at the cell quadrature points (for arbitrary Julia closures). */
@@ -83,7 +88,7 @@ typedef struct {
   int32_t precision;   /* HDG_FP64 or HDG_MIXED */
   double rtol;         /* trace solve: stop at ||r|| <= rtol ||b|| (0 -> 1e-12) */
   int32_t maxit;       /* trace solve iteration cap (0 -> 200000) */
-  int32_t schedule;    /* kernel schedule selector, 0 = default */
+  int32_t schedule;    /* HDG_SCHEDULE_* flags, 0 = default */
   int32_t precond;     /* HDG_PC_*: block-Jacobi per face, or a V-cycle of
                           block-Jacobi on the traces + geometric multigrid on
                           the P1 vertex space of a structured mesh */

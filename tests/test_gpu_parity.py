@@ -437,3 +437,35 @@ def test_mixed_precision_needs_multigrid_path(hdg, ctx):
     ctx.set_mesh(m)
     with pytest.raises(hdg.HDGError, match="UNSUPPORTED"):
         ctx.setup(1, precision=hdg.HDG_MIXED, precond=hdg.PC_BLOCK_JACOBI)
+
+
+# ------------------------------------------------------- HIP graph replay
+@pytest.mark.parametrize("k,n,precision,precond", [(2, 64, 0, 2), (3, 33, 1, 0), (1, 24, 0, 1)])
+def test_graph_replay_equals_eager(hdg, ctx, k, n, precision, precond):
+    """The captured chunks of CG iterations, replayed over several solves with
+    other eager work and synchronisations in between, give the eager
+    launches' result bit for bit every time."""
+    m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=21)
+    other = hdg.Context(0)
+    om = hdg.rectangle_mesh((17, 19), jitter=0.1, seed=2)
+    other.set_mesh(om)
+    other.setup(2, quad_degree=3, schedule=hdg.SCHEDULE_EAGER)
+    other.set_source()
+    other.set_dirichlet(om.boundary_dofs(3), np.zeros(len(om.boundary_dofs(3))))
+    out = {}
+    for sched in (hdg.SCHEDULE_EAGER, 0):
+        ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=QD[k], rtol=1e-12, precision=precision, precond=precond, schedule=sched)
+        ctx.set_source()
+        ctx.set_dirichlet(m.boundary_dofs(k + 1), np.zeros(len(m.boundary_dofs(k + 1))))
+        sols = []
+        for rep in range(3):
+            info = ctx.run()
+            sols.append((ctx.trace(), info.iters))
+            other.run()      # eager work of another context between replays
+        out[sched] = sols
+    eager_u, eager_it = out[hdg.SCHEDULE_EAGER][0]
+    for u, it in out[hdg.SCHEDULE_EAGER] + out[0]:
+        assert it == eager_it
+        assert np.array_equal(u, eager_u)
+    other.close()
