@@ -196,6 +196,7 @@ void free_mesh(hdg_ctx* c) {
   dfree(c, c->mg.vf_idx);
   dfree(c, c->mg.fq4);
   dfree(c, c->mg.tailpack);
+  dfree(c, c->mg.cinv);
   for (int l = 0; l < kMaxLevels; ++l) {
     MgLevel& L = c->mg.lev[l];
     dfree(c, L.a);
@@ -315,6 +316,7 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     if (acc > kTailNodes) break;
     M.tail = t;
   }
+  TRY(dalloc(c, &M.cinv, (size_t)kCoarseDirect * kCoarseDirect));
   if (M.tail < M.nlev) {
     int64_t tn = 0;
     for (int t = M.tail; t < M.nlev; ++t) tn += M.lev[t].n;

@@ -145,6 +145,8 @@ constexpr int kStencil = 9;
 // diagonal inverses and three vectors live in LDS (13 doubles per node,
 // 152 KB of the 160 KB of a CU)
 constexpr int kTailNodes = 1500;
+// coarsest grids up to this many nodes are solved exactly (inverse at set-up)
+constexpr int kCoarseDirect = 16;
 struct MgData {
   int nlev = 0;                  // levels 1..nlev-1 are vertex grids
   int nx = 0, ny = 0;
@@ -162,6 +164,7 @@ struct MgData {
   double* sp = nullptr;          // [4*N] S P0 restricted to the rows: sp[j*N + r] couples
                                  // row r with vertex fq4[4*face(r) + j]
   int tail = kMaxLevels;         // levels tail..nlev-1 run in one workgroup (LDS)
+  double* cinv = nullptr;        // [kCoarseDirect^2] inverse of the coarsest operator (row-major)
   double* tailpack = nullptr;    // stencils (7 n) and dinv (n) of levels tail.. back to back,
                                  // the LDS image the tail kernel loads
   double omega_v = 0.8;          // vertex-level Jacobi weight

@@ -230,24 +230,24 @@ __global__ void mg_assemble_l1_kernel(DevMesh mesh, MgData M) {
 }
 
 // Galerkin coarse stencil R A P on the next level: R = P^T with the weights
-// of prolong_at.
-__global__ void mg_rap_kernel(MgLevel F, MgLevel Cc) {
-  const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (C >= Cc.n) return;
+// of prolong_at.  Eight lanes per coarse node: lane t < 7 takes the fine
+// node at support offset t, and a fixed shuffle tree sums the lanes' stencils.
+__global__ __launch_bounds__(256) void mg_rap_kernel(MgLevel F, MgLevel Cc) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t C = gid >> 3;
+  const int t = (int)(gid & 7);
   const int cnx = Cc.nx, fnx = F.nx, fny = F.ny;
-  const int I = (int)(C % (cnx + 1)), J = (int)(C / (cnx + 1));
-  const unsigned char cm = Cc.mask[C];  // written by mg_cmask_kernel
+  const bool live = C < Cc.n;
+  const int I = live ? (int)(C % (cnx + 1)) : 0, J = live ? (int)(C / (cnx + 1)) : 0;
+  const unsigned char cm = live ? Cc.mask[C] : 1;  // written by mg_cmask_kernel
   double st[kStencil] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (!cm) {
-    const int X = fine_of(I, fnx), Y = fine_of(J, fny);
-    const int sx[7] = {0, 1, -1, 0, 0, 1, -1}, sy[7] = {0, 0, 0, 1, -1, -1, 1};
-    for (int t = 0; t < 7; ++t) {  // fine support of C
-      const int fx = X + sx[t], fy = Y + sy[t];
-      if (fx < 0 || fx > fnx || fy < 0 || fy > fny) continue;
-      const int64_t fi = (int64_t)fy * (fnx + 1) + fx;
-      if (F.mask[fi]) continue;
-      const double w1 = pweight(fnx, fny, fx, fy, I, J);
-      if (w1 == 0.0) continue;
+  if (!cm && t < 7) {
+    const int fx = fine_of(I, fnx) + (t == 1 || t == 5 ? 1 : (t == 2 || t == 6 ? -1 : 0));
+    const int fy = fine_of(J, fny) + (t == 3 || t == 6 ? 1 : (t == 4 || t == 5 ? -1 : 0));
+    const int64_t fi = (int64_t)fy * (fnx + 1) + fx;
+    const bool in = fx >= 0 && fx <= fnx && fy >= 0 && fy <= fny && !F.mask[fi];
+    const double w1 = in ? pweight(fnx, fny, fx, fy, I, J) : 0.0;
+    if (w1 != 0.0) {
       for (int s = 0; s < F.npts; ++s) {
         const double av = F.a[s * F.n + fi];
         if (av == 0.0) continue;
@@ -265,13 +265,75 @@ __global__ void mg_rap_kernel(MgLevel F, MgLevel Cc) {
         for (int q = 0; q < np; ++q) {
           if (Cc.mask[(int64_t)qy[q] * (cnx + 1) + qx[q]]) continue;  // zero column of P
           const int sl = slot_of(qx[q] - I, qy[q] - J);
-          if (sl >= 0) st[sl] = fma(w1 * av, w2, st[sl]);
+#pragma unroll
+          for (int u = 0; u < kStencil; ++u)
+            if (u == sl) st[u] = fma(w1 * av, w2, st[u]);
         }
       }
     }
   }
-  for (int s = 0; s < kStencil; ++s) Cc.a[s * Cc.n + C] = cm ? (s == 0 ? 1.0 : 0.0) : st[s];
+#pragma unroll
+  for (int u = 0; u < kStencil; ++u) {
+    st[u] += __shfl_xor(st[u], 1, 8);
+    st[u] += __shfl_xor(st[u], 2, 8);
+    st[u] += __shfl_xor(st[u], 4, 8);
+  }
+  if (t != 0 || !live) return;
+#pragma unroll
+  for (int u = 0; u < kStencil; ++u) Cc.a[u * Cc.n + C] = cm ? (u == 0 ? 1.0 : 0.0) : st[u];
   Cc.dinv[C] = cm ? 0.0 : 1.0 / st[0];
+}
+
+// Inverse of the coarsest operator (at most kCoarseDirect nodes), by
+// Gauss-Jordan with partial pivoting on [A | I] in LDS: one thread per entry,
+// the pivot row chosen by thread 0.  Fixed nodes have identity rows.
+__global__ __launch_bounds__(1024) void mg_coarse_inverse_kernel(MgLevel L, double* cinv) {
+  constexpr int W = 2 * kCoarseDirect;
+  __shared__ double m[kCoarseDirect * W];
+  __shared__ int piv;
+  const int n = (int)L.n;
+  for (int e = threadIdx.x; e < n * W; e += blockDim.x) m[e] = 0.0;
+  __syncthreads();
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    const int i = v % (L.nx + 1), j = v / (L.nx + 1);
+    m[v * W + n + v] = 1.0;
+    for (int s = 0; s < L.npts; ++s) {
+      const int gx = i + kOffX[s], gy = j + kOffY[s];
+      if (gx < 0 || gx > L.nx || gy < 0 || gy > L.ny) continue;
+      m[v * W + gy * (L.nx + 1) + gx] += L.a[s * L.n + v];
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < n; ++c) {
+    if (threadIdx.x == 0) {
+      int best = c;
+      for (int r = c + 1; r < n; ++r)
+        if (fabs(m[r * W + c]) > fabs(m[best * W + c])) best = r;
+      piv = best;
+    }
+    __syncthreads();
+    const int pr = piv;
+    if (pr != c)
+      for (int e = threadIdx.x; e < 2 * n; e += blockDim.x) {
+        const double a = m[c * W + e];
+        m[c * W + e] = m[pr * W + e];
+        m[pr * W + e] = a;
+      }
+    __syncthreads();
+    const double pv = m[c * W + c];
+    // each thread owns one (row, column) entry of the update
+    double upd = 0.0;
+    const int row = threadIdx.x / (2 * n), col = threadIdx.x % (2 * n);
+    const bool own = threadIdx.x < n * 2 * n;
+    if (own) {
+      const double pc = m[c * W + col] / pv;
+      upd = row == c ? pc : m[row * W + col] - m[row * W + c] * pc;
+    }
+    __syncthreads();
+    if (own) m[row * W + col] = upd;
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) cinv[e] = m[(e / n) * W + n + e % n];
 }
 
 // LDS image of the tail levels: per level its stencils (kStencil n) then dinv (n).
@@ -803,14 +865,24 @@ __global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const Cg
     const TailLevel L = lvl(M.nlev - 1, o);
     double *e = E + o, *t = Tt + o;
     const double* r = R + o;
-    for (int v = threadIdx.x; v < L.n; v += blockDim.x) t[v] = om * L.dinv[v] * r[v];
-    __syncthreads();
-    for (int it = 1; it < M.coarse_sweeps; ++it) {
-      tail_sweep(L, t, e, r, om);
+    if (L.n <= kCoarseDirect) {
+      // exact solve with the inverse computed at set-up
+      for (int v = threadIdx.x; v < L.n; v += blockDim.x) {
+        double acc = 0.0;
+        for (int u = 0; u < L.n; ++u) acc = fma(M.cinv[v * L.n + u], r[u], acc);
+        t[v] = acc;
+      }
       __syncthreads();
-      double* sw = e;
-      e = t;
-      t = sw;
+    } else {
+      for (int v = threadIdx.x; v < L.n; v += blockDim.x) t[v] = om * L.dinv[v] * r[v];
+      __syncthreads();
+      for (int it = 1; it < M.coarse_sweeps; ++it) {
+        tail_sweep(L, t, e, r, om);
+        __syncthreads();
+        double* sw = e;
+        e = t;
+        t = sw;
+      }
     }
     ec = t;
   }
@@ -898,8 +970,10 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPa
 #undef SPK
   for (int l = 1; l + 1 < M.nlev; ++l) {
     hipLaunchKernelGGL(mg_cmask_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
-    hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
+    hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(8 * M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
   }
+  if (M.lev[M.nlev - 1].n <= kCoarseDirect)
+    hipLaunchKernelGGL(mg_coarse_inverse_kernel, dim3(1), dim3(1024), 0, s, M.lev[M.nlev - 1], M.cinv);
   if (M.tail < M.nlev) hipLaunchKernelGGL(mg_tailpack_kernel, dim3(16), dim3(256), 0, s, M);
   return (int)hipGetLastError();
 }
