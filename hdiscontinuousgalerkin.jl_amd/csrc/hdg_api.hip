@@ -434,6 +434,8 @@ int launch_chunk(hdg_ctx* c, int slot, double rtol2, int maxit) {
   return HDG_OK;
 }
 
+int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit);
+
 int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
   if (c->jacobi_dirty) {
@@ -481,6 +483,11 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   HIPCHK(c, hipEventSynchronize(evs[slot ^ 1]));
   (void)hipEventDestroy(evs[0]);
   (void)hipEventDestroy(evs[1]);
+  return finish_solve(c, info, maxit);
+}
+
+// uhat = x (+ Dirichlet values), the FP64 residual, the solve record
+int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit) {
   LAUNCHCHK(c, launch_cg_finish(c->T, c->cg, c->d_uhat, c->stream));
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   LAUNCHCHK(c, launch_true_residual(c->pat, c->T, c->d_uhat, c->d_scal + 1, c->cg, c->stream));

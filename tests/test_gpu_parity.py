@@ -453,7 +453,8 @@ def test_graph_replay_equals_eager(hdg, ctx, k, n, precision, precond):
     other.set_source()
     other.set_dirichlet(om.boundary_dofs(3), np.zeros(len(om.boundary_dofs(3))))
     out = {}
-    for sched in (hdg.SCHEDULE_EAGER, 0):
+    eager, graph = hdg.SCHEDULE_EAGER, 0
+    for sched in (eager, graph):
         ctx.set_mesh(m)
         ctx.setup(k, quad_degree=QD[k], rtol=1e-12, precision=precision, precond=precond, schedule=sched)
         ctx.set_source()
@@ -464,8 +465,8 @@ def test_graph_replay_equals_eager(hdg, ctx, k, n, precision, precond):
             sols.append((ctx.trace(), info.iters))
             other.run()      # eager work of another context between replays
         out[sched] = sols
-    eager_u, eager_it = out[hdg.SCHEDULE_EAGER][0]
-    for u, it in out[hdg.SCHEDULE_EAGER] + out[0]:
+    eager_u, eager_it = out[eager][0]
+    for u, it in out[eager] + out[graph]:
         assert it == eager_it
         assert np.array_equal(u, eager_u)
     other.close()
