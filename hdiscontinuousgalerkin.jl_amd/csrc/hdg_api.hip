@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/hdg.h"
+#include "hdg_host.h"
 #include "hdg_internal.h"
 #include "hdg_tables.h"
 
@@ -32,6 +33,7 @@ struct hdg_ctx {
   DevPattern pat;
   bool structured = false;  // cells follow rectangle_mesh's (nx, ny) numbering
   MgData mg;                // vertex hierarchy (structured meshes)
+  MgData* d_mg = nullptr;   // its device copy (read by the one-workgroup tail kernel)
 
   // discretisation
   bool has_setup = false;
@@ -170,6 +172,7 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->mg.e0);
   dfree(c, c->mg.res0);
   dfree(c, c->mg.sp);
+  dfree(c, c->d_mg);
   dfree(c, c->f_vals);
   dfree(c, c->f_Dinv);
   dfree(c, c->f_sp);
@@ -233,40 +236,30 @@ bool detect_structured(int64_t nn, int64_t nc, const int64_t* cells, int* nx_out
   return true;
 }
 
-// face end points and the vertex -> face lists (any mesh), and the vertex
-// grid hierarchy when the mesh is structured
-int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* faces, const int64_t* cells,
-                            int64_t nc) {
-  std::vector<int> fv(2 * nf), cnt(nn + 1, 0);
+}  // namespace
+
+namespace hdg {
+
+bool face_vertex_maps(int64_t nn, int64_t nf, const int64_t* faces, const int64_t* cells, std::vector<int>& fv,
+                      std::vector<int>& vf_ptr, std::vector<int>& vf_idx, std::vector<int>* fq4) {
+  fv.assign(2 * nf, 0);
+  vf_ptr.assign(nn + 1, 0);
   for (int64_t f = 0; f < nf; ++f) {
     const int a = (int)faces[f] - 1, b = (int)faces[f + nf] - 1;
     fv[2 * f] = std::min(a, b);
     fv[2 * f + 1] = std::max(a, b);
-    cnt[a + 1]++;
-    cnt[b + 1]++;
+    vf_ptr[a + 1]++;
+    vf_ptr[b + 1]++;
   }
-  for (int64_t v = 0; v < nn; ++v) cnt[v + 1] += cnt[v];
-  std::vector<int> idx(cnt[nn]), pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t v = 0; v < nn; ++v) vf_ptr[v + 1] += vf_ptr[v];
+  vf_idx.assign(vf_ptr[nn], 0);
+  std::vector<int> pos(vf_ptr.begin(), vf_ptr.end() - 1);
   for (int64_t f = 0; f < nf; ++f) {
-    idx[pos[fv[2 * f]]++] = (int)f;
-    idx[pos[fv[2 * f + 1]]++] = (int)f;
+    vf_idx[pos[fv[2 * f]]++] = (int)f;
+    vf_idx[pos[fv[2 * f + 1]]++] = (int)f;
   }
-  MgData& M = c->mg;
-  TRY(dalloc(c, &M.fv, 2 * nf));
-  TRY(dalloc(c, &M.vf_ptr, nn + 1));
-  TRY(dalloc(c, &M.vf_idx, idx.size()));
-  HIPCHK(c, hipMemcpyAsync(M.fv, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(M.vf_ptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(M.vf_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  M.nv = nn;
-  int nx = 0, ny = 0;
-  c->structured = detect_structured(nn, nc, cells, &nx, &ny);
-  if (!c->structured) return HDG_OK;
-  M.nx = nx;
-  M.ny = ny;
-  // the four vertices around every face (stencil of S P0)
-  std::vector<int> fq4(4 * nf);
+  if (!fq4) return true;
+  fq4->assign(4 * nf, 0);
   for (int64_t f = 0; f < nf; ++f) {
     const int a = fv[2 * f], b = fv[2 * f + 1];
     int opp[2];
@@ -279,15 +272,18 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
         if (nw != a && nw != b) opp[side] = nw;
       }
     }
-    if (opp[0] < 0) return fail(c, HDG_ERR_ARG, "face %lld has no first cell", (long long)f + 1);
-    fq4[4 * f] = a;
-    fq4[4 * f + 1] = b;
-    fq4[4 * f + 2] = opp[0];
-    fq4[4 * f + 3] = opp[1] < 0 ? opp[0] : opp[1];
+    if (opp[0] < 0) return false;
+    (*fq4)[4 * f] = a;
+    (*fq4)[4 * f + 1] = b;
+    (*fq4)[4 * f + 2] = opp[0];
+    (*fq4)[4 * f + 3] = opp[1] < 0 ? opp[0] : opp[1];
   }
-  TRY(dalloc(c, &M.fq4, 4 * nf));
-  HIPCHK(c, hipMemcpyAsync(M.fq4, fq4.data(), sizeof(int) * fq4.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return true;
+}
+
+void mg_plan_levels(int nx, int ny, MgData& M) {
+  M.nx = nx;
+  M.ny = ny;
   int l = 1, lx = nx, ly = ny;
   for (;;) {
     MgLevel& L = M.lev[l];
@@ -296,12 +292,6 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     L.n = (int64_t)(lx + 1) * (ly + 1);
     // RAP across a one-cell-wide coarse column or row adds the (+1,+1) pair
     L.npts = l == 1 ? 7 : ((M.lev[l - 1].npts > 7 || M.lev[l - 1].nx % 2 || M.lev[l - 1].ny % 2) ? kStencil : 7);
-    TRY(dalloc(c, &L.a, kStencil * L.n));
-    TRY(dalloc(c, &L.dinv, L.n));
-    TRY(dalloc(c, &L.mask, L.n));
-    TRY(dalloc(c, &L.r, L.n));
-    TRY(dalloc(c, &L.e, L.n));
-    TRY(dalloc(c, &L.t, L.n));
     ++l;
     if (std::max(lx, ly) <= 2 || l >= kMaxLevels) break;
     lx = (lx + 1) / 2;  // coarsening with a ragged end on odd sides
@@ -315,6 +305,44 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
     acc += M.lev[t].n;
     if (acc > kTailNodes) break;
     M.tail = t;
+  }
+}
+
+}  // namespace hdg
+
+namespace {
+
+// face end points and the vertex -> face lists (any mesh), and the vertex
+// grid hierarchy when the mesh is structured
+int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* faces, const int64_t* cells,
+                            int64_t nc) {
+  std::vector<int> fv, cnt, idx, fq4;
+  int nx = 0, ny = 0;
+  c->structured = detect_structured(nn, nc, cells, &nx, &ny);
+  if (!face_vertex_maps(nn, nf, faces, cells, fv, cnt, idx, c->structured ? &fq4 : nullptr))
+    return fail(c, HDG_ERR_ARG, "a face has no first cell");
+  MgData& M = c->mg;
+  TRY(dalloc(c, &M.fv, 2 * nf));
+  TRY(dalloc(c, &M.vf_ptr, nn + 1));
+  TRY(dalloc(c, &M.vf_idx, idx.size()));
+  HIPCHK(c, hipMemcpyAsync(M.fv, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M.vf_ptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M.vf_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  M.nv = nn;
+  if (!c->structured) return HDG_OK;
+  TRY(dalloc(c, &M.fq4, 4 * nf));
+  HIPCHK(c, hipMemcpyAsync(M.fq4, fq4.data(), sizeof(int) * fq4.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  mg_plan_levels(nx, ny, M);
+  for (int l = 1; l < M.nlev; ++l) {
+    MgLevel& L = M.lev[l];
+    TRY(dalloc(c, &L.a, kStencil * L.n));
+    TRY(dalloc(c, &L.dinv, L.n));
+    TRY(dalloc(c, &L.mask, L.n));
+    TRY(dalloc(c, &L.r, L.n));
+    TRY(dalloc(c, &L.e, L.n));
+    TRY(dalloc(c, &L.t, L.n));
   }
   TRY(dalloc(c, &M.cinv, (size_t)kCoarseDirect * kCoarseDirect));
   if (M.tail < M.nlev) {
@@ -396,10 +424,10 @@ int enqueue_chunk(hdg_ctx* c, double rtol2, int maxit) {
     LAUNCHCHK(c, launch_cg_spmv(c->pat, op64(c), kv64(c), 0, rtol2, maxit, 0, c->stream));
     if (mixed(c)) {
       LAUNCHCHK(c, launch_mg_update(op32(c), kv64(c), mv32(c), c->stream));
-      LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
     } else if (use_mg(c)) {
       LAUNCHCHK(c, launch_mg_update(op64(c), kv64(c), mv64(c), c->stream));
-      LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
     } else {
       LAUNCHCHK(c, launch_cg_update(c->T, c->cg, 0, 0, c->stream));
     }
@@ -453,14 +481,14 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     LAUNCHCHK(c, launch_mg_power(c->pat, op32(c), c->f_xa, c->f_xb, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
                                  10, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, mv32(c), c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
   } else if (use_mg(c)) {
     LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
     // the CG vectors x, r serve as scratch before the solve initialises them
     LAUNCHCHK(c, launch_mg_power(c->pat, op64(c), c->cg.x, c->cg.r, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
                                  10, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, mv64(c), c->stream));
+    LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
   } else {
     LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   }
@@ -726,6 +754,8 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     TRY(dalloc(c, &c->mg.e0, N));
     TRY(dalloc(c, &c->mg.res0, N));
     TRY(dalloc(c, &c->mg.sp, 4 * N));
+    TRY(dalloc(c, &c->d_mg, 1));
+    HIPCHK(c, hipMemcpyAsync(c->d_mg, &c->mg, sizeof(MgData), hipMemcpyHostToDevice, c->stream));
   }
   if (c->structured && o.precision == HDG_MIXED) {
     TRY(dalloc(c, &c->f_vals, (size_t)kSlots * c->nt * N));

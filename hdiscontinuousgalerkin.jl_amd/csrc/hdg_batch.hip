@@ -1,0 +1,826 @@
+// Batches of independent Poisson solves in grouped launches (include/hdg.h,
+// hdg_batch_*).  Requests with the same order and options form a group whose
+// rectangle meshes are packed into one mesh: node, cell, face and trace-row
+// ranges of instance j follow those of instance j-1.  Element build,
+// condensation, assembly, recovery and the elementwise trace kernels run
+// over the packed mesh once per group; the PCG kernels, the block-Jacobi
+// smoother and the multigrid vertex levels take their instance from launch
+// tables (Seg / VRec), so each instance keeps its own scalars, reductions
+// and hierarchy and gets the bits it would get alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hdg.h"
+#include "hdg_host.h"
+#include "hdg_internal.h"
+#include "hdg_tables.h"
+
+using namespace hdg;
+
+namespace {
+
+struct Inst {
+  int req = 0;                        // request index
+  int64_t nx = 0, ny = 0, nn = 0, nc = 0, nf = 0, N = 0;
+  int64_t node0 = 0, cell0 = 0, face0 = 0, row0 = 0;
+  MgData plan;                        // level sizes and tail (host plan)
+  int64_t lev_off[kMaxLevels] = {0};  // offset of its nodes in the group's level-l buffer
+  int64_t tail_off = 0;               // offset of its LDS image in the group's tailpack
+};
+
+struct LevelBuf {
+  int64_t n = 0;  // nodes of every instance at this level
+  double *a = nullptr, *dinv = nullptr, *r = nullptr, *e = nullptr, *t = nullptr;
+  unsigned char* mask = nullptr;
+};
+
+struct Group {
+  hdg_options opt{};
+  int k = 0, nt = 0, ntr = 0, m = 0;
+  bool mg = false, mixed = false, eager = false;
+  std::vector<Inst> in;
+  int64_t nn = 0, nc = 0, nf = 0, N = 0;
+  hipStream_t s = nullptr;
+  std::vector<void*> owned;
+  size_t bytes = 0;
+  // packed mesh and trace system
+  DevMesh mesh;
+  DevPattern pat;
+  TraceSystem T;
+  CgVectors cg{};
+  RefTables tab;
+  TabLayoutDev lay{};
+  double *d_tab = nullptr, *Ate = nullptr, *bte = nullptr, *Ke = nullptr, *be = nullptr, *qu = nullptr,
+         *uhat = nullptr;
+  unsigned char* dmask = nullptr;
+  double* uD = nullptr;
+  int* err = nullptr;
+  SrcParams* srcs = nullptr;
+  int* cell_inst = nullptr;
+  // launch tables
+  CgState* st = nullptr;
+  Seg *segs = nullptr, *csegs = nullptr;
+  int *binst = nullptr, *cbinst = nullptr;
+  int nbt = 0, nbc = 0;
+  double* part2 = nullptr;  // error-norm partials
+  double* res = nullptr;    // [ninst] ||K uhat - rhs||^2, [ninst] squared L2 error
+  // multigrid
+  MgData Mu;                // the packed mesh's view (level 1 spans every vertex)
+  std::vector<MgData> Mi;   // per-instance hierarchies (views into the level buffers)
+  MgData* dMi = nullptr;
+  LevelBuf lev[kMaxLevels];
+  MgGroup G;
+  float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr,
+        *f_xa = nullptr, *f_xb = nullptr;
+  // read-back
+  CgState* h_st = nullptr;  // pinned, 2 x ninst (chunk slots) + ninst (final)
+  double* h_res = nullptr;  // pinned, 2 x ninst, then the error flag (an int)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int slot = 0;
+  bool active = false;
+  double t_done = 0.0;
+};
+
+}  // namespace
+
+struct hdg_batch {
+  int device = 0;
+  size_t budget = 0, used = 0;
+  std::vector<hdg_request> reqs;
+  std::vector<Group> groups;
+  std::vector<std::pair<int, int>> where;  // request -> (group, instance)
+  bool ran = false;
+  std::string err;
+};
+
+namespace {
+
+int fail(hdg_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+#define BHIP(b, call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return fail(b, HDG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define BLAUNCH(b, call)                                                                     \
+  do {                                                                                       \
+    int e_ = (call);                                                                         \
+    if (e_ != 0)                                                                             \
+      return fail(b, HDG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString((hipError_t)e_)); \
+  } while (0)
+
+#define BTRY(x)                  \
+  do {                           \
+    int r_ = (x);                \
+    if (r_ != HDG_OK) return r_; \
+  } while (0)
+
+template <class T>
+int galloc(hdg_batch* b, Group& g, T** p, size_t count) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (b->budget && b->used + bytes > b->budget)
+    return fail(b, HDG_ERR_HBM_BUDGET, "batch needs more device memory than the budget (%zu of %zu bytes used)",
+                b->used, b->budget);
+  void* q = nullptr;
+  hipError_t e = hipMallocAsync(&q, bytes, g.s);
+  if (e != hipSuccess) return fail(b, HDG_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+  g.owned.push_back(q);
+  g.bytes += bytes;
+  b->used += bytes;
+  *p = static_cast<T*>(q);
+  return HDG_OK;
+}
+
+// copy of a host array that may go out of scope on return (set-up only)
+template <class T>
+int upload(hdg_batch* b, Group& g, T** p, const std::vector<T>& h) {
+  BTRY(galloc(b, g, p, h.size()));
+  if (!h.empty()) {
+    BHIP(b, hipMemcpyAsync(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, g.s));
+    BHIP(b, hipStreamSynchronize(g.s));
+  }
+  return HDG_OK;
+}
+
+void free_group(hdg_batch* b, Group& g) {
+  if (!g.s) return;
+  for (auto& x : g.gexec)
+    if (x) (void)hipGraphExecDestroy(x);
+  for (void* p : g.owned) (void)hipFreeAsync(p, g.s);
+  (void)hipStreamSynchronize(g.s);
+  b->used -= std::min(b->used, g.bytes);
+  g.owned.clear();
+  for (auto& e : g.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (g.h_st) (void)hipHostFree(g.h_st);
+  if (g.h_res) (void)hipHostFree(g.h_res);
+  (void)hipStreamDestroy(g.s);
+  g.s = nullptr;
+}
+
+bool same_group(const hdg_options& a, const hdg_options& o) {
+  return a.k == o.k && a.quad_degree == o.quad_degree && a.tau == o.tau && a.precision == o.precision &&
+         a.rtol == o.rtol && a.maxit == o.maxit && a.schedule == o.schedule && a.precond == o.precond;
+}
+
+SrcParams src_params(const hdg_source& s) {
+  SrcParams p;
+  p.kind = s.kind;
+  p.kappa_on = s.kappa_on ? 1 : 0;
+  p.ka = s.ka;
+  p.kb = s.kb;
+  for (int i = 0; i < 16; ++i) p.coef[i] = s.coef[i];
+  return p;
+}
+
+// Host arrays of a group's packed mesh, filled instance by instance from
+// several threads.
+struct PackedHost {
+  std::vector<double> xy;
+  std::vector<int> cn, cf, fcell, fv, vf_ptr, vf_idx, fq4, cell_inst;
+  std::vector<unsigned char> dmask;
+};
+
+// Generates instance j's rectangle mesh and writes it, shifted by the
+// instance's offsets, into the packed arrays.
+bool pack_instance(const hdg_request& q, const Group& g, int j, PackedHost& H) {
+  const Inst& I = g.in[j];
+  const int64_t nn = I.nn, nc = I.nc, nf = I.nf;
+  std::vector<double> xy(2 * nn);
+  std::vector<int64_t> cells(6 * nc), faces(4 * nf);
+  std::vector<int8_t> side(nf);
+  if (hdg_rectangle_mesh(I.nx, I.ny, 0.0, 0.0, 1.0, 1.0, q.jitter, q.seed, xy.data(), cells.data(), faces.data(),
+                         side.data()) != HDG_OK)
+    return false;
+  std::vector<int> fv, ptr, idx, fq4;
+  if (!face_vertex_maps(nn, nf, faces.data(), cells.data(), fv, ptr, idx, g.mg ? &fq4 : nullptr)) return false;
+  std::copy(xy.begin(), xy.end(), H.xy.begin() + 2 * I.node0);
+  for (int64_t e = 0; e < nc; ++e) {
+    for (int l = 0; l < 3; ++l) {
+      H.cn[3 * (I.cell0 + e) + l] = (int)(I.node0 + cells[6 * e + l] - 1);
+      H.cf[3 * (I.cell0 + e) + l] = (int)(I.face0 + cells[6 * e + 3 + l] - 1);
+    }
+    H.cell_inst[I.cell0 + e] = j;
+  }
+  for (int64_t f = 0; f < nf; ++f) {
+    const int64_t c1 = faces[f + 2 * nf], c2 = faces[f + 3 * nf];
+    H.fcell[2 * (I.face0 + f)] = (int)(I.cell0 + c1 - 1);
+    H.fcell[2 * (I.face0 + f) + 1] = c2 > 0 ? (int)(I.cell0 + c2 - 1) : -1;
+    H.fv[2 * (I.face0 + f)] = (int)(I.node0 + fv[2 * f]);
+    H.fv[2 * (I.face0 + f) + 1] = (int)(I.node0 + fv[2 * f + 1]);
+    if (g.mg)
+      for (int u = 0; u < 4; ++u) H.fq4[4 * (I.face0 + f) + u] = (int)(I.node0 + fq4[4 * f + u]);
+    for (int t = 0; t < g.nt; ++t) H.dmask[I.row0 + f * g.nt + t] = side[f] ? 1 : 0;
+  }
+  // vertex -> face lists: every face has two end points, so instance j's
+  // entries start at 2 * face0
+  for (int64_t v = 0; v < nn; ++v) H.vf_ptr[I.node0 + v] = (int)(2 * I.face0 + ptr[v]);
+  if (j + 1 == (int)g.in.size()) H.vf_ptr[I.node0 + nn] = (int)(2 * I.face0 + ptr[nn]);
+  for (size_t u = 0; u < idx.size(); ++u) H.vf_idx[2 * I.face0 + u] = (int)(I.face0 + idx[u]);
+  return true;
+}
+
+// Lays out one group: offsets, levels, host packing, device arrays, tables.
+int build_group(hdg_batch* b, Group& g) {
+  const int n = (int)g.in.size();
+  RefTables tab;
+  if (!tab.build(g.opt.k, g.opt.quad_degree))
+    return fail(b, HDG_ERR_QUAD, "Quadrature rule of order %d not available", g.opt.quad_degree);
+  g.tab = tab;
+  const TableLayout& L = tab.lay;
+  g.lay = TabLayoutDev{L.nu, L.nt, L.nqp, L.nfq, L.Mref, L.Dref, L.Cref, L.G, L.Href, L.Nw, L.N, L.Mg, L.W, L.total};
+  g.k = g.opt.k;
+  g.nt = L.nt;
+  g.ntr = 3 * L.nt;
+  g.m = 3 * L.nu;
+  // offsets of every instance in the packed mesh
+  for (Inst& I : g.in) {
+    I.nn = (I.nx + 1) * (I.ny + 1);
+    I.nc = 2 * I.nx * I.ny;
+    I.nf = hdg_rectangle_nfaces(I.nx, I.ny);
+    I.N = I.nf * g.nt;
+    I.node0 = g.nn;
+    I.cell0 = g.nc;
+    I.face0 = g.nf;
+    I.row0 = g.N;
+    g.nn += I.nn;
+    g.nc += I.nc;
+    g.nf += I.nf;
+    g.N += I.N;
+  }
+  if (g.nn > INT32_MAX || 2 * g.nf > INT32_MAX || g.nc > INT32_MAX / 9)
+    return fail(b, HDG_ERR_ARG, "group of order %d too large for 32-bit device indices", g.k);
+  BHIP(b, hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+  BHIP(b, hipEventCreateWithFlags(&g.ev[0], hipEventDisableTiming));
+  BHIP(b, hipEventCreateWithFlags(&g.ev[1], hipEventDisableTiming));
+  BHIP(b, hipHostMalloc((void**)&g.h_st, 3 * (size_t)n * sizeof(CgState), hipHostMallocDefault));
+  BHIP(b, hipHostMalloc((void**)&g.h_res, (2 * (size_t)n + 1) * sizeof(double), hipHostMallocDefault));
+
+  // host packing, in parallel over the instances
+  PackedHost H;
+  H.xy.assign(2 * g.nn, 0.0);
+  H.cn.assign(3 * g.nc, 0);
+  H.cf.assign(3 * g.nc, 0);
+  H.cell_inst.assign(g.nc, 0);
+  H.fcell.assign(2 * g.nf, 0);
+  H.fv.assign(2 * g.nf, 0);
+  H.vf_ptr.assign(g.nn + 1, 0);
+  H.vf_idx.assign(2 * g.nf, 0);
+  if (g.mg) H.fq4.assign(4 * g.nf, 0);
+  H.dmask.assign(g.N, 0);
+  {
+    std::atomic<int> next{0}, bad{0};
+    const int nth = std::max(1, std::min<int>(16, std::min<int>(n, (int)std::thread::hardware_concurrency())));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+      th.emplace_back([&] {
+        for (int j = next++; j < n; j = next++)
+          if (!pack_instance(b->reqs[g.in[j].req], g, j, H)) bad++;
+      });
+    for (auto& t : th) t.join();
+    if (bad) return fail(b, HDG_ERR_ARG, "mesh generation failed for a request of order %d", g.k);
+  }
+  // packed mesh, pattern, trace system
+  double* dxy;
+  int *dcn, *dcf, *dfc;
+  BTRY(upload(b, g, &dxy, H.xy));
+  BTRY(upload(b, g, &dcn, H.cn));
+  BTRY(upload(b, g, &dcf, H.cf));
+  BTRY(upload(b, g, &dfc, H.fcell));
+  BTRY(upload(b, g, &g.cell_inst, H.cell_inst));
+  BTRY(upload(b, g, &g.dmask, H.dmask));
+  g.mesh = DevMesh{g.nn, g.nc, g.nf, dxy, dcn, dcf, dfc};
+  BTRY(galloc(b, g, &g.pat.bcol, kSlots * g.nf));
+  BTRY(galloc(b, g, &g.pat.src, 2 * kSlots * g.nf));
+  BTRY(galloc(b, g, &g.pat.nslot, g.nf));
+  BTRY(galloc(b, g, &g.pat.self, g.nf));
+  BTRY(galloc(b, g, &g.pat.tslot, kSlots * g.nf));
+  BLAUNCH(b, launch_pattern(g.mesh, g.pat, g.s));
+  BTRY(galloc(b, g, &g.d_tab, L.total));
+  BHIP(b, hipMemcpyAsync(g.d_tab, tab.data.data(), sizeof(double) * L.total, hipMemcpyHostToDevice, g.s));
+  {
+    std::vector<SrcParams> sp(n);
+    for (int j = 0; j < n; ++j) sp[j] = src_params(b->reqs[g.in[j].req].src);
+    BTRY(upload(b, g, &g.srcs, sp));
+  }
+  const int64_t nc = g.nc, N = g.N;
+  BTRY(galloc(b, g, &g.Ate, nc * g.ntr * g.ntr));
+  BTRY(galloc(b, g, &g.bte, nc * g.ntr));
+  BTRY(galloc(b, g, &g.Ke, nc * g.m * g.ntr));
+  BTRY(galloc(b, g, &g.be, nc * g.m));
+  BTRY(galloc(b, g, &g.qu, nc * g.m));
+  BTRY(galloc(b, g, &g.uhat, N));
+  BTRY(galloc(b, g, &g.uD, N));
+  BTRY(galloc(b, g, &g.err, 1));
+  BHIP(b, hipMemsetAsync(g.uD, 0, sizeof(double) * N, g.s));
+  g.T = TraceSystem{};
+  g.T.nt = g.nt;
+  g.T.nf = g.nf;
+  g.T.N = N;
+  BTRY(galloc(b, g, &g.T.vals, (size_t)kSlots * g.nt * N));
+  BTRY(galloc(b, g, &g.T.rhs, N));
+  BTRY(galloc(b, g, &g.T.Dinv, (size_t)g.nt * N));
+  g.T.dmask = g.dmask;
+  g.T.uD = g.uD;
+
+  // CG segments: instance j gets the blocks it would get alone
+  std::vector<Seg> segs(n), csegs(n);
+  std::vector<int> binst, cbinst;
+  BTRY(galloc(b, g, &g.st, n));
+  BHIP(b, hipMemsetAsync(g.st, 0, sizeof(CgState) * n, g.s));
+  for (int j = 0; j < n; ++j) {
+    const Inst& I = g.in[j];
+    const int nb = cg_blocks(I.N, g.nt), nbe = errornorm_blocks(I.nc);
+    segs[j] = Seg{I.row0, I.row0 + I.N, (int)binst.size(), nb, g.st + j};
+    csegs[j] = Seg{I.cell0, I.cell0 + I.nc, (int)cbinst.size(), nbe, g.st + j};
+    binst.insert(binst.end(), nb, j);
+    cbinst.insert(cbinst.end(), nbe, j);
+  }
+  g.nbt = (int)binst.size();
+  g.nbc = (int)cbinst.size();
+  BTRY(upload(b, g, &g.segs, segs));
+  BTRY(upload(b, g, &g.csegs, csegs));
+  BTRY(upload(b, g, &g.binst, binst));
+  BTRY(upload(b, g, &g.cbinst, cbinst));
+  BTRY(galloc(b, g, &g.part2, g.nbc));
+  BTRY(galloc(b, g, &g.res, 2 * (size_t)n));
+  CgVectors& v = g.cg;
+  v.nblocks = g.nbt;
+  v.st = g.st;
+  BTRY(galloc(b, g, &v.x, N));
+  BTRY(galloc(b, g, &v.r, N));
+  BTRY(galloc(b, g, &v.z, N));
+  BTRY(galloc(b, g, &v.p0, N));
+  BTRY(galloc(b, g, &v.p1, N));
+  BTRY(galloc(b, g, &v.q, N));
+  BTRY(galloc(b, g, &v.part, 4 * (size_t)g.nbt + 64));
+  if (!g.mg) {
+    BHIP(b, hipStreamSynchronize(g.s));
+    return HDG_OK;
+  }
+
+  // multigrid: packed level buffers, per-instance views, launch tables
+  int maxlev = 1, ndepth = 1;
+  for (Inst& I : g.in) {
+    mg_plan_levels((int)I.nx, (int)I.ny, I.plan);
+    if (I.plan.tail >= I.plan.nlev) return fail(b, HDG_ERR_UNSUPPORTED, "vertex hierarchy without a tail");
+    maxlev = std::max(maxlev, I.plan.nlev);
+    ndepth = std::max(ndepth, I.plan.tail);
+  }
+  for (int l = 1; l < maxlev; ++l) {
+    int64_t off = 0;
+    for (Inst& I : g.in)
+      if (l < I.plan.nlev) {
+        I.lev_off[l] = off;
+        off += I.plan.lev[l].n;
+      }
+    LevelBuf& B = g.lev[l];
+    B.n = off;
+    BTRY(galloc(b, g, &B.a, kStencil * off));
+    BTRY(galloc(b, g, &B.dinv, off));
+    BTRY(galloc(b, g, &B.mask, off));
+    BTRY(galloc(b, g, &B.r, off));
+    BTRY(galloc(b, g, &B.e, off));
+    BTRY(galloc(b, g, &B.t, off));
+  }
+  int64_t tailn = 0;
+  for (Inst& I : g.in) {
+    I.tail_off = tailn;
+    for (int t = I.plan.tail; t < I.plan.nlev; ++t) tailn += (kStencil + 1) * I.plan.lev[t].n;
+  }
+  double *cellA, *cinv, *tailpack;
+  BTRY(galloc(b, g, &cellA, 9 * nc));
+  BTRY(galloc(b, g, &cinv, (size_t)n * kCoarseDirect * kCoarseDirect));
+  BTRY(galloc(b, g, &tailpack, tailn));
+  int *dfv, *dvp, *dvi, *dfq4;
+  BTRY(upload(b, g, &dfv, H.fv));
+  BTRY(upload(b, g, &dvp, H.vf_ptr));
+  BTRY(upload(b, g, &dvi, H.vf_idx));
+  BTRY(upload(b, g, &dfq4, H.fq4));
+  MgData& U = g.Mu;
+  U.nlev = 2;
+  U.nv = g.nn;
+  U.fv = dfv;
+  U.vf_ptr = dvp;
+  U.vf_idx = dvi;
+  U.fq4 = dfq4;
+  U.cellA = cellA;
+  {
+    MgLevel& L1 = U.lev[1];
+    const LevelBuf& B = g.lev[1];
+    L1.n = g.nn;
+    L1.a = B.a;
+    L1.dinv = B.dinv;
+    L1.mask = B.mask;
+    L1.r = B.r;
+    L1.e = B.e;
+    L1.t = B.t;
+  }
+  BTRY(galloc(b, g, &U.e0, N));
+  BTRY(galloc(b, g, &U.res0, N));
+  BTRY(galloc(b, g, &U.sp, 4 * N));
+  if (g.mixed) {
+    BTRY(galloc(b, g, &g.f_vals, (size_t)kSlots * g.nt * N));
+    BTRY(galloc(b, g, &g.f_Dinv, (size_t)g.nt * N));
+    BTRY(galloc(b, g, &g.f_sp, 4 * N));
+    BTRY(galloc(b, g, &g.f_e0, N));
+    BTRY(galloc(b, g, &g.f_res0, N));
+    BTRY(galloc(b, g, &g.f_xa, N));
+    BTRY(galloc(b, g, &g.f_xb, N));
+  }
+  const int vsweeps = g.k <= 2 ? 1 : 2;  // as hdg_setup chooses for a single problem
+  g.Mi.assign(n, MgData{});
+  for (int j = 0; j < n; ++j) {
+    const Inst& I = g.in[j];
+    MgData& M = g.Mi[j];
+    M = I.plan;
+    M.nv = I.nn;
+    M.vsweeps = vsweeps;
+    M.cellA = cellA + 9 * I.cell0;
+    M.cinv = cinv + (size_t)j * kCoarseDirect * kCoarseDirect;
+    M.tailpack = tailpack + I.tail_off;
+    for (int l = 1; l < M.nlev; ++l) {
+      MgLevel& V = M.lev[l];
+      const LevelBuf& B = g.lev[l];
+      const int64_t o = I.lev_off[l];
+      V.a = B.a + kStencil * o;
+      V.dinv = B.dinv + o;
+      V.mask = B.mask + o;
+      V.r = B.r + o;
+      V.e = B.e + o;
+      V.t = B.t + o;
+    }
+    if (I.lev_off[1] != I.node0) return fail(b, HDG_ERR_HIP, "level-1 layout differs from the node numbering");
+  }
+  BTRY(upload(b, g, &g.dMi, g.Mi));
+  MgGroup& G = g.G;
+  G.ninst = n;
+  G.dM = g.dMi;
+  G.st = g.st;
+  G.ndepth = ndepth;
+  G.vsweeps = vsweeps;
+  G.omega_v = g.Mi[0].omega_v;
+  for (int d = 1; d < ndepth; ++d) {
+    std::vector<VRec> rec;
+    std::vector<BlockMap> node, res;
+    for (int j = 0; j < n; ++j) {
+      const MgData& M = g.Mi[j];
+      if (d >= M.tail) continue;
+      const int p = (int)rec.size();
+      rec.push_back(VRec{M.lev[d], M.lev[d + 1], g.st + j});
+      const int64_t bn = (M.lev[d].n + 255) / 256, br = (8 * M.lev[d + 1].n + 255) / 256;
+      for (int64_t u = 0; u < bn; ++u) node.push_back(BlockMap{p, (int)u});
+      for (int64_t u = 0; u < br; ++u) res.push_back(BlockMap{p, (int)u});
+    }
+    VRec* drec;
+    BlockMap *dnode, *dres;
+    BTRY(upload(b, g, &drec, rec));
+    BTRY(upload(b, g, &dnode, node));
+    BTRY(upload(b, g, &dres, res));
+    G.node[d] = VGroup{drec, dnode};
+    G.res[d] = VGroup{drec, dres};
+    G.gnode[d] = (int)node.size();
+    G.gres[d] = (int)res.size();
+  }
+  BHIP(b, hipStreamSynchronize(g.s));  // the host arrays go out of scope
+  return HDG_OK;
+}
+
+Grouped rows(const Group& g) { return Grouped{g.segs, g.binst}; }
+TraceOp<double> op64(const Group& g) { return TraceOp<double>{g.nt, g.nf, g.N, g.T.vals, g.T.Dinv, g.dmask}; }
+TraceOp<float> op32(const Group& g) { return TraceOp<float>{g.nt, g.nf, g.N, g.f_vals, g.f_Dinv, g.dmask}; }
+Krylov<double> kv(const Group& g) {
+  const CgVectors& v = g.cg;
+  return Krylov<double>{v.r, v.z, v.p0, v.p1, v.q, v.x, v.part, v.st, v.nblocks};
+}
+MgVecs<double> mv64(const Group& g) { return MgVecs<double>{g.Mu.e0, g.Mu.res0, g.Mu.sp}; }
+MgVecs<float> mv32(const Group& g) { return MgVecs<float>{g.f_e0, g.f_res0, g.f_sp}; }
+
+double rtol2_of(const Group& g) {
+  const double rtol = g.opt.rtol > 0 ? g.opt.rtol : 1e-12;
+  return rtol * rtol;
+}
+int maxit_of(const Group& g) { return g.opt.maxit > 0 ? g.opt.maxit : 200000; }
+
+// condensation, assembly, preconditioner set-up, update(0)
+int enqueue_setup(hdg_batch* b, Group& g) {
+  BHIP(b, hipMemsetAsync(g.err, 0, sizeof(int), g.s));
+  CondenseArgs a;
+  a.mesh = g.mesh;
+  a.tables = g.d_tab;
+  a.lay = g.lay;
+  a.srcs = g.srcs;
+  a.cell_inst = g.cell_inst;
+  a.tau = g.opt.tau;
+  a.Ate = g.Ate;
+  a.bte = g.bte;
+  a.Ke = g.Ke;
+  a.be = g.be;
+  a.err = g.err;
+  BLAUNCH(b, launch_condense(g.k, a, false, g.s));
+  BLAUNCH(b, launch_fill(g.pat, g.Ate, g.bte, g.T, g.s));
+  BLAUNCH(b, launch_jacobi(g.pat, g.T, g.s));
+  const Grouped gr = rows(g);
+  const int n = (int)g.in.size();
+  if (!g.mg) {
+    BLAUNCH(b, launch_cg_init_t(g.pat, g.T, g.cg, gr, g.nbt, g.s));
+    return HDG_OK;
+  }
+  const double factor = g.Mi[0].omega_factor;
+  if (g.mixed) {
+    BLAUNCH(b, launch_to_float(g.T.vals, g.f_vals, (int64_t)kSlots * g.nt * g.N, g.s));
+    BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)g.nt * g.N, g.s));
+    BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, g.f_sp, g.s));
+    for (int j = 0; j < n; ++j) BLAUNCH(b, mg_setup_levels(g.Mi[j], g.s));
+    BLAUNCH(b, launch_mg_power_group(g.pat, op32(g), g.f_xa, g.f_xb, g.cg.part, gr, g.nbt, g.segs, n, factor, 10,
+                                     g.s));
+    BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op32(g), kv(g), mv32(g), gr, g.nbt, g.s));
+    BLAUNCH(b, launch_mg_precond_group(g.pat, op32(g), kv(g), g.Mu, g.G, mv32(g), gr, g.nbt, g.s));
+  } else {
+    BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, nullptr, g.s));
+    for (int j = 0; j < n; ++j) BLAUNCH(b, mg_setup_levels(g.Mi[j], g.s));
+    // the CG vectors x, r serve as scratch before the solve initialises them
+    BLAUNCH(b, launch_mg_power_group(g.pat, op64(g), g.cg.x, g.cg.r, g.cg.part, gr, g.nbt, g.segs, n, factor, 10,
+                                     g.s));
+    BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op64(g), kv(g), mv64(g), gr, g.nbt, g.s));
+    BLAUNCH(b, launch_mg_precond_group(g.pat, op64(g), kv(g), g.Mu, g.G, mv64(g), gr, g.nbt, g.s));
+  }
+  return HDG_OK;
+}
+
+int enqueue_chunk(hdg_batch* b, Group& g) {
+  const int iters = g.mg ? 4 : 64;
+  const Grouped gr = rows(g);
+  for (int it = 0; it < iters; ++it) {
+    BLAUNCH(b, launch_cg_spmv_t(g.pat, op64(g), kv(g), gr, g.nbt, 0, rtol2_of(g), maxit_of(g), 0, g.s));
+    if (g.mixed) {
+      BLAUNCH(b, launch_mg_update_group(op32(g), kv(g), mv32(g), gr, g.nbt, g.s));
+      BLAUNCH(b, launch_mg_precond_group(g.pat, op32(g), kv(g), g.Mu, g.G, mv32(g), gr, g.nbt, g.s));
+    } else if (g.mg) {
+      BLAUNCH(b, launch_mg_update_group(op64(g), kv(g), mv64(g), gr, g.nbt, g.s));
+      BLAUNCH(b, launch_mg_precond_group(g.pat, op64(g), kv(g), g.Mu, g.G, mv64(g), gr, g.nbt, g.s));
+    } else {
+      BLAUNCH(b, launch_cg_update_t(g.T, g.cg, gr, g.nbt, 0, 0, g.s));
+    }
+  }
+  return HDG_OK;
+}
+
+// One chunk of iterations for every instance of the group, then the states
+// into h_st[slot]; captured once as a graph unless HDG_SCHEDULE_EAGER.
+int launch_chunk(hdg_batch* b, Group& g, int slot) {
+  const size_t n = g.in.size();
+  CgState* dst = g.h_st + slot * n;
+  if (!g.eager && !g.gexec[slot]) {
+    hipGraph_t gr = nullptr;
+    BHIP(b, hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal));
+    int r = enqueue_chunk(b, g);
+    hipError_t e = hipMemcpyAsync(dst, g.st, sizeof(CgState) * n, hipMemcpyDeviceToHost, g.s);
+    hipError_t e2 = hipStreamEndCapture(g.s, &gr);
+    if (r != HDG_OK) return r;
+    BHIP(b, e);
+    BHIP(b, e2);
+    e = hipGraphInstantiate(&g.gexec[slot], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    BHIP(b, e);
+  }
+  if (!g.eager) {
+    BHIP(b, hipGraphLaunch(g.gexec[slot], g.s));
+  } else {
+    BTRY(enqueue_chunk(b, g));
+    BHIP(b, hipMemcpyAsync(dst, g.st, sizeof(CgState) * n, hipMemcpyDeviceToHost, g.s));
+  }
+  BHIP(b, hipEventRecord(g.ev[slot], g.s));
+  return HDG_OK;
+}
+
+bool all_done(const Group& g, int slot) {
+  const size_t n = g.in.size();
+  const CgState* s = g.h_st + slot * n;
+  for (size_t j = 0; j < n; ++j)
+    if (!s[j].done) return false;
+  return true;
+}
+
+// uhat, FP64 residuals, recovery, error norms, and the read-back of states,
+// residuals and the error flag
+int enqueue_finish(hdg_batch* b, Group& g) {
+  const int n = (int)g.in.size();
+  const Grouped gr = rows(g);
+  BLAUNCH(b, launch_cg_finish(g.T, g.cg, g.uhat, g.s));
+  BLAUNCH(b, launch_true_residual_t(g.pat, g.T, g.uhat, g.cg.part, gr, g.nbt, g.s));
+  BLAUNCH(b, launch_reduce_segs(g.cg.part, g.segs, n, g.res, g.s));
+  BLAUNCH(b, launch_recover(g.k, g.mesh, g.Ke, g.be, g.uhat, g.qu, g.s));
+  bool want = false;
+  for (const Inst& I : g.in) want = want || b->reqs[I.req].uex_kind >= 0;
+  if (want) {
+    const double p3[3] = {0.0, 0.0, 0.0};
+    BLAUNCH(b, launch_errornorm_t(g.mesh, g.d_tab, g.lay, g.k, g.qu, HDG_UEX_SINSIN, p3, g.part2,
+                                  Grouped{g.csegs, g.cbinst}, g.nbc, g.s));
+    BLAUNCH(b, launch_reduce_segs(g.part2, g.csegs, n, g.res + n, g.s));
+  } else {
+    BHIP(b, hipMemsetAsync(g.res + n, 0, sizeof(double) * n, g.s));
+  }
+  BHIP(b, hipMemcpyAsync(g.h_st + 2 * n, g.st, sizeof(CgState) * n, hipMemcpyDeviceToHost, g.s));
+  BHIP(b, hipMemcpyAsync(g.h_res, g.res, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, g.s));
+  BHIP(b, hipMemcpyAsync(g.h_res + 2 * n, g.err, sizeof(int), hipMemcpyDeviceToHost, g.s));
+  return HDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdg_batch_create(hdg_batch** out, int device, size_t hbm_budget, int n, const hdg_request* reqs) {
+  if (!out) return HDG_ERR_ARG;
+  *out = nullptr;
+  if (n < 1 || !reqs) return HDG_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HDG_ERR_HIP;
+  if (device < 0 || device >= ndev) return HDG_ERR_ARG;
+  hdg_batch* b = new hdg_batch();
+  b->device = device;
+  b->budget = hbm_budget;
+  b->reqs.assign(reqs, reqs + n);
+  b->where.assign(n, {0, 0});
+  auto bail = [&](int rc) {
+    *out = b;  // keeps the message readable through hdg_batch_last_error
+    return rc;
+  };
+  for (int i = 0; i < n; ++i) {
+    hdg_request& q = b->reqs[i];
+    if (q.nx < 1 || q.ny < 1) return bail(fail(b, HDG_ERR_ARG, "request %d: nx, ny must be >= 1", i));
+    if (q.opt.k < 1 || q.opt.k > 4) return bail(fail(b, HDG_ERR_UNSUPPORTED, "request %d: k=%d (1..4)", i, q.opt.k));
+    if (q.opt.quad_degree <= 0) q.opt.quad_degree = q.opt.k + 1;
+    if (q.opt.precision != HDG_FP64 && q.opt.precision != HDG_MIXED)
+      return bail(fail(b, HDG_ERR_ARG, "request %d: unknown precision %d", i, q.opt.precision));
+    if (q.opt.precond < HDG_PC_AUTO || q.opt.precond > HDG_PC_MULTIGRID)
+      return bail(fail(b, HDG_ERR_ARG, "request %d: unknown preconditioner %d", i, q.opt.precond));
+    if (q.opt.precision == HDG_MIXED && q.opt.precond == HDG_PC_BLOCK_JACOBI)
+      return bail(fail(b, HDG_ERR_UNSUPPORTED, "request %d: mixed precision runs on the multigrid path", i));
+    if (q.src.kind < 0 || q.src.kind > 2)
+      return bail(fail(b, HDG_ERR_UNSUPPORTED, "request %d: source kind %d (batches take analytic sources 0..2)",
+                       i, q.src.kind));
+    if (q.uex_kind != -1 && q.uex_kind != HDG_UEX_SINSIN)
+      return bail(fail(b, HDG_ERR_ARG, "request %d: batches report HDG_UEX_SINSIN error norms only", i));
+    int gi = -1;
+    for (size_t t = 0; t < b->groups.size(); ++t)
+      if (same_group(b->groups[t].opt, q.opt)) gi = (int)t;
+    if (gi < 0) {
+      b->groups.emplace_back();
+      Group& g = b->groups.back();
+      g.opt = q.opt;
+      g.mg = q.opt.precond != HDG_PC_BLOCK_JACOBI;
+      g.mixed = g.mg && q.opt.precision == HDG_MIXED;
+      g.eager = (q.opt.schedule & HDG_SCHEDULE_EAGER) != 0;
+      gi = (int)b->groups.size() - 1;
+    }
+    Inst I;
+    I.req = i;
+    I.nx = q.nx;
+    I.ny = q.ny;
+    b->where[i] = {gi, (int)b->groups[gi].in.size()};
+    b->groups[gi].in.push_back(I);
+  }
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(b, HDG_ERR_HIP, "hipSetDevice failed"));
+  {
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  }
+  for (Group& g : b->groups) {
+    const int rc = build_group(b, g);
+    if (rc != HDG_OK) return bail(rc);
+  }
+  *out = b;
+  return HDG_OK;
+}
+
+int hdg_batch_run(hdg_batch* b, hdg_result* res) {
+  if (!b || !res) return HDG_ERR_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return fail(b, HDG_ERR_HIP, "hipSetDevice failed");
+  for (Group& g : b->groups)
+    if (!g.s) return fail(b, HDG_ERR_STATE, "batch was not built");
+  const auto t0 = std::chrono::steady_clock::now();
+  auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  for (Group& g : b->groups) {
+    BTRY(enqueue_setup(b, g));
+    g.slot = 0;
+    g.active = true;
+    BTRY(launch_chunk(b, g, 0));
+  }
+  // every group keeps one chunk in flight while its previous chunk's states
+  // are read; a group leaves the loop when all its instances stopped
+  for (;;) {
+    bool any = false;
+    for (Group& g : b->groups)
+      if (g.active) BTRY(launch_chunk(b, g, g.slot ^ 1));
+    for (Group& g : b->groups) {
+      if (!g.active) continue;
+      BHIP(b, hipEventSynchronize(g.ev[g.slot]));
+      if (all_done(g, g.slot)) {
+        BHIP(b, hipEventSynchronize(g.ev[g.slot ^ 1]));
+        g.active = false;
+        BTRY(enqueue_finish(b, g));
+      } else {
+        g.slot ^= 1;
+        any = true;
+      }
+    }
+    if (!any) break;
+  }
+  int first_bad = HDG_OK;
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    Group& g = b->groups[gi];
+    BHIP(b, hipStreamSynchronize(g.s));
+    g.t_done = now();
+    const int n = (int)g.in.size();
+    int flag = 0;
+    std::memcpy(&flag, g.h_res + 2 * n, sizeof(int));
+    int gerr = HDG_OK;
+    if (flag & 1) gerr = HDG_ERR_NEG_JACOBIAN;
+    else if (flag & 2) gerr = HDG_ERR_LOCAL_SINGULAR;
+    for (int j = 0; j < n; ++j) {
+      const Inst& I = g.in[j];
+      const CgState& s = g.h_st[2 * n + j];
+      hdg_result& r = res[I.req];
+      std::memset(&r, 0, sizeof(r));
+      r.id = I.req;
+      r.iters = s.it;
+      r.relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+      r.true_relres = s.bb > 0 ? std::sqrt(g.h_res[j] / s.bb) : 0.0;
+      r.errornorm = b->reqs[I.req].uex_kind >= 0 ? g.h_res[n + j] : 0.0;
+      r.n_cells = I.nc;
+      r.n_dofs = I.N;
+      r.t_submit = 0.0;
+      r.t_start = 0.0;
+      r.t_done = g.t_done;
+      r.device_bytes = (size_t)((double)g.bytes * I.nc / std::max<int64_t>(g.nc, 1));
+      r.slot = (int)gi;
+      r.status = gerr != HDG_OK ? gerr : (s.done == 1 ? HDG_OK : HDG_ERR_NOT_CONVERGED);
+      if (r.status != HDG_OK && first_bad == HDG_OK) {
+        first_bad = r.status;
+        fail(b, r.status, "request %d: %s", I.req,
+             gerr == HDG_ERR_NEG_JACOBIAN   ? "det(J) is not positive for a cell"
+             : gerr == HDG_ERR_LOCAL_SINGULAR ? "singular local (q,u) matrix; raise quad_degree"
+             : s.done == 2                    ? "trace solve reached maxit"
+                                              : "trace solve broke down");
+      }
+    }
+  }
+  b->ran = true;
+  return first_bad;
+}
+
+int hdg_batch_trace(hdg_batch* b, int i, double* uhat) {
+  if (!b || !uhat || i < 0 || i >= (int)b->reqs.size()) return HDG_ERR_ARG;
+  if (!b->ran) return fail(b, HDG_ERR_STATE, "hdg_batch_run has not been called");
+  if (hipSetDevice(b->device) != hipSuccess) return fail(b, HDG_ERR_HIP, "hipSetDevice failed");
+  Group& g = b->groups[b->where[i].first];
+  const Inst& I = g.in[b->where[i].second];
+  BHIP(b, hipMemcpyAsync(uhat, g.uhat + I.row0, sizeof(double) * I.N, hipMemcpyDeviceToHost, g.s));
+  BHIP(b, hipStreamSynchronize(g.s));
+  return HDG_OK;
+}
+
+int hdg_batch_info(const hdg_batch* b, int* ngroups, size_t* device_bytes) {
+  if (!b) return HDG_ERR_ARG;
+  if (ngroups) *ngroups = (int)b->groups.size();
+  if (device_bytes) *device_bytes = b->used;
+  return HDG_OK;
+}
+
+void hdg_batch_destroy(hdg_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  for (Group& g : b->groups) free_group(b, g);
+  delete b;
+}
+
+const char* hdg_batch_last_error(const hdg_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+}  // extern "C"

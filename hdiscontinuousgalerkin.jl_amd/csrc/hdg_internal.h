@@ -47,11 +47,15 @@ struct TabLayoutDev {
 // Element build + condensation.  Per cell outputs: Ate row-major ntr x ntr,
 // bte[ntr], Ke column-major m x ntr, be[m].  Element blocks mode writes the
 // A..H blocks instead (column-major per cell) for cells [first, first+count).
+// A grouped launch (several instances packed into one mesh) takes the source
+// of cell c from srcs[cell_inst[c]] instead of src.
 struct CondenseArgs {
   DevMesh mesh;
   const double* tables = nullptr;
   TabLayoutDev lay;
   SrcParams src;
+  const SrcParams* srcs = nullptr;
+  const int* cell_inst = nullptr;
   double tau = 1.0;
   double* Ate = nullptr;
   double* bte = nullptr;
@@ -100,6 +104,30 @@ struct CgVectors {
   CgState* st;
   int nblocks;
 };
+
+// ------------------------------------------------------ grouped launches
+// Several independent problems (instances) packed into one mesh share every
+// launch.  Instance j owns rows (or cells) [lo, hi) and the blocks
+// [b0, b0 + nb) of the launch; its blocks visit its rows with stride
+// nb * blockDim, and its reductions run over its own blocks' partial sums
+// only, so an instance gets the bits it would get alone with nb blocks.
+// A selector turns the block index into the instance's segment: Whole is
+// one instance spanning the grid, Grouped looks the block up in a table.
+struct Seg {
+  int64_t lo, hi;
+  int b0, nb;
+  CgState* st;
+};
+struct Whole {
+  int64_t n;
+  CgState* st;
+};
+struct Grouped {
+  const Seg* seg;
+  const int* binst;  // block -> instance
+};
+__device__ __forceinline__ Seg seg_of(const Whole& w) { return Seg{0, w.n, 0, (int)gridDim.x, w.st}; }
+__device__ __forceinline__ Seg seg_of(const Grouped& g) { return g.seg[g.binst[blockIdx.x]]; }
 
 // Views for the CG kernels.  TraceOp<W> is the trace operator stored as W:
 // FP64 for the CG product, FP64 or float for the multigrid preconditioner
@@ -173,6 +201,33 @@ struct MgData {
   int vsweeps = 2;               // pre- and post-smoothing sweeps on the vertex levels (1 or 2)
 };
 
+// A fine level, the next coarser one and the CG state of one instance: the
+// unit of a vertex-level kernel.  VOne launches one instance over the whole
+// grid; VGroup maps each block to (instance, block of that instance).
+struct VRec {
+  MgLevel F, C;
+  const CgState* st;
+};
+struct BlockMap {
+  int inst, lb;
+};
+struct VOne {
+  VRec r;
+};
+struct VGroup {
+  const VRec* rec;
+  const BlockMap* map;
+};
+__device__ __forceinline__ const VRec& vrec_of(const VOne& o, int& lb) {
+  lb = blockIdx.x;
+  return o.r;
+}
+__device__ __forceinline__ const VRec& vrec_of(const VGroup& g, int& lb) {
+  const BlockMap m = g.map[blockIdx.x];
+  lb = m.lb;
+  return g.rec[m.inst];
+}
+
 // launchers (return hipError_t as int)
 int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s);
 int launch_condense(int k, const CondenseArgs& a, bool blocks_mode, hipStream_t s);
@@ -198,9 +253,38 @@ int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev&
                      const double* qu, int uex_kind, const double* par3, double* part,
                      double* result, hipStream_t s);
 int cg_blocks(int64_t N, int nt);
+int errornorm_blocks(int64_t nc);
+
+// Selector-generic forms of the CG and norm launchers (G = Whole or Grouped,
+// grid = blocks of the launch; the Whole forms above call these).
+template <class G>
+int launch_cg_init_t(const DevPattern& P, const TraceSystem& T, CgVectors& v, G g, int grid, hipStream_t s);
+template <class G>
+int launch_cg_spmv_t(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, G g, int grid,
+                     int i_fixed, double rtol2, int maxit, int force, hipStream_t s);
+template <class G>
+int launch_cg_update_t(const TraceSystem& T, CgVectors& v, G g, int grid, int i_fixed, int force, hipStream_t s);
+// per-block partials of ||K uhat - rhs||^2
+template <class G>
+int launch_true_residual_t(const DevPattern& P, const TraceSystem& T, const double* uhat, double* part, G g,
+                           int grid, hipStream_t s);
+// per-block partials of the squared L2 error
+template <class G>
+int launch_errornorm_t(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k, const double* qu,
+                       int uex_kind, const double* par3, double* part, G g, int grid, hipStream_t s);
+// out[j] = sum of part over the blocks of segment j
+int launch_reduce_segs(const double* part, const Seg* seg, int nseg, double* out, hipStream_t s);
+
 // vertex hierarchy and S P0 (FP64, and a float copy of S P0 when sp32 is set)
 int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
                      float* sp32, hipStream_t s);
+// the two halves of mg_setup_numeric: the parts that run over the whole
+// (possibly packed) mesh -- vertex masks, per-cell coarse blocks, S P0 --
+// and the vertex levels of one structured instance (M describes it, with
+// its arrays at its own offsets)
+int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
+                  const double* Ate, float* sp32, hipStream_t s);
+int mg_setup_levels(const MgData& M, hipStream_t s);
 // omega = factor / lambda_max(Dinv S) by power steps on the operator as stored
 template <class V>
 int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
@@ -211,9 +295,43 @@ int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>
                    const MgVecs<W>& mv, hipStream_t s);
 template <class W>
 int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, hipStream_t s);
+// dM: a device copy of M (read by the one-workgroup tail kernel)
 template <class W>
 int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
-                      const MgVecs<W>& mv, hipStream_t s);
+                      const MgData* dM, const MgVecs<W>& mv, hipStream_t s);
+
+// Launch tables of the multigrid preconditioner of a group of instances:
+// per vertex depth d (1 <= d < ndepth), the instances whose level d lies
+// above their one-workgroup tail, with block maps for the fine-node kernels
+// (node, gnode blocks) and the restriction (res, gres blocks); then one
+// tail workgroup per instance.
+struct MgGroup {
+  int ninst = 0;
+  const MgData* dM = nullptr;  // [ninst] per-instance hierarchies (device copies)
+  CgState* st = nullptr;       // [ninst] CG states
+  int ndepth = 1;
+  VGroup node[kMaxLevels];
+  int gnode[kMaxLevels];
+  VGroup res[kMaxLevels];
+  int gres[kMaxLevels];
+  int vsweeps = 2;
+  double omega_v = 0.8;
+};
+// grouped forms: rows of each instance from the Grouped selector (grid
+// blocks); Mu describes the packed mesh (its level 1 spans every
+// instance's vertices)
+template <class V>
+int launch_mg_power_group(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, Grouped g, int grid,
+                          const Seg* seg, int ninst, double factor, int steps, hipStream_t s);
+template <class W>
+int launch_mg_init_group(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
+                         const MgVecs<W>& mv, Grouped g, int grid, hipStream_t s);
+template <class W>
+int launch_mg_update_group(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, Grouped g, int grid,
+                           hipStream_t s);
+template <class W>
+int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& Mu,
+                            const MgGroup& G, const MgVecs<W>& mv, Grouped g, int grid, hipStream_t s);
 // float copy of n doubles
 int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
 

@@ -227,7 +227,13 @@ __device__ inline void build_row(int r, const double* T, const TabLayoutDev& L, 
   }
 }
 
-template <int K, bool BLOCKS>
+template <bool GROUPED>
+__device__ __forceinline__ const SrcParams& src_of(const CondenseArgs& a, int64_t cell) {
+  if constexpr (GROUPED) return a.srcs[a.cell_inst[cell]];
+  else return a.src;
+}
+
+template <int K, bool BLOCKS, bool GROUPED>
 __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
   using D = Dims<K>;
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, SEG = D::SEG, EPW = D::EPW;
@@ -249,6 +255,7 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
     const int64_t cell = first + (valid ? base + seg : count - 1);
     Geo g;
     cell_geometry(a.mesh, cell, g);
+    const SrcParams& S = src_of<GROUPED>(a, cell);
     // quadrature-point data shared by the segment: lane q evaluates
     // detJ * f(x_q) and detJ / kappa(x_q); rows collect them by shuffles
     const int ui = (ls < NQ) ? ls % NU : (ls < M ? ls - NQ : 0);  // scalar basis index of this row
@@ -266,15 +273,14 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
           xq += T[L.Mg + v * nqp + q] * g.x[2 * v];
           yq += T[L.Mg + v * nqp + q] * g.x[2 * v + 1];
         }
-        fv = g.detJ * (a.src.kind == 3 ? __ldg(a.src.f_qp + cell + a.mesh.nc * (int64_t)q)
-                                       : Src::f(a.src, xq, yq));
-        kv = g.detJ / Src::kappa(a.src, xq, yq);
+        fv = g.detJ * (S.kind == 3 ? __ldg(S.f_qp + cell + a.mesh.nc * (int64_t)q) : Src::f(S, xq, yq));
+        kv = g.detJ / Src::kappa(S, xq, yq);
       }
       const int qn = min(SEG, nqp - q0);
       for (int qq = 0; qq < qn; ++qq) {
         const double fq = __shfl(fv, sbase + qq, 64);
         beacc += T[L.Nw + ui * nqp + q0 + qq] * fq;
-        if (a.src.kappa_on) {
+        if (S.kappa_on) {
           const double kq = __shfl(kv, sbase + qq, 64);
           const double wn = T[L.Nw + ui * nqp + q0 + qq] * kq;
           if (qq == 0 && q0 == 0) {
@@ -555,20 +561,24 @@ __device__ inline double sum_partials(const double* p, int n, double* sh) {
 }
 
 // CG iteration i, first half: convergence test on the (r, r) partials of
-// update(i-1), beta, p = z + beta p_old, q = S p, partial (p, q).
-template <int NT>
+// update(i-1), beta, p = z + beta p_old, q = S p, partial (p, q).  G selects
+// the instance of the block (one, or one of a group).
+template <int NT, class G>
 __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceOp<double> T, Krylov<double> v,
-                                                             int i_fixed, double rtol2, int maxit, int force) {
+                                                             G gsel, int i_fixed, double rtol2, int maxit,
+                                                             int force) {
   constexpr int NW = kCgThreads / 64;
   __shared__ double sh[NW];
-  CgState* st = v.st;
+  const Seg S = seg_of(gsel);
+  CgState* st = S.st;
   if (!force && st->done) return;
   const int i = i_fixed > 0 ? i_fixed : st->i_spmv;
-  const int nb = v.nblocks;
-  const double rz = sum_partials<NW>(v.part + (size_t)nb * (1 + (i & 1)), nb, sh);
-  const double rr = sum_partials<NW>(v.part + (size_t)nb * 3, nb, sh);
+  const size_t nbt = (size_t)v.nblocks;  // partial-sum slot stride of the whole launch
+  const double rz = sum_partials<NW>(v.part + nbt * (1 + (i & 1)) + S.b0, S.nb, sh);
+  const double rr = sum_partials<NW>(v.part + nbt * 3 + S.b0, S.nb, sh);
   const double bb = (i == 1) ? rr : st->bb;
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int lb = blockIdx.x - S.b0;
+  const bool lead = lb == 0 && threadIdx.x == 0;
   if (!force) {
     const bool conv = rr <= rtol2 * bb;
     if (conv || i - 1 >= maxit) {
@@ -594,8 +604,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
   const double* __restrict__ z = v.z;
   double* __restrict__ qo = v.q;
   double part = 0.0;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N;
-       r += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t r = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
     const int64_t f = r / NT;
     const int* bc = P.bcol + f * kSlots;
     double acc = 0.0;
@@ -616,20 +625,23 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
   if (threadIdx.x == 0) v.part[blockIdx.x] = bs;
 }
 
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, int i_fixed, int force) {
+template <int NT, class G>
+__global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, G gsel, int i_fixed,
+                                                            int force) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
-  CgState* st = v.st;
+  const Seg S = seg_of(gsel);
+  CgState* st = S.st;
   if (!force && st->done) return;
   const int i = i_fixed > 0 ? i_fixed : st->i_upd;
-  if (!force && blockIdx.x == 0 && threadIdx.x == 0) st->i_spmv = i + 1;
-  const int nb = v.nblocks;
-  const double pq = sum_partials<NT>(v.part, nb, sh);
+  const int lb = blockIdx.x - S.b0;
+  if (!force && lb == 0 && threadIdx.x == 0) st->i_spmv = i + 1;
+  const size_t nbt = (size_t)v.nblocks;
+  const double pq = sum_partials<NT>(v.part + S.b0, S.nb, sh);
   const double rz = st->rz[i & 1];
   if (!force && !(pq > 0.0)) {  // loss of positive curvature or NaN
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lb == 0 && threadIdx.x == 0) {
       st->done = 3;
       st->it = i - 1;
     }
@@ -638,10 +650,10 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
   const double alpha = rz / pq;
   const double* __restrict__ p = (i & 1) ? v.p1 : v.p0;
   double prz = 0.0, prr = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
     double rn = 0.0;
-    if (r < T.N) {
+    if (r < S.hi) {
       const double xn = fma(alpha, p[r], v.x[r]);
       rn = fma(-alpha, v.q[r], v.r[r]);
       if (!force) {
@@ -651,11 +663,11 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
     }
     rs[threadIdx.x] = rn;
     __syncthreads();
-    if (r < T.N) {
-      const int lb = (threadIdx.x / NT) * NT;
+    if (r < S.hi) {
+      const int lr = (threadIdx.x / NT) * NT;
       double zr = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) zr = fma(T.Dinv[(int64_t)b * T.N + r], rs[lb + b], zr);
+      for (int b = 0; b < NT; ++b) zr = fma(T.Dinv[(int64_t)b * T.N + r], rs[lr + b], zr);
       v.z[r] = zr;
       prz = fma(rn, zr, prz);
       prr = fma(rn, rn, prr);
@@ -665,23 +677,25 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
   const double a = block_sum<NT>(prz, sh);
   const double b = block_sum<NT>(prr, sh);
   if (threadIdx.x == 0 && !force) {
-    v.part[(size_t)nb * (1 + ((i + 1) & 1)) + blockIdx.x] = a;
-    v.part[(size_t)nb * 3 + blockIdx.x] = b;
+    v.part[nbt * (1 + ((i + 1) & 1)) + blockIdx.x] = a;
+    v.part[nbt * 3 + blockIdx.x] = b;
   }
 }
 
 // update(0): r0 = b~ = -(rhs - K uD) on free dofs, x = 0, z = Dinv r0,
 // p = 0; partials of (r0, z0) and (r0, r0).
-template <int NT>
-__global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSystem T, CgVectors v) {
+template <int NT, class G>
+__global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSystem T, CgVectors v, G gsel) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
+  const Seg S = seg_of(gsel);
+  const int lb = blockIdx.x - S.b0;
   double prz = 0.0, prr = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
     double b = 0.0;
-    if (r < T.N) {
+    if (r < S.hi) {
       if (!T.dmask[r]) {
         const int64_t f = r / NT;
         double s = T.rhs[r];
@@ -700,11 +714,11 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
     }
     rs[threadIdx.x] = b;
     __syncthreads();
-    if (r < T.N) {
-      const int lb = (threadIdx.x / NT) * NT;
+    if (r < S.hi) {
+      const int lr = (threadIdx.x / NT) * NT;
       double zr = 0.0;
 #pragma unroll
-      for (int c = 0; c < NT; ++c) zr = fma(T.Dinv[(int64_t)c * T.N + r], rs[lb + c], zr);
+      for (int c = 0; c < NT; ++c) zr = fma(T.Dinv[(int64_t)c * T.N + r], rs[lr + c], zr);
       v.z[r] = zr;
       prz = fma(b, zr, prz);
       prr = fma(b, b, prr);
@@ -717,8 +731,8 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
     v.part[(size_t)v.nblocks * 2 + blockIdx.x] = a;  // parity of update(0) + 1 = 1
     v.part[(size_t)v.nblocks * 3 + blockIdx.x] = bb;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    CgState* st = v.st;
+  if (lb == 0 && threadIdx.x == 0) {
+    CgState* st = S.st;
     st->done = 0;
     st->it = 0;
     st->pq = 0.0;
@@ -753,14 +767,24 @@ __global__ void cg_finish_kernel(TraceSystem T, const double* x, double* uhat) {
   uhat[r] = T.dmask[r] ? T.uD[r] : x[r];
 }
 
+// Sum of each instance's partials by one block per instance: out[j] over
+// part[seg[j].b0 .. seg[j].b0 + seg[j].nb), in reduce_kernel's order.
+__global__ __launch_bounds__(256) void reduce_seg_kernel(const double* part, const Seg* seg, double* out) {
+  __shared__ double sh[4];
+  const Seg S = seg[blockIdx.x];
+  const double s = sum_partials<4>(part + S.b0, S.nb, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 // ||K uhat - rhs||^2 over the free dofs (the FP64 residual of the solve).
-template <int NT>
+template <int NT, class G>
 __global__ __launch_bounds__(kCgThreads) void true_residual_kernel(DevPattern P, TraceSystem T,
-                                                                   const double* uhat, double* part) {
+                                                                   const double* uhat, double* part, G gsel) {
   __shared__ double sh[kCgThreads / 64];
+  const Seg S = seg_of(gsel);
+  const int lb = blockIdx.x - S.b0;
   double acc2 = 0.0;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N;
-       r += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t r = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
     if (T.dmask[r]) continue;
     const int64_t f = r / NT;
     double s = -T.rhs[r];
@@ -799,17 +823,20 @@ __global__ __launch_bounds__(256) void recover_kernel(DevMesh m, const double* _
 }
 
 // ------------------------------------------------------------ errornorm
-template <int K>
+// Per block: sum over its cells of int (u_h - u_ex)^2; G selects the
+// instance (cells [lo, hi)).
+template <int K, class G>
 __global__ __launch_bounds__(256) void errornorm_kernel(DevMesh m, const double* __restrict__ Tg,
                                                         TabLayoutDev L, const double* __restrict__ qu,
                                                         int kind, double c0, double c1, double c2,
-                                                        double* part) {
+                                                        double* part, G gsel) {
   using D = Dims<K>;
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M;
   __shared__ double sh[4];
+  const Seg S = seg_of(gsel);
+  const int lb = blockIdx.x - S.b0;
   double acc = 0.0;
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < m.nc;
-       c += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t c = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; c < S.hi; c += (int64_t)S.nb * blockDim.x) {
     Geo g;
     cell_geometry(m, c, g);
     double el = 0.0;
@@ -848,9 +875,11 @@ int condense_launch(const CondenseArgs& a, bool blocks, hipStream_t s) {
   const int grid = grid_for(n, (int)per_block, 8192);
   const size_t lds = sizeof(double) * a.lay.total;
   if (blocks)
-    hipLaunchKernelGGL((condense_kernel<K, true>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, true, false>), dim3(grid), dim3(256), lds, s, a);
+  else if (a.srcs)
+    hipLaunchKernelGGL((condense_kernel<K, false, true>), dim3(grid), dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL((condense_kernel<K, false>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, false, false>), dim3(grid), dim3(256), lds, s, a);
   return (int)hipGetLastError();
 }
 
@@ -904,29 +933,91 @@ int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s) {
-#define INIT(NT) hipLaunchKernelGGL(cg_init_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, P, T, v);
+template <class G>
+int launch_cg_init_t(const DevPattern& P, const TraceSystem& T, CgVectors& v, G g, int grid, hipStream_t s) {
+#define INIT(NT) hipLaunchKernelGGL((cg_init_kernel<NT, G>), dim3(grid), dim3(64 * NT), 0, s, P, T, v, g);
   HDG_NT_DISPATCH(T.nt, INIT)
 #undef INIT
   return (int)hipGetLastError();
 }
 
-int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i_fixed,
-                   double rtol2, int maxit, int force, hipStream_t s) {
+template <class G>
+int launch_cg_spmv_t(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, G g, int grid,
+                     int i_fixed, double rtol2, int maxit, int force, hipStream_t s) {
 #define SPMV(NT)                                                                                          \
-  hipLaunchKernelGGL(cg_spmv_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, v, i_fixed, rtol2, \
+  hipLaunchKernelGGL((cg_spmv_kernel<NT, G>), dim3(grid), dim3(kCgThreads), 0, s, P, T, v, g, i_fixed, rtol2, \
                      maxit, force);
   HDG_NT_DISPATCH(T.nt, SPMV)
 #undef SPMV
   return (int)hipGetLastError();
 }
 
-int launch_cg_update(const TraceSystem& T, CgVectors& v, int i_fixed, int force, hipStream_t s) {
+template <class G>
+int launch_cg_update_t(const TraceSystem& T, CgVectors& v, G g, int grid, int i_fixed, int force,
+                       hipStream_t s) {
 #define UPD(NT) \
-  hipLaunchKernelGGL(cg_update_kernel<NT>, dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, i_fixed, force);
+  hipLaunchKernelGGL((cg_update_kernel<NT, G>), dim3(grid), dim3(64 * NT), 0, s, T, v, g, i_fixed, force);
   HDG_NT_DISPATCH(T.nt, UPD)
 #undef UPD
   return (int)hipGetLastError();
+}
+
+template <class G>
+int launch_true_residual_t(const DevPattern& P, const TraceSystem& T, const double* uhat, double* part, G g,
+                           int grid, hipStream_t s) {
+#define TR(NT) \
+  hipLaunchKernelGGL((true_residual_kernel<NT, G>), dim3(grid), dim3(kCgThreads), 0, s, P, T, uhat, part, g);
+  HDG_NT_DISPATCH(T.nt, TR)
+#undef TR
+  return (int)hipGetLastError();
+}
+
+template <class G>
+int launch_errornorm_t(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k, const double* qu,
+                       int uex_kind, const double* par3, double* part, G g, int grid, hipStream_t s) {
+#define ERR(KK)                                                                                       \
+  hipLaunchKernelGGL((errornorm_kernel<KK, G>), dim3(grid), dim3(256), 0, s, m, tables, lay, qu, uex_kind, \
+                     par3[0], par3[1], par3[2], part, g);
+  switch (k) {
+    case 1: ERR(1); break;
+    case 2: ERR(2); break;
+    case 3: ERR(3); break;
+    case 4: ERR(4); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+#undef ERR
+  return (int)hipGetLastError();
+}
+
+#define GROUP_INSTANTIATE(G)                                                                                  \
+  template int launch_cg_init_t<G>(const DevPattern&, const TraceSystem&, CgVectors&, G, int, hipStream_t);     \
+  template int launch_cg_spmv_t<G>(const DevPattern&, const TraceOp<double>&, const Krylov<double>&, G, int, int, \
+                                   double, int, int, hipStream_t);                                              \
+  template int launch_cg_update_t<G>(const TraceSystem&, CgVectors&, G, int, int, int, hipStream_t);             \
+  template int launch_true_residual_t<G>(const DevPattern&, const TraceSystem&, const double*, double*, G, int,  \
+                                         hipStream_t);                                                          \
+  template int launch_errornorm_t<G>(const DevMesh&, const double*, const TabLayoutDev&, int, const double*,    \
+                                     int, const double*, double*, G, int, hipStream_t);
+GROUP_INSTANTIATE(Whole)
+GROUP_INSTANTIATE(Grouped)
+#undef GROUP_INSTANTIATE
+
+int launch_reduce_segs(const double* part, const Seg* seg, int nseg, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_seg_kernel, dim3(nseg), dim3(256), 0, s, part, seg, out);
+  return (int)hipGetLastError();
+}
+
+int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s) {
+  return launch_cg_init_t(P, T, v, Whole{T.N, v.st}, v.nblocks, s);
+}
+
+int launch_cg_spmv(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, int i_fixed,
+                   double rtol2, int maxit, int force, hipStream_t s) {
+  return launch_cg_spmv_t(P, T, v, Whole{T.N, v.st}, v.nblocks, i_fixed, rtol2, maxit, force, s);
+}
+
+int launch_cg_update(const TraceSystem& T, CgVectors& v, int i_fixed, int force, hipStream_t s) {
+  return launch_cg_update_t(T, v, Whole{T.N, v.st}, v.nblocks, i_fixed, force, s);
 }
 
 int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s) {
@@ -947,11 +1038,8 @@ int launch_cg_finish(const TraceSystem& T, CgVectors& v, double* uhat, hipStream
 
 int launch_true_residual(const DevPattern& P, const TraceSystem& T, const double* uhat, double* out2,
                          CgVectors& v, hipStream_t s) {
-#define TR(NT)                                                                            \
-  hipLaunchKernelGGL(true_residual_kernel<NT>, dim3(v.nblocks), dim3(kCgThreads), 0, s, P, T, uhat, \
-                     v.part);
-  HDG_NT_DISPATCH(T.nt, TR)
-#undef TR
+  const int e = launch_true_residual_t(P, T, uhat, v.part, Whole{T.N, v.st}, v.nblocks, s);
+  if (e) return e;
   hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, v.part, v.nblocks, out2);
   return (int)hipGetLastError();
 }
@@ -978,20 +1066,13 @@ int launch_recover(int k, const DevMesh& m, const double* Ke, const double* be, 
 int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev& lay, int k,
                      const double* qu, int uex_kind, const double* par3, double* part, double* result,
                      hipStream_t s) {
-  const int g = grid_for(m.nc, 256, 2048);
-#define ERR(KK)                                                                                    \
-  hipLaunchKernelGGL(errornorm_kernel<KK>, dim3(g), dim3(256), 0, s, m, tables, lay, qu, uex_kind, \
-                     par3[0], par3[1], par3[2], part);
-  switch (k) {
-    case 1: ERR(1); break;
-    case 2: ERR(2); break;
-    case 3: ERR(3); break;
-    case 4: ERR(4); break;
-    default: return (int)hipErrorInvalidValue;
-  }
-#undef ERR
+  const int g = errornorm_blocks(m.nc);
+  const int e = launch_errornorm_t(m, tables, lay, k, qu, uex_kind, par3, part, Whole{m.nc, nullptr}, g, s);
+  if (e) return e;
   hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, part, g, result);
   return (int)hipGetLastError();
 }
+
+int errornorm_blocks(int64_t nc) { return grid_for(nc, 256, 2048); }
 
 }  // namespace hdg

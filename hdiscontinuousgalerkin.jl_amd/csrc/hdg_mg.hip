@@ -198,7 +198,7 @@ __global__ void mg_cell_coarse_kernel(DevMesh mesh, MgData M, TraceSystem T, con
 }
 
 // Level-1 stencils gathered from the (up to) six cells around each vertex.
-__global__ void mg_assemble_l1_kernel(DevMesh mesh, MgData M) {
+__global__ void mg_assemble_l1_kernel(MgData M) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const MgLevel L = M.lev[1];
   if (v >= L.n) return;
@@ -407,12 +407,14 @@ __device__ inline double hash01(uint64_t i) {
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
-template <int NT, class V>
-__global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceOp<V> T, V* x, double* part) {
+template <int NT, class V, class G>
+__global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceOp<V> T, V* x, double* part, G gsel) {
   __shared__ double sh[NT];
+  const Seg S = seg_of(gsel);
+  const int lb = blockIdx.x - S.b0;
   double s2 = 0.0;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x) {
-    const V v = (V)(T.dmask[r] ? 0.0 : 0.5 + hash01((uint64_t)r));
+  for (int64_t r = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
+    const V v = (V)(T.dmask[r] ? 0.0 : 0.5 + hash01((uint64_t)(r - S.lo)));
     x[r] = v;
     s2 = fma((double)v, (double)v, s2);
   }
@@ -421,24 +423,26 @@ __global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceOp<V> T, V* x, d
 }
 
 // y = Dinv S (x / |x|), |x|^2 from the previous step's partials.
-template <int NT, class V>
+template <int NT, class V, class G>
 __global__ __launch_bounds__(64 * NT) void pow_step_kernel(DevPattern P, TraceOp<V> T, const V* x, V* y,
-                                                           const double* part_in, double* part_out, int nb) {
+                                                           const double* part_in, double* part_out, G gsel) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
-  const double nx = sqrt(sum_partials_mg<NT>(part_in, nb, sh));
+  const Seg S = seg_of(gsel);
+  const int lb = blockIdx.x - S.b0;
+  const double nx = sqrt(sum_partials_mg<NT>(part_in + S.b0, S.nb, sh));
   const double inv = nx > 0.0 ? 1.0 / nx : 0.0;
   double s2 = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
-    rs[threadIdx.x] = r < T.N ? inv * s_row<NT, V>(P, T, x, r) : 0.0;
+    rs[threadIdx.x] = r < S.hi ? inv * s_row<NT, V>(P, T, x, r) : 0.0;
     __syncthreads();
-    if (r < T.N) {
-      const int lb = (threadIdx.x / NT) * NT;
+    if (r < S.hi) {
+      const int lr = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
       const V zv = (V)z;
       y[r] = zv;
       s2 = fma((double)zv, (double)zv, s2);
@@ -455,24 +459,34 @@ __global__ __launch_bounds__(256) void pow_final_kernel(const double* part, int 
   if (threadIdx.x == 0) st->omega = lam > 0.0 ? cfac / lam : 0.5;
 }
 
+// one block per instance of a group
+__global__ __launch_bounds__(256) void pow_final_seg_kernel(const double* part, const Seg* seg, double cfac) {
+  __shared__ double sh[4];
+  const Seg S = seg[blockIdx.x];
+  const double lam = sqrt(sum_partials_mg<4>(part + S.b0, S.nb, sh));
+  if (threadIdx.x == 0) S.st->omega = lam > 0.0 ? cfac / lam : 0.5;
+}
+
 // ------------------------------------------------------- level 0 (trace)
 // update(i) with the multigrid preconditioner: x += alpha p; r -= alpha q;
 // e0 = omega Dinv r (first smoothing step, preconditioner storage W);
 // partial (r, r).
-template <int NT, class W>
-__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov<double> v, MgVecs<W> mv) {
+template <int NT, class W, class G>
+__global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov<double> v, MgVecs<W> mv, G gsel) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
-  CgState* st = v.st;
+  const Seg S = seg_of(gsel);
+  CgState* st = S.st;
   if (st->done) return;
   const int i = st->i_upd;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->i_pre = i;
-  const int nb = v.nblocks;
-  const double pq = sum_partials_mg<NT>(v.part, nb, sh);
+  const int lb = blockIdx.x - S.b0;
+  if (lb == 0 && threadIdx.x == 0) st->i_pre = i;
+  const size_t nbt = (size_t)v.nblocks;
+  const double pq = sum_partials_mg<NT>(v.part + S.b0, S.nb, sh);
   const double rz = st->rz[i & 1];
   if (!(pq > 0.0)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lb == 0 && threadIdx.x == 0) {
       st->done = 3;
       st->it = i - 1;
     }
@@ -481,10 +495,10 @@ __global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov
   const double alpha = rz / pq, om = st->omega;
   const double* __restrict__ p = (i & 1) ? v.p1 : v.p0;
   double prr = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
     double rn = 0.0;
-    if (r < T.N) {
+    if (r < S.hi) {
       v.x[r] = fma(alpha, p[r], v.x[r]);
       rn = fma(-alpha, v.q[r], v.r[r]);
       v.r[r] = rn;
@@ -492,33 +506,35 @@ __global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov
     }
     rs[threadIdx.x] = rn;
     __syncthreads();
-    if (r < T.N) {
-      const int lb = (threadIdx.x / NT) * NT;
+    if (r < S.hi) {
+      const int lr = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
       mv.e0[r] = (W)(om * z);
     }
     __syncthreads();
   }
   const double b = block_sum_mg<NT>(prr, sh);
-  if (threadIdx.x == 0) v.part[(size_t)nb * 3 + blockIdx.x] = b;
+  if (threadIdx.x == 0) v.part[nbt * 3 + blockIdx.x] = b;
 }
 
 // init (update(0)) for the multigrid path: r0 = b~ = -(rhs - K uD), x = p = 0,
 // e0 = omega Dinv r0.
-template <int NT, class W>
+template <int NT, class W, class G>
 __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSystem T64, TraceOp<W> T,
-                                                          Krylov<double> v, MgVecs<W> mv) {
+                                                          Krylov<double> v, MgVecs<W> mv, G gsel) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
-  const double om = v.st->omega;
+  const Seg S = seg_of(gsel);
+  const int lb = blockIdx.x - S.b0;
+  const double om = S.st->omega;
   double prr = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
     double b = 0.0;
-    if (r < T.N) {
+    if (r < S.hi) {
       if (!T64.dmask[r]) {
         const int64_t f = r / NT;
         double s = T64.rhs[r];
@@ -538,30 +554,31 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
     }
     rs[threadIdx.x] = b;
     __syncthreads();
-    if (r < T.N) {
-      const int lb = (threadIdx.x / NT) * NT;
+    if (r < S.hi) {
+      const int lr = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int c = 0; c < NT; ++c) z = fma((double)T.Dinv[(int64_t)c * T.N + r], rs[lb + c], z);
+      for (int c = 0; c < NT; ++c) z = fma((double)T.Dinv[(int64_t)c * T.N + r], rs[lr + c], z);
       mv.e0[r] = (W)(om * z);
     }
     __syncthreads();
   }
   const double bb = block_sum_mg<NT>(prr, sh);
   if (threadIdx.x == 0) v.part[(size_t)v.nblocks * 3 + blockIdx.x] = bb;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    v.st->done = 0;
-    v.st->it = 0;
-    v.st->pq = 0.0;
-    v.st->i_pre = 0;  // the first smoother call completes update(0)
+  if (lb == 0 && threadIdx.x == 0) {
+    S.st->done = 0;
+    S.st->it = 0;
+    S.st->pq = 0.0;
+    S.st->i_pre = 0;  // the first smoother call completes update(0)
   }
 }
 
-// res0 = r - S e0.
+// res0 = r - S e0 over all rows (of every instance when packed; v.st null:
+// no convergence check).
 template <int NT, class W>
 __global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceOp<W> T, Krylov<double> v,
                                                         MgVecs<W> mv) {
-  if (v.st->done) return;
+  if (v.st && v.st->done) return;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x)
     mv.res0[r] = T.dmask[r] ? (W)0 : (W)(v.r[r] - s_row<NT, W>(P, T, mv.e0, r));
 }
@@ -569,7 +586,7 @@ __global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceOp<W>
 // r1 = P0^T res0, vertex by vertex over the incident faces.
 template <int NT, class W>
 __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgData M, MgVecs<W> mv) {
-  if (v.st->done) return;
+  if (v.st && v.st->done) return;
   const MgLevel L = M.lev[1];
   const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (vv >= L.n) return;
@@ -586,22 +603,25 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgD
 // z = e0 + P0 e1 + omega Dinv (res0 - (S P0) e1): the post-smoothing step.
 // S (e0 + P0 e1) = S e0 + (S P0) e1 and res0 = r - S e0, so the trace matrix
 // is not read again; partial (r, z) for the next iteration.
-template <int NT, class W>
+template <int NT, class W, class G>
 __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylov<double> v, MgData M,
-                                                             MgVecs<W> mv) {
+                                                             MgVecs<W> mv, G gsel) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
   __shared__ double sh[NT];
-  if (v.st->done) return;
-  const int i = v.st->i_pre;
-  if (blockIdx.x == 0 && threadIdx.x == 0) v.st->i_spmv = i + 1;
-  const double om = v.st->omega;
+  const Seg S = seg_of(gsel);
+  CgState* st = S.st;
+  if (st->done) return;
+  const int i = st->i_pre;
+  const int lb = blockIdx.x - S.b0;
+  if (lb == 0 && threadIdx.x == 0) st->i_spmv = i + 1;
+  const double om = st->omega;
   const double* __restrict__ e1 = M.lev[1].e;
   double prz = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * NTH; r0 < T.N; r0 += (int64_t)gridDim.x * NTH) {
+  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
     double rr = 0.0, pe = 0.0;
-    if (r < T.N && !T.dmask[r]) {
+    if (r < S.hi && !T.dmask[r]) {
       const int64_t f = r / NT;
       double s = (double)mv.res0[r];
 #pragma unroll
@@ -615,11 +635,11 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
     }
     rs[threadIdx.x] = rr;
     __syncthreads();
-    if (r < T.N) {
-      const int lb = (threadIdx.x / NT) * NT;
+    if (r < S.hi) {
+      const int lr = (threadIdx.x / NT) * NT;
       double z = 0.0;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lb + b], z);
+      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
       const double zv = fma(om, z, (double)mv.e0[r] + pe);
       v.z[r] = zv;
       prz = fma(v.r[r], zv, prz);
@@ -650,26 +670,37 @@ __device__ inline double stencil_apply(const MgLevel& L, int64_t v, int i, int j
   return s;
 }
 
+// The vertex-level kernels take a selector (VOne, VGroup) that names the
+// instance of the block: its fine level F, coarse level C and CG state.
+
 // two pre-smoothing sweeps from zero in one pass: e1 = om dinv r is formed at
 // the neighbours on the fly, e = e1 + om dinv (r - A e1)
-__global__ __launch_bounds__(256) void v_pre_kernel(MgLevel L, double om, const CgState* st) {
-  if (st->done) return;
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <class VS>
+__global__ __launch_bounds__(256) void v_pre_kernel(VS sel, double om) {
+  int lb;
+  const VRec& R = vrec_of(sel, lb);
+  if (R.st->done) return;
+  const MgLevel& L = R.F;
+  const int64_t v = (int64_t)lb * blockDim.x + threadIdx.x;
   if (v >= L.n) return;
   const int i = (int)(v % (L.nx + 1)), j = (int)(v / (L.nx + 1));
   auto e1 = [&](int64_t u, int, int) { return om * L.dinv[u] * L.r[u]; };
   L.e[v] = e1(v, i, j) + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, e1));
 }
 
-// out = in + om dinv (r - A in)
-__global__ __launch_bounds__(256) void v_smooth_kernel(MgLevel L, const double* in, double* out, double om,
-                                                       const CgState* st) {
-  if (st->done) return;
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// second post-smoothing sweep: e = t + om dinv (r - A t)
+template <class VS>
+__global__ __launch_bounds__(256) void v_smooth_kernel(VS sel, double om) {
+  int lb;
+  const VRec& R = vrec_of(sel, lb);
+  if (R.st->done) return;
+  const MgLevel& L = R.F;
+  const int64_t v = (int64_t)lb * blockDim.x + threadIdx.x;
   if (v >= L.n) return;
   const int i = (int)(v % (L.nx + 1)), j = (int)(v / (L.nx + 1));
+  const double* __restrict__ in = L.t;
   auto xv = [&](int64_t u, int, int) { return in[u]; };
-  out[v] = in[v] + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, xv));
+  L.e[v] = in[v] + om * L.dinv[v] * (L.r[v] - stencil_apply(L, v, i, j, xv));
 }
 
 // fine residual r - A e at (fx, fy); 0 outside the grid and on fixed nodes.
@@ -691,10 +722,14 @@ __device__ inline double fine_res(const MgLevel& F, int fx, int fy, bool one_swe
 // neighbours, weighted as prolong_at weighs C there.  Eight lanes per
 // coarse node: lane t < 7 forms the fine residual at stencil offset t, and a
 // fixed shuffle tree sums them.
-__global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, int one_sweep, double om,
-                                                         const CgState* st) {
-  if (st->done) return;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <class VS>
+__global__ __launch_bounds__(256) void v_restrict_kernel(VS sel, int one_sweep, double om) {
+  int lb;
+  const VRec& R = vrec_of(sel, lb);
+  if (R.st->done) return;
+  const MgLevel& F = R.F;
+  const MgLevel& Cc = R.C;
+  const int64_t g = (int64_t)lb * blockDim.x + threadIdx.x;
   const int64_t C = g >> 3;
   const int t = (int)(g & 7);
   double val = 0.0;
@@ -715,10 +750,14 @@ __global__ __launch_bounds__(256) void v_restrict_kernel(MgLevel F, MgLevel Cc, 
 // e' = e + P ec on free nodes (formed at the neighbours on the fly),
 // out = e' + om dinv (r - A e').  With one_sweep the pre-smoothed iterate is
 // e = om dinv r (formed on the fly) and out is the level's final e.
-__global__ __launch_bounds__(256) void v_post_kernel(MgLevel F, MgLevel Cc, int one_sweep, double om,
-                                                     const CgState* st) {
-  if (st->done) return;
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <class VS>
+__global__ __launch_bounds__(256) void v_post_kernel(VS sel, int one_sweep, double om) {
+  int lb;
+  const VRec& R = vrec_of(sel, lb);
+  if (R.st->done) return;
+  const MgLevel& F = R.F;
+  const MgLevel& Cc = R.C;
+  const int64_t v = (int64_t)lb * blockDim.x + threadIdx.x;
   if (v >= F.n) return;
   const int i = (int)(v % (F.nx + 1)), j = (int)(v / (F.nx + 1));
   if (one_sweep) {
@@ -785,14 +824,18 @@ __device__ inline void tail_sweep(const TailLevel& L, const double* in, double* 
   }
 }
 
-// Levels lt..nlev-1 of the V-cycle in one workgroup (their sizes sum to at
-// most kTailNodes), everything in LDS: reads lev[lt].r, writes lev[lt].e.
+// Levels lt = tail .. nlev-1 of the V-cycle in one workgroup (their sizes
+// sum to at most kTailNodes), everything in LDS: reads lev[lt].r, writes
+// lev[lt].e.  Block b serves the hierarchy Ms[b] with CG state sts[b] (one
+// block for a single problem, one per instance for a group).
 // Same sweeps as the per-level kernels: two pre-smoothing sweeps,
 // restriction of the residual, coarse_sweeps sweeps on the coarsest level,
 // prolongation and two post-smoothing sweeps.
-__global__ __launch_bounds__(1024) void v_tail_kernel(MgData M, int lt, const CgState* st) {
+__global__ __launch_bounds__(1024) void v_tail_kernel(const MgData* __restrict__ Ms, const CgState* sts) {
   __shared__ double pool[(kStencil + 4) * kTailNodes];
-  if (st->done) return;
+  const MgData& M = Ms[blockIdx.x];
+  if (sts[blockIdx.x].done) return;
+  const int lt = M.tail;
   const double om = M.omega_v;
   double* const E = pool;                    // scratch / residual
   double* const Tt = pool + kTailNodes;      // iterate
@@ -932,10 +975,11 @@ int grid1(int64_t n, int t = 256) {
     default: return (int)hipErrorInvalidValue; \
   }
 
-// V-cycle from level l down; the result is left in lev[l].e.
-int vcycle(const MgData& M, int l, const CgState* st, hipStream_t s) {
+// V-cycle from level l down; the result is left in lev[l].e.  dM is the
+// device copy of M that the tail kernel reads.
+int vcycle(const MgData& M, const MgData* dM, int l, const CgState* st, hipStream_t s) {
   if (l >= M.tail) {
-    hipLaunchKernelGGL(v_tail_kernel, dim3(1), dim3(1024), 0, s, M, l, st);
+    hipLaunchKernelGGL(v_tail_kernel, dim3(1), dim3(1024), 0, s, dM, st);
     return (int)hipGetLastError();
   }
   const MgLevel& L = M.lev[l];
@@ -945,29 +989,52 @@ int vcycle(const MgData& M, int l, const CgState* st, hipStream_t s) {
     return (int)hipGetLastError();
   }
   const MgLevel& Cc = M.lev[l + 1];
+  const VOne one_rec{VRec{L, Cc, st}};
   const int g = grid1(L.n);
   const int one = M.vsweeps > 1 ? 0 : 1;
-  if (!one) hipLaunchKernelGGL(v_pre_kernel, dim3(g), dim3(256), 0, s, L, om, st);
-  hipLaunchKernelGGL(v_restrict_kernel, dim3(grid1(8 * Cc.n)), dim3(256), 0, s, L, Cc, one, om, st);
-  int e = vcycle(M, l + 1, st, s);
+  if (!one) hipLaunchKernelGGL(v_pre_kernel<VOne>, dim3(g), dim3(256), 0, s, one_rec, om);
+  hipLaunchKernelGGL(v_restrict_kernel<VOne>, dim3(grid1(8 * Cc.n)), dim3(256), 0, s, one_rec, one, om);
+  int e = vcycle(M, dM, l + 1, st, s);
   if (e) return e;
-  hipLaunchKernelGGL(v_post_kernel, dim3(g), dim3(256), 0, s, L, Cc, one, om, st);
-  if (!one) hipLaunchKernelGGL(v_smooth_kernel, dim3(g), dim3(256), 0, s, L, (const double*)L.t, L.e, om, st);
+  hipLaunchKernelGGL(v_post_kernel<VOne>, dim3(g), dim3(256), 0, s, one_rec, one, om);
+  if (!one) hipLaunchKernelGGL(v_smooth_kernel<VOne>, dim3(g), dim3(256), 0, s, one_rec, om);
+  return (int)hipGetLastError();
+}
+
+// The same V-cycle for every instance of a group: depth by depth down to the
+// tails, every tail in one launch (one workgroup each), depth by depth up.
+// An instance whose tail starts at depth t takes part in depths 1..t-1 only.
+int vcycle_group(const MgGroup& G, hipStream_t s) {
+  const int one = G.vsweeps > 1 ? 0 : 1;
+  const double om = G.omega_v;
+  for (int d = 1; d < G.ndepth; ++d) {
+    if (!one) hipLaunchKernelGGL(v_pre_kernel<VGroup>, dim3(G.gnode[d]), dim3(256), 0, s, G.node[d], om);
+    hipLaunchKernelGGL(v_restrict_kernel<VGroup>, dim3(G.gres[d]), dim3(256), 0, s, G.res[d], one, om);
+  }
+  hipLaunchKernelGGL(v_tail_kernel, dim3(G.ninst), dim3(1024), 0, s, G.dM, (const CgState*)G.st);
+  for (int d = G.ndepth - 1; d >= 1; --d) {
+    hipLaunchKernelGGL(v_post_kernel<VGroup>, dim3(G.gnode[d]), dim3(256), 0, s, G.node[d], one, om);
+    if (!one) hipLaunchKernelGGL(v_smooth_kernel<VGroup>, dim3(G.gnode[d]), dim3(256), 0, s, G.node[d], om);
+  }
   return (int)hipGetLastError();
 }
 
 }  // namespace
 
-int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
-                     float* sp32, hipStream_t s) {
+int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
+                  const double* Ate, float* sp32, hipStream_t s) {
   hipLaunchKernelGGL(mg_vmask_kernel, dim3(grid1(M.nv)), dim3(256), 0, s, M, T);
 #define CELL(NT) hipLaunchKernelGGL(mg_cell_coarse_kernel<NT>, dim3(grid1(mesh.nc)), dim3(256), 0, s, mesh, M, T, Ate);
   MG_NT_DISPATCH(T.nt, CELL)
 #undef CELL
-  hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, mesh, M);
 #define SPK(NT) hipLaunchKernelGGL(mg_sp_kernel<NT>, dim3(grid1(T.N)), dim3(256), 0, s, P, T, M, sp32);
   MG_NT_DISPATCH(T.nt, SPK)
 #undef SPK
+  return (int)hipGetLastError();
+}
+
+int mg_setup_levels(const MgData& M, hipStream_t s) {
+  hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, M);
   for (int l = 1; l + 1 < M.nlev; ++l) {
     hipLaunchKernelGGL(mg_cmask_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
     hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(8 * M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
@@ -978,50 +1045,101 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPa
   return (int)hipGetLastError();
 }
 
+int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
+                     float* sp32, hipStream_t s) {
+  const int e = mg_setup_mesh(mesh, M, T, P, Ate, sp32, s);
+  if (e) return e;
+  return mg_setup_levels(M, s);
+}
+
 // omega = factor / lambda_max(Dinv S) by power steps on free dofs; xa, xb
 // are scratch vectors of N entries, part holds 2 nb partials.
-template <class V>
-int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
-                    const MgData& M, int steps, hipStream_t s) {
-  double* pa = part;
-  double* pb = part + nb;
-#define PINIT(NT) hipLaunchKernelGGL((pow_init_kernel<NT, V>), dim3(nb), dim3(64 * NT), 0, s, T, xa, pa);
+template <class V, class G>
+int mg_power_steps(const DevPattern& P, const TraceOp<V>& T, V*& xa, V*& xb, double*& pa, double*& pb, G g,
+                   int grid, int steps, hipStream_t s) {
+#define PINIT(NT) hipLaunchKernelGGL((pow_init_kernel<NT, V, G>), dim3(grid), dim3(64 * NT), 0, s, T, xa, pa, g);
   MG_NT_DISPATCH(T.nt, PINIT)
 #undef PINIT
   for (int k = 0; k < steps; ++k) {
-#define PSTEP(NT) \
-    hipLaunchKernelGGL((pow_step_kernel<NT, V>), dim3(nb), dim3(64 * NT), 0, s, P, T, (const V*)xa, xb, pa, pb, nb);
+#define PSTEP(NT)                                                                                              \
+    hipLaunchKernelGGL((pow_step_kernel<NT, V, G>), dim3(grid), dim3(64 * NT), 0, s, P, T, (const V*)xa, xb, pa, \
+                       pb, g);
     MG_NT_DISPATCH(T.nt, PSTEP)
 #undef PSTEP
     std::swap(xa, xb);
     std::swap(pa, pb);
   }
+  return (int)hipGetLastError();
+}
+
+template <class V>
+int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
+                    const MgData& M, int steps, hipStream_t s) {
+  double* pa = part;
+  double* pb = part + nb;
+  const int e = mg_power_steps(P, T, xa, xb, pa, pb, Whole{T.N, st}, nb, steps, s);
+  if (e) return e;
   hipLaunchKernelGGL(pow_final_kernel, dim3(1), dim3(256), 0, s, pa, nb, st, M.omega_factor);
+  return (int)hipGetLastError();
+}
+
+template <class V>
+int launch_mg_power_group(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, Grouped g, int grid,
+                          const Seg* seg, int ninst, double factor, int steps, hipStream_t s) {
+  double* pa = part;
+  double* pb = part + grid;
+  const int e = mg_power_steps(P, T, xa, xb, pa, pb, g, grid, steps, s);
+  if (e) return e;
+  hipLaunchKernelGGL(pow_final_seg_kernel, dim3(ninst), dim3(256), 0, s, pa, seg, factor);
+  return (int)hipGetLastError();
+}
+
+template <class W, class G>
+int mg_init_t(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
+              const MgVecs<W>& mv, G g, int grid, hipStream_t s) {
+#define INIT(NT) \
+  hipLaunchKernelGGL((mg_init_kernel<NT, W, G>), dim3(grid), dim3(64 * NT), 0, s, P, T64, T, v, mv, g);
+  MG_NT_DISPATCH(T.nt, INIT)
+#undef INIT
+  return (int)hipGetLastError();
+}
+
+template <class W, class G>
+int mg_update_t(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, G g, int grid, hipStream_t s) {
+#define UPD(NT) hipLaunchKernelGGL((mg_update_kernel<NT, W, G>), dim3(grid), dim3(64 * NT), 0, s, T, v, mv, g);
+  MG_NT_DISPATCH(T.nt, UPD)
+#undef UPD
   return (int)hipGetLastError();
 }
 
 template <class W>
 int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
                    const MgVecs<W>& mv, hipStream_t s) {
-#define INIT(NT) hipLaunchKernelGGL((mg_init_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, P, T64, T, v, mv);
-  MG_NT_DISPATCH(T.nt, INIT)
-#undef INIT
-  return (int)hipGetLastError();
+  return mg_init_t(P, T64, T, v, mv, Whole{T.N, v.st}, v.nblocks, s);
+}
+
+template <class W>
+int launch_mg_init_group(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
+                         const MgVecs<W>& mv, Grouped g, int grid, hipStream_t s) {
+  return mg_init_t(P, T64, T, v, mv, g, grid, s);
 }
 
 template <class W>
 int launch_mg_update(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, hipStream_t s) {
-#define UPD(NT) hipLaunchKernelGGL((mg_update_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, mv);
-  MG_NT_DISPATCH(T.nt, UPD)
-#undef UPD
-  return (int)hipGetLastError();
+  return mg_update_t(T, v, mv, Whole{T.N, v.st}, v.nblocks, s);
+}
+
+template <class W>
+int launch_mg_update_group(const TraceOp<W>& T, const Krylov<double>& v, const MgVecs<W>& mv, Grouped g, int grid,
+                           hipStream_t s) {
+  return mg_update_t(T, v, mv, g, grid, s);
 }
 
 // z = V-cycle(r), starting from e0 = omega Dinv r already in mv.e0; writes
 // the (r, z) partials of the next iteration.
 template <class W>
 int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
-                      const MgVecs<W>& mv, hipStream_t s) {
+                      const MgData* dM, const MgVecs<W>& mv, hipStream_t s) {
   const int gr = grid1(T.N, 256);
   const int grows = gr > 4096 ? 4096 : gr;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv);
@@ -1031,10 +1149,37 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
   hipLaunchKernelGGL((mg_restrict0_kernel<NT, W>), dim3(grid1(M.lev[1].n)), dim3(256), 0, s, v, M, mv);
   MG_NT_DISPATCH(T.nt, RST)
 #undef RST
-  int e = vcycle(M, 1, v.st, s);
+  int e = vcycle(M, dM, 1, v.st, s);
   if (e) return e;
 #define SMO(NT) \
-  hipLaunchKernelGGL((mg_smooth0_kernel<NT, W>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, mv);
+  hipLaunchKernelGGL((mg_smooth0_kernel<NT, W, Whole>), dim3(v.nblocks), dim3(64 * NT), 0, s, T, v, M, mv, \
+                     Whole{T.N, v.st});
+  MG_NT_DISPATCH(T.nt, SMO)
+#undef SMO
+  return (int)hipGetLastError();
+}
+
+// Grouped: the residual and the restriction to level 1 run over the packed
+// mesh (every instance, no convergence check), the vertex levels through
+// the group's tables, the smoother per instance.
+template <class W>
+int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& Mu,
+                            const MgGroup& G, const MgVecs<W>& mv, Grouped g, int grid, hipStream_t s) {
+  Krylov<double> vu = v;
+  vu.st = nullptr;
+  const int gr = grid1(T.N, 256);
+  const int grows = gr > 8192 ? 8192 : gr;
+#define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, vu, mv);
+  MG_NT_DISPATCH(T.nt, RES)
+#undef RES
+#define RST(NT) \
+  hipLaunchKernelGGL((mg_restrict0_kernel<NT, W>), dim3(grid1(Mu.lev[1].n)), dim3(256), 0, s, vu, Mu, mv);
+  MG_NT_DISPATCH(T.nt, RST)
+#undef RST
+  int e = vcycle_group(G, s);
+  if (e) return e;
+#define SMO(NT) \
+  hipLaunchKernelGGL((mg_smooth0_kernel<NT, W, Grouped>), dim3(grid), dim3(64 * NT), 0, s, T, v, Mu, mv, g);
   MG_NT_DISPATCH(T.nt, SMO)
 #undef SMO
   return (int)hipGetLastError();
@@ -1043,11 +1188,20 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
 #define MG_INSTANTIATE(W)                                                                                      \
   template int launch_mg_power<W>(const DevPattern&, const TraceOp<W>&, W*, W*, double*, int, CgState*,          \
                                   const MgData&, int, hipStream_t);                                             \
+  template int launch_mg_power_group<W>(const DevPattern&, const TraceOp<W>&, W*, W*, double*, Grouped, int,     \
+                                        const Seg*, int, double, int, hipStream_t);                              \
   template int launch_mg_init<W>(const DevPattern&, const TraceSystem&, const TraceOp<W>&, const Krylov<double>&, \
                                  const MgVecs<W>&, hipStream_t);                                                \
+  template int launch_mg_init_group<W>(const DevPattern&, const TraceSystem&, const TraceOp<W>&,                 \
+                                       const Krylov<double>&, const MgVecs<W>&, Grouped, int, hipStream_t);      \
   template int launch_mg_update<W>(const TraceOp<W>&, const Krylov<double>&, const MgVecs<W>&, hipStream_t);   \
+  template int launch_mg_update_group<W>(const TraceOp<W>&, const Krylov<double>&, const MgVecs<W>&, Grouped,  \
+                                         int, hipStream_t);                                                     \
   template int launch_mg_precond<W>(const DevPattern&, const TraceOp<W>&, const Krylov<double>&, const MgData&,  \
-                                    const MgVecs<W>&, hipStream_t);
+                                    const MgData*, const MgVecs<W>&, hipStream_t);                              \
+  template int launch_mg_precond_group<W>(const DevPattern&, const TraceOp<W>&, const Krylov<double>&,           \
+                                          const MgData&, const MgGroup&, const MgVecs<W>&, Grouped, int,        \
+                                          hipStream_t);
 MG_INSTANTIATE(double)
 MG_INSTANTIATE(float)
 #undef MG_INSTANTIATE

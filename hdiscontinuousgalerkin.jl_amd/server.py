@@ -1,10 +1,15 @@
-"""Python handle on libhdg's serving loop (hdg_server_* in include/hdg.h).
+"""Python handles on libhdg's serving loop (hdg_server_* in include/hdg.h)
+and batches (hdg_batch_*).
 
     srv = Server(device=0, hbm_budget=8 << 30, slots=8)
     rid = srv.submit(nx=256, ny=256, k=2)
     res = srv.wait(rid)            # hdg_result as a dict
     srv.stats()                    # p50/p99 latency, cells/s
     srv.close()
+
+    with Batch([make_request(...), ...]) as b:
+        results = b.run()          # one dict per request, grouped launches
+        uhat = b.trace(0)
 """
 from __future__ import annotations
 
@@ -83,6 +88,71 @@ class Server:
     def close(self):
         if self._h:
             lib().hdg_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Batch:
+    """Many requests solved together: requests of one order and one set of
+    options are packed into one mesh and share every launch (hdg_batch_*)."""
+
+    def __init__(self, reqs, device=0, hbm_budget=0):
+        reqs = list(reqs)
+        arr = (L.Request * len(reqs))(*reqs)
+        h = C.c_void_p()
+        code = lib().hdg_batch_create(C.byref(h), int(device), int(hbm_budget), len(reqs), arr)
+        self._h = h if h.value else None
+        self._n = len(reqs)
+        self._ndofs = [None] * len(reqs)
+        if code != L.HDG_OK:
+            msg = ""
+            if self._h:
+                raw = lib().hdg_batch_last_error(self._h)
+                msg = raw.decode() if raw else ""
+                self.close()
+            if code in (1, 2, 3, 10):
+                raise L.ArgumentError(code, msg)
+            raise L.HDGError(code, msg)
+
+    def _chk(self, code, ok=(L.HDG_OK,)):
+        if code not in ok:
+            raw = lib().hdg_batch_last_error(self._h)
+            raise L.HDGError(code, raw.decode() if raw else "")
+        return code
+
+    def run(self, check=True):
+        """Runs every instance; returns one dict per request.  With check,
+        raises when an instance failed (each dict holds its own status)."""
+        res = (L.Result * self._n)()
+        code = lib().hdg_batch_run(self._h, res)
+        if check:
+            self._chk(code)
+        elif code == 7:  # HDG_ERR_HIP
+            self._chk(code)
+        out = [{f: getattr(r, f) for f, _ in L.Result._fields_} for r in res]
+        self._ndofs = [r["n_dofs"] for r in out]
+        return out
+
+    def trace(self, i):
+        if self._ndofs[i] is None:
+            raise L.HDGError(8, "run the batch first")
+        out = np.zeros(int(self._ndofs[i]))
+        self._chk(lib().hdg_batch_trace(self._h, int(i), out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def info(self):
+        ng, nb = C.c_int(), C.c_size_t()
+        self._chk(lib().hdg_batch_info(self._h, C.byref(ng), C.byref(nb)))
+        return {"groups": int(ng.value), "device_bytes": int(nb.value)}
+
+    def close(self):
+        if self._h:
+            lib().hdg_batch_destroy(self._h)
             self._h = None
 
     def __enter__(self):

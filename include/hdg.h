@@ -282,6 +282,40 @@ int hdg_server_stats(hdg_server* srv, hdg_server_summary* out);
 void hdg_server_destroy(hdg_server* srv);
 const char* hdg_server_last_error(const hdg_server* srv);
 
+/* ---- batches: many instances in grouped launches ------------------------- */
+/* No counterpart in the reference (one problem per script run).  A batch
+   holds n independent requests (same fields as the serving loop's; u = 0 on
+   the boundary, f and kappa from src, HDG_SRC_VALUES not accepted).
+   Requests of the same order and options form a group: their rectangle
+   meshes are packed into one mesh (a disjoint union), so element build,
+   condensation, assembly, recovery and every elementwise kernel of the
+   solve run once per group, and the PCG scalars, reductions and multigrid
+   vertex levels of the instances are selected per block from launch tables.
+   One launch per kernel serves the whole group; groups of different orders
+   run side by side on their own streams.  Each instance's results are
+   bitwise those of solving it alone with hdg_run (same reductions over the
+   same block partition).  Meshes, operators and solutions stay resident
+   between runs. */
+typedef struct hdg_batch hdg_batch;
+
+/* Builds and uploads the packed meshes of every group and allocates every
+   device array (hbm_budget 0 = no cap; HDG_ERR_HBM_BUDGET when the batch
+   does not fit).  reqs is copied.  On failure *out may still receive the
+   batch, so that hdg_batch_last_error explains it; destroy it then. */
+int hdg_batch_create(hdg_batch** out, int device, size_t hbm_budget, int n, const hdg_request* reqs);
+/* Element build + condensation + assembly + trace solve + recovery (and the
+   error norm where uex_kind >= 0) of every instance; res[i] for request i
+   (id = i, slot = its group, times in seconds since the start of this
+   call).  Returns HDG_OK when every instance converged, else the first
+   failing status (each res[i].status holds its own). */
+int hdg_batch_run(hdg_batch* b, hdg_result* res);
+/* Trace solution of request i after hdg_batch_run (res[i].n_dofs doubles). */
+int hdg_batch_trace(hdg_batch* b, int i, double* uhat);
+/* Number of groups and device bytes held. */
+int hdg_batch_info(const hdg_batch* b, int* ngroups, size_t* device_bytes);
+void hdg_batch_destroy(hdg_batch* b);
+const char* hdg_batch_last_error(const hdg_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
