@@ -477,13 +477,13 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     const int64_t N = c->T.N;
     LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
     LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, c->f_sp, c->stream));
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, c->f_sp, c->stream));
     LAUNCHCHK(c, launch_mg_power(c->pat, op32(c), c->f_xa, c->f_xb, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
                                  10, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
     LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
   } else if (use_mg(c)) {
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
     // the CG vectors x, r serve as scratch before the solve initialises them
     LAUNCHCHK(c, launch_mg_power(c->pat, op64(c), c->cg.x, c->cg.r, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
                                  10, c->stream));

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -82,6 +83,7 @@ struct Group {
   MgData* dMi = nullptr;
   LevelBuf lev[kMaxLevels];
   MgGroup G;
+  MgSetupGroup S;
   float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr,
         *f_xa = nullptr, *f_xb = nullptr;
   // read-back
@@ -89,8 +91,6 @@ struct Group {
   double* h_res = nullptr;  // pinned, 2 x ninst, then the error flag (an int)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
-  int slot = 0;
-  bool active = false;
   double t_done = 0.0;
 };
 
@@ -103,6 +103,7 @@ struct hdg_batch {
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> where;  // request -> (group, instance)
   bool ran = false;
+  std::mutex mu;  // guards err (groups run in host threads of their own)
   std::string err;
 };
 
@@ -114,7 +115,10 @@ int fail(hdg_batch* b, int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
-  if (b) b->err = buf;
+  if (b) {
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->err = buf;
+  }
   return code;
 }
 
@@ -504,6 +508,42 @@ int build_group(hdg_batch* b, Group& g) {
     G.gnode[d] = (int)node.size();
     G.gres[d] = (int)res.size();
   }
+  // set-up tables: level-1 assembly, then cmask / RAP for every level pair
+  MgSetupGroup& S = g.S;
+  S.ninst = n;
+  S.dM = g.dMi;
+  {
+    std::vector<BlockMap> l1;
+    for (int j = 0; j < n; ++j)
+      for (int64_t u = 0; u < (g.Mi[j].lev[1].n + 255) / 256; ++u) l1.push_back(BlockMap{j, (int)u});
+    BlockMap* dl1;
+    BTRY(upload(b, g, &dl1, l1));
+    S.l1map = dl1;
+    S.gl1 = (int)l1.size();
+  }
+  S.ndepth = maxlev - 1;
+  for (int d = 1; d < S.ndepth; ++d) {
+    std::vector<VRec> rec;
+    std::vector<BlockMap> cm, rp;
+    for (int j = 0; j < n; ++j) {
+      const MgData& M = g.Mi[j];
+      if (d + 1 >= M.nlev) continue;
+      const int p = (int)rec.size();
+      rec.push_back(VRec{M.lev[d], M.lev[d + 1], nullptr});
+      const int64_t bc = (M.lev[d + 1].n + 255) / 256, br = (8 * M.lev[d + 1].n + 255) / 256;
+      for (int64_t u = 0; u < bc; ++u) cm.push_back(BlockMap{p, (int)u});
+      for (int64_t u = 0; u < br; ++u) rp.push_back(BlockMap{p, (int)u});
+    }
+    VRec* drec;
+    BlockMap *dcm, *drp;
+    BTRY(upload(b, g, &drec, rec));
+    BTRY(upload(b, g, &dcm, cm));
+    BTRY(upload(b, g, &drp, rp));
+    S.cmask[d] = VGroup{drec, dcm};
+    S.rap[d] = VGroup{drec, drp};
+    S.gcmask[d] = (int)cm.size();
+    S.grap[d] = (int)rp.size();
+  }
   BHIP(b, hipStreamSynchronize(g.s));  // the host arrays go out of scope
   return HDG_OK;
 }
@@ -553,14 +593,14 @@ int enqueue_setup(hdg_batch* b, Group& g) {
     BLAUNCH(b, launch_to_float(g.T.vals, g.f_vals, (int64_t)kSlots * g.nt * g.N, g.s));
     BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)g.nt * g.N, g.s));
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, g.f_sp, g.s));
-    for (int j = 0; j < n; ++j) BLAUNCH(b, mg_setup_levels(g.Mi[j], g.s));
+    BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
     BLAUNCH(b, launch_mg_power_group(g.pat, op32(g), g.f_xa, g.f_xb, g.cg.part, gr, g.nbt, g.segs, n, factor, 10,
                                      g.s));
     BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op32(g), kv(g), mv32(g), gr, g.nbt, g.s));
     BLAUNCH(b, launch_mg_precond_group(g.pat, op32(g), kv(g), g.Mu, g.G, mv32(g), gr, g.nbt, g.s));
   } else {
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, nullptr, g.s));
-    for (int j = 0; j < n; ++j) BLAUNCH(b, mg_setup_levels(g.Mi[j], g.s));
+    BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
     // the CG vectors x, r serve as scratch before the solve initialises them
     BLAUNCH(b, launch_mg_power_group(g.pat, op64(g), g.cg.x, g.cg.r, g.cg.part, gr, g.nbt, g.segs, n, factor, 10,
                                      g.s));
@@ -720,44 +760,54 @@ int hdg_batch_create(hdg_batch** out, int device, size_t hbm_budget, int n, cons
   return HDG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One group from condensation to its results on the host, on the group's
+// stream: one chunk in flight while the previous chunk's states are read.
+int run_group(hdg_batch* b, Group& g, std::chrono::steady_clock::time_point t0) {
+  if (hipSetDevice(b->device) != hipSuccess) return fail(b, HDG_ERR_HIP, "hipSetDevice failed");
+  BTRY(enqueue_setup(b, g));
+  int slot = 0;
+  BTRY(launch_chunk(b, g, slot));
+  for (;;) {
+    const int nxt = slot ^ 1;
+    BTRY(launch_chunk(b, g, nxt));
+    BHIP(b, hipEventSynchronize(g.ev[slot]));
+    if (all_done(g, slot)) break;
+    slot = nxt;
+  }
+  BHIP(b, hipEventSynchronize(g.ev[slot ^ 1]));
+  BTRY(enqueue_finish(b, g));
+  BHIP(b, hipStreamSynchronize(g.s));
+  g.t_done = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return HDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int hdg_batch_run(hdg_batch* b, hdg_result* res) {
   if (!b || !res) return HDG_ERR_ARG;
   if (hipSetDevice(b->device) != hipSuccess) return fail(b, HDG_ERR_HIP, "hipSetDevice failed");
   for (Group& g : b->groups)
     if (!g.s) return fail(b, HDG_ERR_STATE, "batch was not built");
   const auto t0 = std::chrono::steady_clock::now();
-  auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-  for (Group& g : b->groups) {
-    BTRY(enqueue_setup(b, g));
-    g.slot = 0;
-    g.active = true;
-    BTRY(launch_chunk(b, g, 0));
+  // groups run side by side, each driven by a host thread of its own
+  std::vector<int> rc(b->groups.size(), HDG_OK);
+  {
+    std::vector<std::thread> th;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi)
+      th.emplace_back([&, gi] { rc[gi] = run_group(b, b->groups[gi], t0); });
+    for (auto& t : th) t.join();
   }
-  // every group keeps one chunk in flight while its previous chunk's states
-  // are read; a group leaves the loop when all its instances stopped
-  for (;;) {
-    bool any = false;
-    for (Group& g : b->groups)
-      if (g.active) BTRY(launch_chunk(b, g, g.slot ^ 1));
-    for (Group& g : b->groups) {
-      if (!g.active) continue;
-      BHIP(b, hipEventSynchronize(g.ev[g.slot]));
-      if (all_done(g, g.slot)) {
-        BHIP(b, hipEventSynchronize(g.ev[g.slot ^ 1]));
-        g.active = false;
-        BTRY(enqueue_finish(b, g));
-      } else {
-        g.slot ^= 1;
-        any = true;
-      }
-    }
-    if (!any) break;
-  }
+  for (int r : rc)
+    if (r != HDG_OK) return r;
   int first_bad = HDG_OK;
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     Group& g = b->groups[gi];
-    BHIP(b, hipStreamSynchronize(g.s));
-    g.t_done = now();
     const int n = (int)g.in.size();
     int flag = 0;
     std::memcpy(&flag, g.h_res + 2 * n, sizeof(int));

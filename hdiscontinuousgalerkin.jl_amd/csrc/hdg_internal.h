@@ -171,8 +171,10 @@ constexpr int kMaxLevels = 16;
 constexpr int kStencil = 9;
 // node budget of the levels solved inside one workgroup: their stencils,
 // diagonal inverses and three vectors live in LDS (13 doubles per node,
-// 152 KB of the 160 KB of a CU)
-constexpr int kTailNodes = 1500;
+// 53 KB), small enough that the workgroup finds room on a CU beside the
+// blocks of kernels running concurrently on other streams
+constexpr int kTailNodes = 512;
+constexpr int kTailThreads = 512;
 // coarsest grids up to this many nodes are solved exactly (inverse at set-up)
 constexpr int kCoarseDirect = 16;
 struct MgData {
@@ -276,15 +278,15 @@ int launch_errornorm_t(const DevMesh& m, const double* tables, const TabLayoutDe
 int launch_reduce_segs(const double* part, const Seg* seg, int nseg, double* out, hipStream_t s);
 
 // vertex hierarchy and S P0 (FP64, and a float copy of S P0 when sp32 is set)
-int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
-                     float* sp32, hipStream_t s);
+int mg_setup_numeric(const DevMesh& mesh, MgData& M, const MgData* dM, TraceSystem& T, const DevPattern& P,
+                     const double* Ate, float* sp32, hipStream_t s);
 // the two halves of mg_setup_numeric: the parts that run over the whole
 // (possibly packed) mesh -- vertex masks, per-cell coarse blocks, S P0 --
 // and the vertex levels of one structured instance (M describes it, with
-// its arrays at its own offsets)
+// its arrays at its own offsets; dM is its device copy)
 int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
                   const double* Ate, float* sp32, hipStream_t s);
-int mg_setup_levels(const MgData& M, hipStream_t s);
+int mg_setup_levels(const MgData& M, const MgData* dM, hipStream_t s);
 // omega = factor / lambda_max(Dinv S) by power steps on the operator as stored
 template <class V>
 int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
@@ -317,6 +319,22 @@ struct MgGroup {
   int vsweeps = 2;
   double omega_v = 0.8;
 };
+// Set-up tables of a group's vertex hierarchies: level-1 assembly blocks
+// (instance, block) and, per depth d, the instances with a level d+1, with
+// block maps over the coarse nodes (cmask) and 8 lanes per coarse node (RAP).
+struct MgSetupGroup {
+  int ninst = 0;
+  const MgData* dM = nullptr;
+  const BlockMap* l1map = nullptr;
+  int gl1 = 0;
+  int ndepth = 1;
+  VGroup cmask[kMaxLevels];
+  int gcmask[kMaxLevels];
+  VGroup rap[kMaxLevels];
+  int grap[kMaxLevels];
+};
+int mg_setup_levels_group(const MgSetupGroup& S, hipStream_t s);
+
 // grouped forms: rows of each instance from the Grouped selector (grid
 // blocks); Mu describes the packed mesh (its level 1 spans every
 // instance's vertices)

@@ -99,7 +99,7 @@ struct Geo {
   int ori[3];     // face_orientation (src/mesh.jl:51-54)
 };
 
-__device__ inline void cell_geometry(const DevMesh& m, int64_t c, Geo& g) {
+__device__ __forceinline__ void cell_geometry(const DevMesh& m, int64_t c, Geo& g) {
   int nd[3];
 #pragma unroll
   for (int v = 0; v < 3; ++v) {
@@ -151,6 +151,25 @@ __device__ inline double seg_bcast(double v, int src) {
 }
 
 // ------------------------------------------------------------- condense
+// v0 or v1 (i = 0, 1) and v0, v1 or v2 (i = 0, 1, 2).  BLEND forms them as
+// sums of products with 0/1 weights, so that the choice stays a choice
+// between values (exact for finite values up to the sign of a zero); the
+// plain ternaries may become a lane-dependent address into the cell's
+// geometry, which then lives in scratch.  At k = 4 the blend costs the
+// registers that keep two waves per SIMD (condensation of 512^2 cells:
+// 18.0 ms blended vs 11.7 ms with ternaries and scratch), so k = 4 keeps
+// the ternaries.
+template <bool BLEND>
+__device__ __forceinline__ double pick2(int i, double v0, double v1) {
+  if constexpr (BLEND) return (double)(i == 0) * v0 + (double)(i == 1) * v1;
+  else return i ? v1 : v0;
+}
+template <bool BLEND>
+__device__ __forceinline__ double pick3(int i, double v0, double v1, double v2) {
+  if constexpr (BLEND) return (double)(i == 0) * v0 + (double)(i == 1) * v1 + (double)(i == 2) * v2;
+  else return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+
 // One row of the bordered local system, for lane row r of the segment.
 //   r <  NQ : [ A(r,:) | -B(r,:) | -E(r,:) | 0      ]   q test functions
 //   r <  M  : [ B(:,i)'|  C(i,:) |  F(i,:) | be(i)  ]   u test functions
@@ -160,24 +179,24 @@ __device__ inline double seg_bcast(double v, int src) {
 // (examples/poisson2D_HDG.jl:88-153; the vector basis is block-by-component,
 // VectorFunctionSpaces.jl:189-195).
 template <int K>
-__device__ inline void build_row(int r, const double* T, const TabLayoutDev& L, const Geo& g,
+__device__ __forceinline__ void build_row(int r, const double* T, const TabLayoutDev& L, const Geo& g,
                                  double tau, const double* Aacc, double beacc,
                                  double (&row)[Dims<K>::NC]) {
   using D = Dims<K>;
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NT = D::NT, NTR = D::NTR;
-  // lane-dependent indices select among register values with ternaries
-  // (a dynamic index into a register array would move it to scratch)
+  // lane-dependent choices among the fields of g go through pick2 / pick3
+  constexpr bool BL = K <= 3;
   if (r < NQ) {
     const int c = r / NU, i = r - c * NU;
 #pragma unroll
     for (int s = 0; s < NQ; ++s) row[s] = (s / NU == c) ? Aacc[s % NU] : 0.0;
-    const double j0 = c ? g.Ji[2] : g.Ji[0], j1 = c ? g.Ji[3] : g.Ji[1];
+    const double j0 = pick2<BL>(c, g.Ji[0], g.Ji[2]), j1 = pick2<BL>(c, g.Ji[1], g.Ji[3]);
 #pragma unroll
     for (int j = 0; j < NU; ++j)
       row[NQ + j] = -g.detJ * (j0 * T[L.Dref + i * NU + j] + j1 * T[L.Dref + NU * NU + i * NU + j]);
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const double sc = -g.len[f] * (c ? g.nrm[2 * f + 1] : g.nrm[2 * f]);
+      const double sc = -g.len[f] * pick2<BL>(c, g.nrm[2 * f], g.nrm[2 * f + 1]);
       const double* Gf = T + L.G + ((f * 2 + g.ori[f]) * NU + i) * NT;
 #pragma unroll
       for (int t = 0; t < NT; ++t) row[M + f * NT + t] = sc * Gf[t];
@@ -206,11 +225,12 @@ __device__ inline void build_row(int r, const double* T, const TabLayoutDev& L, 
     row[M + NTR] = beacc;
   } else if (r < D::ROWS) {
     const int a = r - M, fa = a / NT, ta = a - fa * NT;
-    const int of = fa == 0 ? g.ori[0] : (fa == 1 ? g.ori[1] : g.ori[2]);
+    const int of = BL ? (fa == 0) * g.ori[0] + (fa == 1) * g.ori[1] + (fa == 2) * g.ori[2]
+                      : (fa == 0 ? g.ori[0] : (fa == 1 ? g.ori[1] : g.ori[2]));
     const double* Gf = T + L.G + (fa * 2 + of) * NU * NT;
-    const double el = fa == 0 ? g.len[0] : (fa == 1 ? g.len[1] : g.len[2]);
-    const double nx = el * (fa == 0 ? g.nrm[0] : (fa == 1 ? g.nrm[2] : g.nrm[4]));
-    const double ny = el * (fa == 0 ? g.nrm[1] : (fa == 1 ? g.nrm[3] : g.nrm[5]));
+    const double el = pick3<BL>(fa, g.len[0], g.len[1], g.len[2]);
+    const double nx = el * pick3<BL>(fa, g.nrm[0], g.nrm[2], g.nrm[4]);
+    const double ny = el * pick3<BL>(fa, g.nrm[1], g.nrm[3], g.nrm[5]);
 #pragma unroll
     for (int s = 0; s < NQ; ++s) row[s] = (s < NU ? nx : ny) * Gf[(s % NU) * NT + ta];
 #pragma unroll

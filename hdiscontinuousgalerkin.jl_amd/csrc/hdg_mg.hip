@@ -198,9 +198,12 @@ __global__ void mg_cell_coarse_kernel(DevMesh mesh, MgData M, TraceSystem T, con
 }
 
 // Level-1 stencils gathered from the (up to) six cells around each vertex.
-__global__ void mg_assemble_l1_kernel(MgData M) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const MgLevel L = M.lev[1];
+// One hierarchy (map null: Ms[0] over the grid) or one per block via map.
+__global__ void mg_assemble_l1_kernel(const MgData* __restrict__ Ms, const BlockMap* __restrict__ map) {
+  const BlockMap bm = map ? map[blockIdx.x] : BlockMap{0, (int)blockIdx.x};
+  const MgData& M = Ms[bm.inst];
+  const int64_t v = (int64_t)bm.lb * blockDim.x + threadIdx.x;
+  const MgLevel& L = M.lev[1];
   if (v >= L.n) return;
   const int nx = M.nx, ny = M.ny;
   const int i = (int)(v % (nx + 1)), j = (int)(v / (nx + 1));
@@ -232,8 +235,13 @@ __global__ void mg_assemble_l1_kernel(MgData M) {
 // Galerkin coarse stencil R A P on the next level: R = P^T with the weights
 // of prolong_at.  Eight lanes per coarse node: lane t < 7 takes the fine
 // node at support offset t, and a fixed shuffle tree sums the lanes' stencils.
-__global__ __launch_bounds__(256) void mg_rap_kernel(MgLevel F, MgLevel Cc) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <class VS>
+__global__ __launch_bounds__(256) void mg_rap_kernel(VS sel) {
+  int lb;
+  const VRec& R = vrec_of(sel, lb);
+  const MgLevel& F = R.F;
+  const MgLevel& Cc = R.C;
+  const int64_t gid = (int64_t)lb * blockDim.x + threadIdx.x;
   const int64_t C = gid >> 3;
   const int t = (int)(gid & 7);
   const int cnx = Cc.nx, fnx = F.nx, fny = F.ny;
@@ -287,11 +295,16 @@ __global__ __launch_bounds__(256) void mg_rap_kernel(MgLevel F, MgLevel Cc) {
 // Inverse of the coarsest operator (at most kCoarseDirect nodes), by
 // Gauss-Jordan with partial pivoting on [A | I] in LDS: one thread per entry,
 // the pivot row chosen by thread 0.  Fixed nodes have identity rows.
-__global__ __launch_bounds__(1024) void mg_coarse_inverse_kernel(MgLevel L, double* cinv) {
+// Block b inverts the coarsest level of Ms[b] (skipped above kCoarseDirect).
+__global__ __launch_bounds__(1024) void mg_coarse_inverse_kernel(const MgData* __restrict__ Ms) {
   constexpr int W = 2 * kCoarseDirect;
   __shared__ double m[kCoarseDirect * W];
   __shared__ int piv;
+  const MgData& M = Ms[blockIdx.x];
+  const MgLevel& L = M.lev[M.nlev - 1];
+  double* const cinv = M.cinv;
   const int n = (int)L.n;
+  if (n > kCoarseDirect) return;
   for (int e = threadIdx.x; e < n * W; e += blockDim.x) m[e] = 0.0;
   __syncthreads();
   for (int v = threadIdx.x; v < n; v += blockDim.x) {
@@ -336,8 +349,10 @@ __global__ __launch_bounds__(1024) void mg_coarse_inverse_kernel(MgLevel L, doub
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) cinv[e] = m[(e / n) * W + n + e % n];
 }
 
-// LDS image of the tail levels: per level its stencils (kStencil n) then dinv (n).
-__global__ void mg_tailpack_kernel(MgData M) {
+// LDS image of the tail levels: per level its stencils (kStencil n) then
+// dinv (n); blockIdx.y selects the hierarchy Ms[y].
+__global__ void mg_tailpack_kernel(const MgData* __restrict__ Ms) {
+  const MgData& M = Ms[blockIdx.y];
   int o = 0;
   for (int l = M.tail; l < M.nlev; ++l) {
     const int n = (int)M.lev[l].n;
@@ -349,8 +364,13 @@ __global__ void mg_tailpack_kernel(MgData M) {
 
 // Coarse masks (a coarse node is fixed when its fine node is), written
 // before mg_rap_kernel reads the masks of neighbouring nodes.
-__global__ void mg_cmask_kernel(MgLevel F, MgLevel Cc) {
-  const int64_t C = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <class VS>
+__global__ void mg_cmask_kernel(VS sel) {
+  int lb;
+  const VRec& R = vrec_of(sel, lb);
+  const MgLevel& F = R.F;
+  const MgLevel& Cc = R.C;
+  const int64_t C = (int64_t)lb * blockDim.x + threadIdx.x;
   if (C >= Cc.n) return;
   const int I = (int)(C % (Cc.nx + 1)), J = (int)(C / (Cc.nx + 1));
   Cc.mask[C] = F.mask[(int64_t)fine_of(J, F.ny) * (F.nx + 1) + fine_of(I, F.nx)];
@@ -831,7 +851,7 @@ __device__ inline void tail_sweep(const TailLevel& L, const double* in, double* 
 // Same sweeps as the per-level kernels: two pre-smoothing sweeps,
 // restriction of the residual, coarse_sweeps sweeps on the coarsest level,
 // prolongation and two post-smoothing sweeps.
-__global__ __launch_bounds__(1024) void v_tail_kernel(const MgData* __restrict__ Ms, const CgState* sts) {
+__global__ __launch_bounds__(kTailThreads) void v_tail_kernel(const MgData* __restrict__ Ms, const CgState* sts) {
   __shared__ double pool[(kStencil + 4) * kTailNodes];
   const MgData& M = Ms[blockIdx.x];
   if (sts[blockIdx.x].done) return;
@@ -979,7 +999,7 @@ int grid1(int64_t n, int t = 256) {
 // device copy of M that the tail kernel reads.
 int vcycle(const MgData& M, const MgData* dM, int l, const CgState* st, hipStream_t s) {
   if (l >= M.tail) {
-    hipLaunchKernelGGL(v_tail_kernel, dim3(1), dim3(1024), 0, s, dM, st);
+    hipLaunchKernelGGL(v_tail_kernel, dim3(1), dim3(kTailThreads), 0, s, dM, st);
     return (int)hipGetLastError();
   }
   const MgLevel& L = M.lev[l];
@@ -1011,7 +1031,7 @@ int vcycle_group(const MgGroup& G, hipStream_t s) {
     if (!one) hipLaunchKernelGGL(v_pre_kernel<VGroup>, dim3(G.gnode[d]), dim3(256), 0, s, G.node[d], om);
     hipLaunchKernelGGL(v_restrict_kernel<VGroup>, dim3(G.gres[d]), dim3(256), 0, s, G.res[d], one, om);
   }
-  hipLaunchKernelGGL(v_tail_kernel, dim3(G.ninst), dim3(1024), 0, s, G.dM, (const CgState*)G.st);
+  hipLaunchKernelGGL(v_tail_kernel, dim3(G.ninst), dim3(kTailThreads), 0, s, G.dM, (const CgState*)G.st);
   for (int d = G.ndepth - 1; d >= 1; --d) {
     hipLaunchKernelGGL(v_post_kernel<VGroup>, dim3(G.gnode[d]), dim3(256), 0, s, G.node[d], one, om);
     if (!one) hipLaunchKernelGGL(v_smooth_kernel<VGroup>, dim3(G.gnode[d]), dim3(256), 0, s, G.node[d], om);
@@ -1033,23 +1053,34 @@ int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, co
   return (int)hipGetLastError();
 }
 
-int mg_setup_levels(const MgData& M, hipStream_t s) {
-  hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, M);
+int mg_setup_levels(const MgData& M, const MgData* dM, hipStream_t s) {
+  hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(grid1(M.lev[1].n)), dim3(256), 0, s, dM, (const BlockMap*)nullptr);
   for (int l = 1; l + 1 < M.nlev; ++l) {
-    hipLaunchKernelGGL(mg_cmask_kernel, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
-    hipLaunchKernelGGL(mg_rap_kernel, dim3(grid1(8 * M.lev[l + 1].n)), dim3(256), 0, s, M.lev[l], M.lev[l + 1]);
+    const VOne rec{VRec{M.lev[l], M.lev[l + 1], nullptr}};
+    hipLaunchKernelGGL(mg_cmask_kernel<VOne>, dim3(grid1(M.lev[l + 1].n)), dim3(256), 0, s, rec);
+    hipLaunchKernelGGL(mg_rap_kernel<VOne>, dim3(grid1(8 * M.lev[l + 1].n)), dim3(256), 0, s, rec);
   }
-  if (M.lev[M.nlev - 1].n <= kCoarseDirect)
-    hipLaunchKernelGGL(mg_coarse_inverse_kernel, dim3(1), dim3(1024), 0, s, M.lev[M.nlev - 1], M.cinv);
-  if (M.tail < M.nlev) hipLaunchKernelGGL(mg_tailpack_kernel, dim3(16), dim3(256), 0, s, M);
+  hipLaunchKernelGGL(mg_coarse_inverse_kernel, dim3(1), dim3(1024), 0, s, dM);
+  if (M.tail < M.nlev) hipLaunchKernelGGL(mg_tailpack_kernel, dim3(16, 1), dim3(256), 0, s, dM);
   return (int)hipGetLastError();
 }
 
-int mg_setup_numeric(const DevMesh& mesh, MgData& M, TraceSystem& T, const DevPattern& P, const double* Ate,
-                     float* sp32, hipStream_t s) {
+int mg_setup_levels_group(const MgSetupGroup& S, hipStream_t s) {
+  hipLaunchKernelGGL(mg_assemble_l1_kernel, dim3(S.gl1), dim3(256), 0, s, S.dM, S.l1map);
+  for (int d = 1; d < S.ndepth; ++d) {
+    hipLaunchKernelGGL(mg_cmask_kernel<VGroup>, dim3(S.gcmask[d]), dim3(256), 0, s, S.cmask[d]);
+    hipLaunchKernelGGL(mg_rap_kernel<VGroup>, dim3(S.grap[d]), dim3(256), 0, s, S.rap[d]);
+  }
+  hipLaunchKernelGGL(mg_coarse_inverse_kernel, dim3(S.ninst), dim3(1024), 0, s, S.dM);
+  hipLaunchKernelGGL(mg_tailpack_kernel, dim3(16, S.ninst), dim3(256), 0, s, S.dM);
+  return (int)hipGetLastError();
+}
+
+int mg_setup_numeric(const DevMesh& mesh, MgData& M, const MgData* dM, TraceSystem& T, const DevPattern& P,
+                     const double* Ate, float* sp32, hipStream_t s) {
   const int e = mg_setup_mesh(mesh, M, T, P, Ate, sp32, s);
   if (e) return e;
-  return mg_setup_levels(M, s);
+  return mg_setup_levels(M, dM, s);
 }
 
 // omega = factor / lambda_max(Dinv S) by power steps on free dofs; xa, xb
