@@ -181,4 +181,65 @@ function errornorm(ctx::Context)
     out[]
 end
 
+# ---- batches (hdg_batch_*; no counterpart in the reference, which solves one
+# problem per script run).  Request and Result mirror hdg_request / hdg_result.
+
+struct Request
+    nx::Int64; ny::Int64; jitter::Float64; seed::UInt64
+    opt::Options; src::Source; uex_kind::Int32; want_trace::Int32
+end
+
+struct Result
+    id::Int64; status::Int32; iters::Int32
+    relres::Float64; true_relres::Float64; errornorm::Float64
+    n_cells::Int64; n_dofs::Int64
+    t_submit::Float64; t_start::Float64; t_done::Float64
+    device_bytes::Csize_t; slot::Int32
+end
+
+"""
+    Batch(reqs; device = 0, hbm_budget = 0)
+
+Many Poisson solves on rectangle meshes in grouped launches: requests of one
+order and options are packed into one mesh and share every kernel launch;
+each result is bitwise that of the request solved alone.
+"""
+mutable struct Batch
+    h::Ptr{Cvoid}
+    n::Int
+    function Batch(reqs::Vector{Request}; device::Integer = 0, hbm_budget::Integer = 0)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:hdg_batch_create, libhdg), Cint, (Ptr{Ptr{Cvoid}}, Cint, Csize_t, Cint, Ptr{Request}),
+                   r, device, hbm_budget, length(reqs), reqs)
+        if st != 0
+            msg = r[] == C_NULL ? "" :
+                  unsafe_string(ccall((:hdg_batch_last_error, libhdg), Cstring, (Ptr{Cvoid},), r[]))
+            r[] == C_NULL || ccall((:hdg_batch_destroy, libhdg), Cvoid, (Ptr{Cvoid},), r[])
+            error("hdg_batch_create failed with status $st: $msg")
+        end
+        b = new(r[], length(reqs))
+        finalizer(x -> ccall((:hdg_batch_destroy, libhdg), Cvoid, (Ptr{Cvoid},), x.h), b)
+        b
+    end
+end
+
+"""Condense, assemble, solve and recover every instance; one Result per request."""
+function run!(b::Batch)
+    res = Vector{Result}(undef, b.n)
+    st = ccall((:hdg_batch_run, libhdg), Cint, (Ptr{Cvoid}, Ptr{Result}), b.h, res)
+    if st != 0
+        msg = unsafe_string(ccall((:hdg_batch_last_error, libhdg), Cstring, (Ptr{Cvoid},), b.h))
+        @warn "hdg_batch_run: status $st ($msg); see each Result.status"
+    end
+    res
+end
+
+"""Trace solution of request i (1-based) after run!."""
+function trace(b::Batch, i::Integer, n_dofs::Integer)
+    out = Vector{Float64}(undef, n_dofs)
+    st = ccall((:hdg_batch_trace, libhdg), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), b.h, i - 1, out)
+    st == 0 || error("hdg_batch_trace failed with status $st")
+    out
+end
+
 end # module
