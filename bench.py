@@ -213,12 +213,15 @@ def run_stream(args):
     H = _import_package()
     rng = np.random.default_rng(SEED)
     reqs = stream_requests(H, args.workload, rng)
+    if args.eager:
+        for r in reqs:
+            r.opt.schedule = H._lib.SCHEDULE_EAGER
     budget = int(args.hbm_gib * (1 << 30))
     # warm the kernels of every order once (first-launch code loading)
     with H.Server(slots=1) as w:
         for k in (1, 2, 3, 4):
             w.wait(w.submit(H.make_request(nx=8, ny=8, k=k)))
-    srv = H.Server(slots=args.slots, hbm_budget=budget)
+    srv = H.Server(slots=args.slots, hbm_budget=budget, batch=args.batch, batch_cells=args.batch_cells)
     ids = []
     t0 = time.perf_counter()
     if args.workload == "batched":
@@ -258,6 +261,7 @@ def run_stream(args):
                                    f"Poisson arrivals {args.rate}/s for {args.seconds} s, n ~ U{{32..512}}, "
                                    "k ~ U{1..4}, 4x4 N(0,1) sine-series f")),
                    "slots": args.slots, "hbm_budget_gib": args.hbm_gib,
+                   "batch": args.batch, "batch_cells": args.batch_cells,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "requests": len(res), "failed": len(res) - len(ok), "cells": int(cells)},
         "p50_ms": 1e3 * float(np.percentile(lat, 50)) if len(lat) else None,
@@ -278,7 +282,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2,
+                    help="untimed steps (the first solve after a set-up runs eagerly, the second captures "
+                         "the CG graphs that later steps replay)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + list(STREAM_WORKLOADS), default="p2-512")
     ap.add_argument("--slots", type=int, default=8, help="batched/serving: concurrent solves")
     ap.add_argument("--hbm-gib", type=float, default=64.0, help="batched/serving: device-memory budget")
@@ -289,6 +295,12 @@ def main():
     ap.add_argument("--precond", choices=("auto", "bj", "mg"), default="auto",
                     help="trace-solve preconditioner: auto (multigrid on structured meshes), "
                          "block-Jacobi, multigrid")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="serving / --via-server: requests a slot solves together in grouped launches "
+                         "(1: one at a time)")
+    ap.add_argument("--batch-cells", type=int, default=4_000_000, help="cell cap of such a batch")
+    ap.add_argument("--eager", action="store_true",
+                    help="serving / --via-server: launch every kernel from the host (no CG graphs)")
     ap.add_argument("--via-server", action="store_true",
                     help="batched: run the instances through the serving loop instead of one batch")
     ap.add_argument("--calibrate", action="store_true",

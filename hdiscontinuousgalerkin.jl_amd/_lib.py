@@ -31,7 +31,7 @@ EXPORTS = (
     "hdg_get_condensed", "hdg_get_element_blocks", "hdg_solve", "hdg_set_trace", "hdg_recover",
     "hdg_errornorm", "hdg_run", "hdg_dubiner", "hdg_quadrature", "hdg_time_kernel", "hdg_get_trace",
     "hdg_server_create", "hdg_server_submit", "hdg_server_wait", "hdg_server_trace", "hdg_server_footprint",
-    "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error",
+    "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error", "hdg_server_set_batching",
     "hdg_batch_create", "hdg_batch_run", "hdg_batch_trace", "hdg_batch_info", "hdg_batch_destroy",
     "hdg_batch_last_error",
 )
@@ -151,6 +151,7 @@ def _declare(L):
         "hdg_server_stats": (C.c_int, [C.c_void_p, _P(ServerSummary)]),
         "hdg_server_destroy": (None, [C.c_void_p]),
         "hdg_server_last_error": (C.c_char_p, [C.c_void_p]),
+        "hdg_server_set_batching": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
         "hdg_batch_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int, _P(Request)]),
         "hdg_batch_run": (C.c_int, [C.c_void_p, _P(Result)]),
         "hdg_batch_trace": (C.c_int, [C.c_void_p, C.c_int, _d]),

@@ -69,8 +69,9 @@ struct hdg_ctx {
   float *f_e0 = nullptr, *f_res0 = nullptr, *f_xa = nullptr, *f_xb = nullptr;
 
   // captured chunks of CG iterations, one per read-back slot (valid until
-  // the next hdg_setup / hdg_set_mesh)
+  // the next hdg_setup / hdg_set_mesh); solves since the set-up
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int solves = 0;
 
   bool assembled = false, solved = false, recovered = false;
 };
@@ -143,6 +144,7 @@ void drop_graphs(hdg_ctx* c) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+  c->solves = 0;
 }
 
 void free_problem(hdg_ctx* c) {
@@ -437,9 +439,12 @@ int enqueue_chunk(hdg_ctx* c, double rtol2, int maxit) {
 
 // A chunk of iterations followed by the read-back of the CG state into
 // h_st[slot], eagerly or by replaying its captured graph (built once per
-// set-up: every launch argument is fixed until the next hdg_setup).
+// set-up: every launch argument is fixed until the next hdg_setup).  The
+// first solve after a set-up runs eagerly: capturing and instantiating the
+// graphs costs more than one solve's launches saves, and many contexts
+// capturing at once (the serving loop) serialise in the runtime.
 int launch_chunk(hdg_ctx* c, int slot, double rtol2, int maxit) {
-  const bool eager = (c->opt.schedule & HDG_SCHEDULE_EAGER) != 0;
+  const bool eager = (c->opt.schedule & HDG_SCHEDULE_EAGER) != 0 || c->solves == 0;
   if (!eager && !c->gexec[slot]) {
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -511,6 +516,7 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   HIPCHK(c, hipEventSynchronize(evs[slot ^ 1]));
   (void)hipEventDestroy(evs[0]);
   (void)hipEventDestroy(evs[1]);
+  ++c->solves;
   return finish_solve(c, info, maxit);
 }
 

@@ -67,7 +67,7 @@ struct Group {
          *uhat = nullptr;
   unsigned char* dmask = nullptr;
   double* uD = nullptr;
-  int* err = nullptr;
+  int* err = nullptr;  // [ninst] condensation error flags
   SrcParams* srcs = nullptr;
   int* cell_inst = nullptr;
   // launch tables
@@ -88,9 +88,10 @@ struct Group {
         *f_xa = nullptr, *f_xb = nullptr;
   // read-back
   CgState* h_st = nullptr;  // pinned, 2 x ninst (chunk slots) + ninst (final)
-  double* h_res = nullptr;  // pinned, 2 x ninst, then the error flag (an int)
+  double* h_res = nullptr;  // pinned, 2 x ninst, then ninst error flags (ints)
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
+  int runs = 0;  // completed runs (the first runs eagerly, later ones replay graphs)
   double t_done = 0.0;
 };
 
@@ -280,7 +281,7 @@ int build_group(hdg_batch* b, Group& g) {
   BHIP(b, hipEventCreateWithFlags(&g.ev[0], hipEventDisableTiming));
   BHIP(b, hipEventCreateWithFlags(&g.ev[1], hipEventDisableTiming));
   BHIP(b, hipHostMalloc((void**)&g.h_st, 3 * (size_t)n * sizeof(CgState), hipHostMallocDefault));
-  BHIP(b, hipHostMalloc((void**)&g.h_res, (2 * (size_t)n + 1) * sizeof(double), hipHostMallocDefault));
+  BHIP(b, hipHostMalloc((void**)&g.h_res, (3 * (size_t)n + 1) * sizeof(double), hipHostMallocDefault));
 
   // host packing, in parallel over the instances
   PackedHost H;
@@ -337,7 +338,7 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &g.qu, nc * g.m));
   BTRY(galloc(b, g, &g.uhat, N));
   BTRY(galloc(b, g, &g.uD, N));
-  BTRY(galloc(b, g, &g.err, 1));
+  BTRY(galloc(b, g, &g.err, n));
   BHIP(b, hipMemsetAsync(g.uD, 0, sizeof(double) * N, g.s));
   g.T = TraceSystem{};
   g.T.nt = g.nt;
@@ -566,7 +567,7 @@ int maxit_of(const Group& g) { return g.opt.maxit > 0 ? g.opt.maxit : 200000; }
 
 // condensation, assembly, preconditioner set-up, update(0)
 int enqueue_setup(hdg_batch* b, Group& g) {
-  BHIP(b, hipMemsetAsync(g.err, 0, sizeof(int), g.s));
+  BHIP(b, hipMemsetAsync(g.err, 0, sizeof(int) * g.in.size(), g.s));
   CondenseArgs a;
   a.mesh = g.mesh;
   a.tables = g.d_tab;
@@ -629,11 +630,13 @@ int enqueue_chunk(hdg_batch* b, Group& g) {
 }
 
 // One chunk of iterations for every instance of the group, then the states
-// into h_st[slot]; captured once as a graph unless HDG_SCHEDULE_EAGER.
+// into h_st[slot]; captured once as a graph, from the second run on, unless
+// HDG_SCHEDULE_EAGER (as for one problem, the first run is eager).
 int launch_chunk(hdg_batch* b, Group& g, int slot) {
   const size_t n = g.in.size();
   CgState* dst = g.h_st + slot * n;
-  if (!g.eager && !g.gexec[slot]) {
+  const bool eager = g.eager || g.runs == 0;
+  if (!eager && !g.gexec[slot]) {
     hipGraph_t gr = nullptr;
     BHIP(b, hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal));
     int r = enqueue_chunk(b, g);
@@ -646,7 +649,7 @@ int launch_chunk(hdg_batch* b, Group& g, int slot) {
     (void)hipGraphDestroy(gr);
     BHIP(b, e);
   }
-  if (!g.eager) {
+  if (!eager) {
     BHIP(b, hipGraphLaunch(g.gexec[slot], g.s));
   } else {
     BTRY(enqueue_chunk(b, g));
@@ -685,7 +688,7 @@ int enqueue_finish(hdg_batch* b, Group& g) {
   }
   BHIP(b, hipMemcpyAsync(g.h_st + 2 * n, g.st, sizeof(CgState) * n, hipMemcpyDeviceToHost, g.s));
   BHIP(b, hipMemcpyAsync(g.h_res, g.res, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, g.s));
-  BHIP(b, hipMemcpyAsync(g.h_res + 2 * n, g.err, sizeof(int), hipMemcpyDeviceToHost, g.s));
+  BHIP(b, hipMemcpyAsync(g.h_res + 2 * n, g.err, sizeof(int) * n, hipMemcpyDeviceToHost, g.s));
   return HDG_OK;
 }
 
@@ -782,6 +785,7 @@ int run_group(hdg_batch* b, Group& g, std::chrono::steady_clock::time_point t0) 
   BTRY(enqueue_finish(b, g));
   BHIP(b, hipStreamSynchronize(g.s));
   g.t_done = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  ++g.runs;
   return HDG_OK;
 }
 
@@ -809,13 +813,10 @@ int hdg_batch_run(hdg_batch* b, hdg_result* res) {
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     Group& g = b->groups[gi];
     const int n = (int)g.in.size();
-    int flag = 0;
-    std::memcpy(&flag, g.h_res + 2 * n, sizeof(int));
-    int gerr = HDG_OK;
-    if (flag & 1) gerr = HDG_ERR_NEG_JACOBIAN;
-    else if (flag & 2) gerr = HDG_ERR_LOCAL_SINGULAR;
+    const int* flags = reinterpret_cast<const int*>(g.h_res + 2 * n);
     for (int j = 0; j < n; ++j) {
       const Inst& I = g.in[j];
+      const int gerr = (flags[j] & 1) ? HDG_ERR_NEG_JACOBIAN : ((flags[j] & 2) ? HDG_ERR_LOCAL_SINGULAR : HDG_OK);
       const CgState& s = g.h_st[2 * n + j];
       hdg_result& r = res[I.req];
       std::memset(&r, 0, sizeof(r));

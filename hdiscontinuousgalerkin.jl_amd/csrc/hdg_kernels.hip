@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
         for (int c = M; c < D::NC; ++c) row[c] *= ip;
       }
       if (valid) {
-        if (bad && ls == 0) atomicOr(a.err, g.detJ > 0.0 ? 2 : 1);
+        if (bad && ls == 0) atomicOr(a.err + (GROUPED ? a.cell_inst[cell] : 0), g.detJ > 0.0 ? 2 : 1);
         if (ls < M) {
           double* ke = a.Ke + cell * (int64_t)(M * NTR);
 #pragma unroll

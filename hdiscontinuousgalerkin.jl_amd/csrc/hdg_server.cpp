@@ -19,8 +19,8 @@
 
 namespace {
 
-constexpr int kLookahead = 8;  // queued requests a free slot may pick from
-
+constexpr int kLookahead = 8;     // queued requests a free slot may pick from
+constexpr int kBatchLook = 512;   // queued requests scanned for a batch
 struct Pending {
   int64_t id;
   hdg_request req;
@@ -51,6 +51,11 @@ bool same_options(const hdg_options& a, const hdg_options& b) {
          a.rtol == b.rtol && a.maxit == b.maxit && a.schedule == b.schedule && a.precond == b.precond;
 }
 
+// requests hdg_batch_* accepts: analytic sources, SINSIN or no error norm
+bool batchable(const hdg_request& q) {
+  return q.src.kind >= 0 && q.src.kind <= 2 && (q.uex_kind == -1 || q.uex_kind == HDG_UEX_SINSIN);
+}
+
 }  // namespace
 
 struct hdg_server {
@@ -68,6 +73,8 @@ struct hdg_server {
   std::vector<std::thread> threads;
   std::vector<double> latency, qwait;
   int64_t completed = 0, failed = 0, cells = 0;
+  int batch_max = 1;        // requests per grouped launch (1: one request per slot)
+  int64_t batch_cells = 0;  // cell cap of a batch
   double first_start = 1e300, last_done = 0.0;
   std::chrono::steady_clock::time_point t0;
   std::string err;
@@ -153,6 +160,62 @@ hdg_result run_request(hdg_server* S, Slot& sl, const Pending& p, std::vector<do
   return r;
 }
 
+// Records a finished request (lock held).
+void record(hdg_server* S, const hdg_result& r, std::vector<double>& trace) {
+  if (r.status == HDG_OK) {
+    ++S->completed;
+    S->cells += r.n_cells;
+    S->latency.push_back(r.t_done - r.t_submit);
+    S->qwait.push_back(r.t_start - r.t_submit);
+    S->first_start = std::min(S->first_start, r.t_start);
+    S->last_done = std::max(S->last_done, r.t_done);
+  } else {
+    ++S->failed;
+  }
+  Done& d = S->done[r.id];
+  d.res = r;
+  d.trace.swap(trace);
+}
+
+// Runs several requests as one hdg_batch in slot w (lock not held).
+void run_batch(hdg_server* S, int w, const std::vector<Pending>& batch) {
+  const int n = (int)batch.size();
+  std::vector<hdg_request> rq(n);
+  for (int i = 0; i < n; ++i) rq[i] = batch[i].req;
+  const double t_start = S->now();
+  std::vector<hdg_result> res(n);
+  std::vector<std::vector<double>> traces(n);
+  hdg_batch* b = nullptr;
+  int st = hdg_batch_create(&b, S->device, 0, n, rq.data());
+  double t_run = S->now();
+  if (st == HDG_OK) {
+    hdg_batch_run(b, res.data());  // per-request statuses in res
+    for (int i = 0; i < n; ++i)
+      if (rq[i].want_trace && res[i].status == HDG_OK) {
+        traces[i].resize(res[i].n_dofs);
+        if (hdg_batch_trace(b, i, traces[i].data()) != HDG_OK) traces[i].clear();
+      }
+  } else {
+    for (int i = 0; i < n; ++i) {
+      std::memset(&res[i], 0, sizeof(hdg_result));
+      res[i].status = st;
+      res[i].t_done = S->now() - t_run;
+    }
+  }
+  if (b) hdg_batch_destroy(b);
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (int i = 0; i < n; ++i) {
+    hdg_result& r = res[i];
+    r.id = batch[i].id;
+    r.t_submit = batch[i].t_submit;
+    r.t_start = t_start;
+    r.t_done = t_run + r.t_done;
+    r.slot = w;
+    record(S, r, traces[i]);
+  }
+  S->cv_done.notify_all();
+}
+
 void worker(hdg_server* S, int w) {
   Slot& sl = S->slots[w];
   std::unique_lock<std::mutex> lk(S->mu);
@@ -188,6 +251,42 @@ void worker(hdg_server* S, int w) {
     }
     Pending p = S->queue[pick];
     S->queue.erase(S->queue.begin() + pick);
+    if (S->batch_max > 1 && batchable(p.req)) {
+      // continuous batching: the queued requests with the same options that
+      // fit the cell cap and the budget join this one in grouped launches
+      std::vector<Pending> batch{p};
+      size_t foot = p.foot;
+      int64_t cells = 2 * p.req.nx * p.req.ny;
+      const size_t base = S->reserved - sl.held;
+      for (size_t j = 0; j < S->queue.size() && j < (size_t)kBatchLook && (int)batch.size() < S->batch_max;) {
+        const Pending& q = S->queue[j];
+        const int64_t qc = 2 * q.req.nx * q.req.ny;
+        if (batchable(q.req) && same_options(q.req.opt, p.req.opt) && cells + qc <= S->batch_cells &&
+            (S->budget == 0 || base + foot + q.foot <= S->budget)) {
+          batch.push_back(q);
+          foot += q.foot;
+          cells += qc;
+          S->queue.erase(S->queue.begin() + j);
+        } else {
+          ++j;
+        }
+      }
+      // the slot's single-request context gives its memory back first
+      hdg_ctx* old_ctx = sl.ctx;
+      sl.ctx = nullptr;
+      sl.has_mesh = sl.has_setup = false;
+      S->reserved = base + foot;
+      sl.held = foot;
+      S->peak = std::max(S->peak, S->reserved);
+      lk.unlock();
+      if (old_ctx) hdg_destroy(old_ctx);
+      run_batch(S, w, batch);
+      lk.lock();
+      S->reserved -= sl.held;
+      sl.held = 0;
+      S->cv_work.notify_all();
+      continue;
+    }
     S->reserved = S->reserved - sl.held + p.foot;
     sl.held = p.foot;
     S->peak = std::max(S->peak, S->reserved);
@@ -202,15 +301,7 @@ void worker(hdg_server* S, int w) {
     S->reserved = S->reserved - sl.held + r.device_bytes;
     sl.held = r.device_bytes;
     S->peak = std::max(S->peak, S->reserved);
-    if (r.status == HDG_OK) {
-      ++S->completed;
-      S->cells += r.n_cells;
-      S->latency.push_back(r.t_done - r.t_submit);
-      S->qwait.push_back(r.t_start - r.t_submit);
-      S->first_start = std::min(S->first_start, r.t_start);
-      S->last_done = std::max(S->last_done, r.t_done);
-    } else {
-      ++S->failed;
+    if (r.status != HDG_OK) {
       // a failed context may hold partial state: start the slot afresh
       hdg_ctx* c = sl.ctx;
       sl.ctx = nullptr;
@@ -221,9 +312,7 @@ void worker(hdg_server* S, int w) {
       hdg_destroy(c);
       lk.lock();
     }
-    Done& d = S->done[r.id];
-    d.res = r;
-    d.trace.swap(trace);
+    record(S, r, trace);
     S->cv_done.notify_all();
     S->cv_work.notify_all();
   }
@@ -272,6 +361,14 @@ int hdg_server_create(hdg_server** out, int device, size_t hbm_budget, int slots
   S->slots.resize(slots);
   for (int w = 0; w < slots; ++w) S->threads.emplace_back(worker, S, w);
   *out = S;
+  return HDG_OK;
+}
+
+int hdg_server_set_batching(hdg_server* S, int max_requests, int64_t max_cells) {
+  if (!S || max_requests < 1 || max_cells < 0) return HDG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->batch_max = max_requests;
+  S->batch_cells = max_cells > 0 ? max_cells : INT64_MAX;
   return HDG_OK;
 }
 

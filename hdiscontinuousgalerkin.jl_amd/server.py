@@ -49,12 +49,18 @@ def footprint(req):
 
 
 class Server:
-    def __init__(self, device=0, hbm_budget=0, slots=4):
+    """batch > 1: a free slot solves up to `batch` queued requests with the
+    same options together in grouped launches (at most batch_cells cells,
+    0 = no cap); results are bitwise those of solving them alone."""
+
+    def __init__(self, device=0, hbm_budget=0, slots=4, batch=1, batch_cells=0):
         h = C.c_void_p()
         code = lib().hdg_server_create(C.byref(h), int(device), int(hbm_budget), int(slots))
         if code != L.HDG_OK:
             raise L.HDGError(code, "hdg_server_create failed")
         self._h = h
+        if batch > 1:
+            self._chk(lib().hdg_server_set_batching(self._h, int(batch), int(batch_cells)))
 
     def _chk(self, code):
         if code != L.HDG_OK:

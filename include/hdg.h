@@ -63,9 +63,11 @@ enum { HDG_FP64 = 0, HDG_MIXED = 1 };
    mesh is a structured rectangle_mesh and block-Jacobi otherwise. */
 enum { HDG_PC_AUTO = 0, HDG_PC_BLOCK_JACOBI = 1, HDG_PC_MULTIGRID = 2 };
 
-/* Schedule flags.  By default each chunk of CG iterations is captured once
-   per hdg_setup as a HIP graph and replayed; HDG_SCHEDULE_EAGER launches
-   every kernel from the host instead (same results, bit for bit). */
+/* Schedule flags.  By default the first solve after hdg_setup launches its
+   kernels from the host, and from the second solve on each chunk of CG
+   iterations is captured once as a HIP graph and replayed (capture costs
+   more than one solve saves); HDG_SCHEDULE_EAGER never captures.  Same
+   results, bit for bit. */
 enum { HDG_SCHEDULE_EAGER = 1 };
 
 /* Sources.  HDG_SRC_REFERENCE is the reference's f = 2 pi^2 sin(pi x) sin(pi y)
@@ -265,6 +267,16 @@ typedef struct {
 
 /* hbm_budget 0 = no cap; slots >= 1. */
 int hdg_server_create(hdg_server** srv, int device, size_t hbm_budget, int slots);
+/* Continuous batching: a free slot that takes a request also takes up to
+   max_requests - 1 more queued requests with the same options (order,
+   quadrature, tau, precision, tolerances, preconditioner) while their cells
+   stay within max_cells (0 = no cap) and their footprints within the
+   budget, and solves them together in grouped launches (hdg_batch_*).
+   Results are bitwise those of solving each request alone.  Requests a
+   batch does not accept (HDG_SRC_VALUES, HDG_UEX_LINEAR) run alone.
+   max_requests = 1 (the default) turns batching off.  Call before
+   submitting. */
+int hdg_server_set_batching(hdg_server* srv, int max_requests, int64_t max_cells);
 /* Queues a request (copied); returns at once with its id. */
 int hdg_server_submit(hdg_server* srv, const hdg_request* req, int64_t* id);
 /* Blocks until request `id` completes (timeout_s < 0: no timeout), fills

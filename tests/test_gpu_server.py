@@ -1,6 +1,8 @@
 """GPU tests of the serving loop (hdg_server_*): results identical alone
 and in a mix of concurrent requests, HBM-budget admission, error isolation.
 """
+import collections
+
 import numpy as np
 import pytest
 
@@ -96,3 +98,38 @@ def test_footprint_matches_device_use(hdg):
             res = s.wait(s.submit(r))
         assert res["status"] == 0
         assert res["device_bytes"] <= hdg.footprint(r) <= 1.5 * res["device_bytes"] + (16 << 20)
+
+
+def test_server_batching_equals_alone(hdg):
+    """With continuous batching, queued requests that share options are
+    solved together in grouped launches; each result is bitwise the one the
+    request gets alone, and a request whose mesh folds over (negative
+    Jacobian) fails alone inside its batch."""
+    specs = [(48, 40, 2), (64, 64, 1), (33, 17, 3), (96, 80, 2), (20, 100, 1), (50, 50, 1),
+             (40, 72, 3), (81, 81, 2), (16, 16, 1), (100, 60, 1), (64, 32, 3), (24, 24, 2)]
+    reqs = [_req(hdg, nx, ny, k, 700 + i, want_trace=True) for i, (nx, ny, k) in enumerate(specs)]
+    alone = []
+    with hdg.Server(slots=1) as s1:
+        for r in reqs:
+            res = s1.wait(s1.submit(r))
+            assert res["status"] == 0
+            alone.append((res, s1.trace(res["id"], res["n_dofs"])))
+    folded = _req(hdg, 16, 16, 1, 799)
+    folded.jitter = 0.95
+    with hdg.Server(slots=2, batch=16) as sb:
+        # two large requests hold both slots while the rest queue up
+        busy = [sb.submit(_req(hdg, 256, 256, 3, 790 + i)) for i in range(2)]
+        ids = [sb.submit(r) for r in reqs]
+        bad = sb.submit(folded)
+        res = [sb.wait(i) for i in ids]
+        tr = [sb.trace(i, r["n_dofs"]) for i, r in zip(ids, res)]
+        rbad = sb.wait(bad)
+        assert all(sb.wait(i)["status"] == 0 for i in busy)
+    for (ra, ta), rb, tb in zip(alone, res, tr):
+        assert rb["status"] == 0 and rb["iters"] == ra["iters"]
+        assert rb["true_relres"] == ra["true_relres"]
+        assert np.array_equal(ta, tb)
+    assert rbad["status"] == 2  # HDG_ERR_NEG_JACOBIAN
+    # several requests shared one batch (same slot, same start)
+    together = collections.Counter((r["slot"], r["t_start"]) for r in res + [rbad])
+    assert max(together.values()) >= 3
