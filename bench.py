@@ -159,6 +159,9 @@ def run_batched(args):
     H = _import_package()
     rng = np.random.default_rng(SEED)
     reqs = stream_requests(H, "batched", rng)
+    if args.eager:
+        for r in reqs:
+            r.opt.schedule = H._lib.SCHEDULE_EAGER
     t0 = time.perf_counter()
     b = H.Batch(reqs, hbm_budget=int(args.hbm_gib * (1 << 30)))
     t_build = time.perf_counter() - t0
@@ -300,7 +303,7 @@ def main():
                          "(1: one at a time)")
     ap.add_argument("--batch-cells", type=int, default=4_000_000, help="cell cap of such a batch")
     ap.add_argument("--eager", action="store_true",
-                    help="serving / --via-server: launch every kernel from the host (no CG graphs)")
+                    help="batched / serving: launch every kernel from the host (no CG graphs)")
     ap.add_argument("--via-server", action="store_true",
                     help="batched: run the instances through the serving loop instead of one batch")
     ap.add_argument("--calibrate", action="store_true",
