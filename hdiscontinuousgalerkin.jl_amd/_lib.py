@@ -20,6 +20,7 @@ STATUS = {
 HDG_FP64, HDG_MIXED = 0, 1
 PC_AUTO, PC_BLOCK_JACOBI, PC_MULTIGRID = 0, 1, 2
 SCHEDULE_EAGER = 1
+SCHEDULE_AUTOTUNE = 2
 SRC_REFERENCE, SRC_KAPPA, SRC_SINE_SERIES, SRC_VALUES = 0, 1, 2, 3
 UEX_SINSIN, UEX_LINEAR = 0, 1
 
@@ -33,7 +34,7 @@ EXPORTS = (
     "hdg_server_create", "hdg_server_submit", "hdg_server_wait", "hdg_server_trace", "hdg_server_footprint",
     "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error", "hdg_server_set_batching",
     "hdg_batch_create", "hdg_batch_run", "hdg_batch_trace", "hdg_batch_info", "hdg_batch_destroy",
-    "hdg_batch_last_error",
+    "hdg_batch_last_error", "hdg_autotune", "hdg_set_schedule",
 )
 
 
@@ -87,6 +88,11 @@ class Result(C.Structure):
                 ("true_relres", C.c_double), ("errornorm", C.c_double), ("n_cells", C.c_int64),
                 ("n_dofs", C.c_int64), ("t_submit", C.c_double), ("t_start", C.c_double),
                 ("t_done", C.c_double), ("device_bytes", C.c_size_t), ("slot", C.c_int32)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("condense_seg", C.c_int32), ("condense_grid", C.c_int32), ("condense_us", C.c_double),
+                ("default_us", C.c_double)]
 
 
 class ServerSummary(C.Structure):
@@ -152,6 +158,8 @@ def _declare(L):
         "hdg_server_destroy": (None, [C.c_void_p]),
         "hdg_server_last_error": (C.c_char_p, [C.c_void_p]),
         "hdg_server_set_batching": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+        "hdg_autotune": (C.c_int, [C.c_void_p, _P(Schedule)]),
+        "hdg_set_schedule": (C.c_int, [C.c_void_p, _P(Schedule)]),
         "hdg_batch_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int, _P(Request)]),
         "hdg_batch_run": (C.c_int, [C.c_void_p, _P(Result)]),
         "hdg_batch_trace": (C.c_int, [C.c_void_p, C.c_int, _d]),

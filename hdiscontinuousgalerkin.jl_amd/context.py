@@ -172,6 +172,18 @@ class Context:
         self._chk(lib().hdg_run(self._h, C.byref(info)))
         return info
 
+    def autotune(self):
+        """Times the condensation kernel's schedules on the current problem
+        and keeps the fastest (hdg_autotune); returns the choice as a dict.
+        Assemble again before solving."""
+        sc = L.Schedule()
+        self._chk(lib().hdg_autotune(self._h, C.byref(sc)))
+        return {f: getattr(sc, f) for f, _ in L.Schedule._fields_}
+
+    def set_schedule(self, condense_seg=0, condense_grid=0):
+        sc = L.Schedule(condense_seg=int(condense_seg), condense_grid=int(condense_grid))
+        self._chk(lib().hdg_set_schedule(self._h, C.byref(sc)))
+
     def time_kernel(self, which, reps=50):
         out = np.zeros(1)
         self._chk(lib().hdg_time_kernel(self._h, int(which), int(reps), ptr(out)))

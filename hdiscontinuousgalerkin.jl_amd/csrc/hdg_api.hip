@@ -72,6 +72,10 @@ struct hdg_ctx {
   // the next hdg_setup / hdg_set_mesh); solves since the set-up
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int solves = 0;
+  // condensation schedule (autotuned or default) and its measurements
+  CondenseSched csched;
+  bool tuned = false;
+  double tuned_us = 0.0, default_us = 0.0;
 
   bool assembled = false, solved = false, recovered = false;
 };
@@ -183,6 +187,8 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->f_xa);
   dfree(c, c->f_xb);
   c->has_setup = c->assembled = c->solved = c->recovered = false;
+  c->csched = CondenseSched{};
+  c->tuned = false;
 }
 
 void free_mesh(hdg_ctx* c) {
@@ -372,9 +378,7 @@ int check_err_flag(hdg_ctx* c) {
   return HDG_OK;
 }
 
-int assemble_impl(hdg_ctx* c) {
-  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
-  HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+CondenseArgs condense_args(const hdg_ctx* c) {
   CondenseArgs a;
   a.mesh = c->mesh;
   a.tables = c->d_tab;
@@ -387,7 +391,21 @@ int assemble_impl(hdg_ctx* c) {
   a.Ke = c->d_Ke;
   a.be = c->d_be;
   a.err = c->d_err;
-  LAUNCHCHK(c, launch_condense(c->k, a, false, c->stream));
+  return a;
+}
+
+int autotune_impl(hdg_ctx* c) {
+  LAUNCHCHK(c, tune_condense(c->k, condense_args(c), c->stream, &c->csched, &c->tuned_us, &c->default_us));
+  c->tuned = true;
+  c->assembled = c->solved = c->recovered = false;
+  return HDG_OK;
+}
+
+int assemble_impl(hdg_ctx* c) {
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  if ((c->opt.schedule & HDG_SCHEDULE_AUTOTUNE) && !c->tuned) TRY(autotune_impl(c));
+  HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  LAUNCHCHK(c, launch_condense(c->k, condense_args(c), false, c->stream, c->csched));
   LAUNCHCHK(c, launch_fill(c->pat, c->d_Ate, c->d_bte, c->T, c->stream));
   LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
   c->jacobi_dirty = false;
@@ -1074,6 +1092,33 @@ int hdg_quadrature(int dim, int order, int cap, double* pts, double* w) {
   return n;
 }
 
+int hdg_autotune(hdg_ctx* c, hdg_schedule* out) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  TRY(select_device(c));
+  TRY(autotune_impl(c));
+  if (out) {
+    out->condense_seg = c->csched.seg;
+    out->condense_grid = c->csched.grid;
+    out->condense_us = c->tuned_us;
+    out->default_us = c->default_us;
+  }
+  return HDG_OK;
+}
+
+int hdg_set_schedule(hdg_ctx* c, const hdg_schedule* sc) {
+  if (!c || !sc) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  const int seg = sc->condense_seg, nat = condense_native_seg(c->k);
+  if (seg != 0 && ((seg != 16 && seg != 32 && seg != 64) || seg < nat))
+    return fail(c, HDG_ERR_ARG, "condense_seg=%d: 0 or 16/32/64 lanes, at least %d at k=%d", seg, nat, c->k);
+  if (sc->condense_grid < 0 || sc->condense_grid > (1 << 20))
+    return fail(c, HDG_ERR_ARG, "condense_grid=%d out of range", sc->condense_grid);
+  c->csched = CondenseSched{seg, sc->condense_grid};
+  c->tuned = true;
+  return HDG_OK;
+}
+
 int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   if (!c || !avg_us || reps < 1) return HDG_ERR_ARG;
   if (!c->assembled) return fail(c, HDG_ERR_STATE, "nothing assembled to time");
@@ -1081,18 +1126,7 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   // make sure the solver state exists (pq, rz) so the kernels divide by
   // finite numbers; timing runs leave x and r untouched
   if (!c->solved) LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
-  CondenseArgs a;
-  a.mesh = c->mesh;
-  a.tables = c->d_tab;
-  a.lay = c->lay;
-  a.src = c->src;
-  a.src.f_qp = c->d_fqp;
-  a.tau = c->opt.tau;
-  a.Ate = c->d_Ate;
-  a.bte = c->d_bte;
-  a.Ke = c->d_Ke;
-  a.be = c->d_be;
-  a.err = c->d_err;
+  const CondenseArgs a = condense_args(c);
   double* scratch = nullptr;
   const int64_t ncopy = (int64_t)kSlots * c->nt * c->T.N;  // the size of the trace matrix values
   if (which == 3) TRY(dalloc(c, &scratch, ncopy));
@@ -1100,7 +1134,7 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
     if (which == 0) return launch_cg_spmv(c->pat, op64(c), kv64(c), 2, 0.0, 0, 1, c->stream);
     if (which == 1) return launch_cg_update(c->T, c->cg, 2, 1, c->stream);
     if (which == 3) return launch_copy(c->T.vals, scratch, ncopy, c->stream);
-    return launch_condense(c->k, a, false, c->stream);
+    return launch_condense(c->k, a, false, c->stream, c->csched);
   };
   int rc = once();  // warm-up
   if (rc) { dfree(c, scratch); return fail(c, HDG_ERR_HIP, "kernel launch failed"); }

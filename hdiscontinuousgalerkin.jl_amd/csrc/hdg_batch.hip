@@ -84,6 +84,7 @@ struct Group {
   LevelBuf lev[kMaxLevels];
   MgGroup G;
   MgSetupGroup S;
+  CondenseSched csched;  // condensation schedule (autotuned at create with HDG_SCHEDULE_AUTOTUNE)
   float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr,
         *f_xa = nullptr, *f_xb = nullptr;
   // read-back
@@ -247,6 +248,31 @@ bool pack_instance(const hdg_request& q, const Group& g, int j, PackedHost& H) {
   return true;
 }
 
+CondenseArgs group_condense_args(const Group& g) {
+  CondenseArgs a;
+  a.mesh = g.mesh;
+  a.tables = g.d_tab;
+  a.lay = g.lay;
+  a.srcs = g.srcs;
+  a.cell_inst = g.cell_inst;
+  a.tau = g.opt.tau;
+  a.Ate = g.Ate;
+  a.bte = g.bte;
+  a.Ke = g.Ke;
+  a.be = g.be;
+  a.err = g.err;
+  return a;
+}
+
+// With HDG_SCHEDULE_AUTOTUNE: the condensation schedule that runs fastest
+// on the group's packed mesh (the arrays it writes are rewritten by every
+// run).
+int tune_group(hdg_batch* b, Group& g) {
+  if (!(g.opt.schedule & HDG_SCHEDULE_AUTOTUNE)) return HDG_OK;
+  BLAUNCH(b, tune_condense(g.k, group_condense_args(g), g.s, &g.csched, nullptr, nullptr));
+  return HDG_OK;
+}
+
 // Lays out one group: offsets, levels, host packing, device arrays, tables.
 int build_group(hdg_batch* b, Group& g) {
   const int n = (int)g.in.size();
@@ -382,6 +408,7 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &v.q, N));
   BTRY(galloc(b, g, &v.part, 4 * (size_t)g.nbt + 64));
   if (!g.mg) {
+    BTRY(tune_group(b, g));
     BHIP(b, hipStreamSynchronize(g.s));
     return HDG_OK;
   }
@@ -509,6 +536,7 @@ int build_group(hdg_batch* b, Group& g) {
     G.gnode[d] = (int)node.size();
     G.gres[d] = (int)res.size();
   }
+  BTRY(tune_group(b, g));
   // set-up tables: level-1 assembly, then cmask / RAP for every level pair
   MgSetupGroup& S = g.S;
   S.ninst = n;
@@ -568,19 +596,7 @@ int maxit_of(const Group& g) { return g.opt.maxit > 0 ? g.opt.maxit : 200000; }
 // condensation, assembly, preconditioner set-up, update(0)
 int enqueue_setup(hdg_batch* b, Group& g) {
   BHIP(b, hipMemsetAsync(g.err, 0, sizeof(int) * g.in.size(), g.s));
-  CondenseArgs a;
-  a.mesh = g.mesh;
-  a.tables = g.d_tab;
-  a.lay = g.lay;
-  a.srcs = g.srcs;
-  a.cell_inst = g.cell_inst;
-  a.tau = g.opt.tau;
-  a.Ate = g.Ate;
-  a.bte = g.bte;
-  a.Ke = g.Ke;
-  a.be = g.be;
-  a.err = g.err;
-  BLAUNCH(b, launch_condense(g.k, a, false, g.s));
+  BLAUNCH(b, launch_condense(g.k, group_condense_args(g), false, g.s, g.csched));
   BLAUNCH(b, launch_fill(g.pat, g.Ate, g.bte, g.T, g.s));
   BLAUNCH(b, launch_jacobi(g.pat, g.T, g.s));
   const Grouped gr = rows(g);

@@ -232,7 +232,23 @@ __device__ __forceinline__ const VRec& vrec_of(const VGroup& g, int& lb) {
 
 // launchers (return hipError_t as int)
 int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s);
-int launch_condense(int k, const CondenseArgs& a, bool blocks_mode, hipStream_t s);
+// Schedule of the condensation kernel: lanes per element (16, 32 or 64;
+// 0 = the narrowest that holds the rows) and the grid cap (blocks of 256
+// threads that stride over the cells; 0 = 8192).  Results do not depend on
+// it; the autotuner picks it by measurement.
+struct CondenseSched {
+  int seg = 0;
+  int grid = 0;
+};
+int launch_condense(int k, const CondenseArgs& a, bool blocks_mode, hipStream_t s,
+                    const CondenseSched& sc = CondenseSched{});
+int condense_native_seg(int k);
+// Autotuner: times every (seg, grid) schedule of the condensation kernel on
+// the cells of a (3 warm launches each, HIP events on stream s) and returns
+// the fastest; the default schedule's time in default_us.  Overwrites the
+// outputs of a.
+int tune_condense(int k, const CondenseArgs& a, hipStream_t s, CondenseSched* best, double* best_us,
+                  double* default_us);
 int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s);
 int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);

@@ -253,10 +253,14 @@ __device__ __forceinline__ const SrcParams& src_of(const CondenseArgs& a, int64_
   else return a.src;
 }
 
-template <int K, bool BLOCKS, bool GROUPED>
+// SEG: lanes per element (a power of two >= the rows of the bordered
+// system); EPW = 64 / SEG elements per wavefront.  The result does not
+// depend on SEG (the pivot key below is the same for every SEG).
+template <int K, bool BLOCKS, bool GROUPED, int SEG>
 __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
   using D = Dims<K>;
-  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, SEG = D::SEG, EPW = D::EPW;
+  static_assert(SEG >= D::ROWS && 64 % SEG == 0, "segment too small");
+  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, EPW = 64 / SEG;
   extern __shared__ double T[];
   const TabLayoutDev L = a.lay;
   for (int i = threadIdx.x; i < L.total; i += blockDim.x) T[i] = __ldg(a.tables + i);
@@ -351,16 +355,15 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
       // the rows not yet used as pivots (implicit row exchanges).  Pivot rows
       // stay unscaled (each row is divided by its own pivot at the end), so
       // an elimination step is one fma per column.  The pivot search reduces
-      // a 32-bit key: the upper word of |a_ij| with its low log2(SEG) bits
-      // replaced by the lane (ties and near-ties go to the lower lane).
-      constexpr int LB = SEG == 16 ? 4 : (SEG == 32 ? 5 : 6);
+      // a 32-bit key: the upper word of |a_ij| with its low 6 bits replaced
+      // by 63 - (row index) (ties and near-ties go to the lower row, the
+      // same choice for every SEG).
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~(unsigned)(SEG - 1)) |
-                                    (unsigned)(SEG - 1 - ls));
+        int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~63u) | (unsigned)(63 - ls));
 #pragma unroll
         for (int off = SEG / 2; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off, SEG));
-        const int bl = SEG - 1 - (key & ((1 << LB) - 1));
+        const int bl = 63 - (key & 63);
         const int src = sbase + bl;
         const double pv = seg_bcast<SEG>(row[j], src);
         bad |= !(fabs(pv) > thr);
@@ -887,20 +890,35 @@ int grid_for(int64_t n, int threads, int cap) {
   return (int)g;
 }
 
-template <int K>
-int condense_launch(const CondenseArgs& a, bool blocks, hipStream_t s) {
-  using D = Dims<K>;
+template <int K, int SEG>
+int condense_launch_seg(const CondenseArgs& a, bool blocks, int grid_cap, hipStream_t s) {
   const int64_t n = blocks ? a.count : a.mesh.nc;
-  const int64_t per_block = 4 * D::EPW;
-  const int grid = grid_for(n, (int)per_block, 8192);
+  const int64_t per_block = 4 * (64 / SEG);
+  const int grid = grid_for(n, (int)per_block, grid_cap > 0 ? grid_cap : 8192);
   const size_t lds = sizeof(double) * a.lay.total;
   if (blocks)
-    hipLaunchKernelGGL((condense_kernel<K, true, false>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, true, false, SEG>), dim3(grid), dim3(256), lds, s, a);
   else if (a.srcs)
-    hipLaunchKernelGGL((condense_kernel<K, false, true>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, false, true, SEG>), dim3(grid), dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL((condense_kernel<K, false, false>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, false, false, SEG>), dim3(grid), dim3(256), lds, s, a);
   return (int)hipGetLastError();
+}
+
+// seg 0 picks the narrowest segment that holds the rows
+template <int K>
+int condense_launch(const CondenseArgs& a, bool blocks, const CondenseSched& sc, hipStream_t s) {
+  constexpr int NAT = Dims<K>::SEG;
+  const int seg = sc.seg > 0 ? sc.seg : NAT;
+  if (seg < NAT) return (int)hipErrorInvalidValue;
+  if constexpr (NAT <= 16) {
+    if (seg == 16) return condense_launch_seg<K, 16>(a, blocks, sc.grid, s);
+  }
+  if constexpr (NAT <= 32) {
+    if (seg == 32) return condense_launch_seg<K, 32>(a, blocks, sc.grid, s);
+  }
+  if (seg == 64) return condense_launch_seg<K, 64>(a, blocks, sc.grid, s);
+  return (int)hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -917,13 +935,65 @@ int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
-int launch_condense(int k, const CondenseArgs& a, bool blocks, hipStream_t s) {
+int launch_condense(int k, const CondenseArgs& a, bool blocks, hipStream_t s, const CondenseSched& sc) {
   switch (k) {
-    case 1: return condense_launch<1>(a, blocks, s);
-    case 2: return condense_launch<2>(a, blocks, s);
-    case 3: return condense_launch<3>(a, blocks, s);
-    case 4: return condense_launch<4>(a, blocks, s);
+    case 1: return condense_launch<1>(a, blocks, sc, s);
+    case 2: return condense_launch<2>(a, blocks, sc, s);
+    case 3: return condense_launch<3>(a, blocks, sc, s);
+    case 4: return condense_launch<4>(a, blocks, sc, s);
     default: return (int)hipErrorInvalidValue;
+  }
+}
+
+int tune_condense(int k, const CondenseArgs& a, hipStream_t s, CondenseSched* best, double* best_us,
+                  double* default_us) {
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return (int)e;
+  e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return (int)e;
+  }
+  const int grids[] = {512, 1024, 2048, 4096, 8192, 16384};
+  constexpr int kReps = 3;
+  double bt = 1e300, dt = 0.0;
+  CondenseSched bs;
+  int rc = 0;
+  for (int seg = condense_native_seg(k); seg <= 64 && rc == 0; seg *= 2)
+    for (int g : grids) {
+      const CondenseSched sc{seg, g};
+      rc = launch_condense(k, a, false, s, sc);  // warm-up
+      if (rc) break;
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < kReps && rc == 0; ++r) rc = launch_condense(k, a, false, s, sc);
+      (void)hipEventRecord(e1, s);
+      if (rc || (rc = (int)hipEventSynchronize(e1))) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      const double us = 1e3 * ms / kReps;
+      if (us < bt) {
+        bt = us;
+        bs = sc;
+      }
+      if (seg == condense_native_seg(k) && g == 8192) dt = us;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *best = bs;
+  if (best_us) *best_us = bt;
+  if (default_us) *default_us = dt;
+  return 0;
+}
+
+int condense_native_seg(int k) {
+  switch (k) {
+    case 1: return Dims<1>::SEG;
+    case 2: return Dims<2>::SEG;
+    case 3: return Dims<3>::SEG;
+    case 4: return Dims<4>::SEG;
+    default: return 64;
   }
 }
 

@@ -68,7 +68,7 @@ enum { HDG_PC_AUTO = 0, HDG_PC_BLOCK_JACOBI = 1, HDG_PC_MULTIGRID = 2 };
    iterations is captured once as a HIP graph and replayed (capture costs
    more than one solve saves); HDG_SCHEDULE_EAGER never captures.  Same
    results, bit for bit. */
-enum { HDG_SCHEDULE_EAGER = 1 };
+enum { HDG_SCHEDULE_EAGER = 1, HDG_SCHEDULE_AUTOTUNE = 2 };
 
 /* Sources.  HDG_SRC_REFERENCE is the reference's f = 2 pi^2 sin(pi x) sin(pi y)
    (examples/poisson2D_HDG.jl:55).  HDG_SRC_VALUES takes f already evaluated
@@ -209,6 +209,29 @@ int hdg_dubiner(int k, double r, double s, double* val, double* grad);
    -HDG_ERR_QUAD; pts holds 2 (or 1) coordinates per point; cap = capacity
    in points. */
 int hdg_quadrature(int dim, int order, int cap, double* pts, double* w);
+
+/* ---- autotuning ----------------------------------------------------------- */
+/* No counterpart in the reference.  The condensation kernel runs one element
+   per segment of 16, 32 or 64 lanes (any width that holds the element's
+   rows) over a grid of blocks that stride over the cells.  hdg_autotune
+   times every (width, grid cap) pair on the current problem with HIP events
+   and keeps the fastest for this context until the next hdg_setup
+   (HDG_SCHEDULE_AUTOTUNE in hdg_options.schedule does the same at the first
+   assembly after hdg_setup, and for every group of a batch at
+   hdg_batch_create).  Results do not depend on the schedule, bit for bit.
+   Needs hdg_setup and hdg_set_source; overwrites the assembled operators
+   (assemble again before solving). */
+typedef struct {
+  int32_t condense_seg;   /* lanes per element */
+  int32_t condense_grid;  /* grid cap, blocks of 256 threads */
+  double condense_us;     /* measured duration of the chosen schedule */
+  double default_us;      /* of the default (narrowest width, 8192 blocks) */
+} hdg_schedule;
+int hdg_autotune(hdg_ctx* ctx, hdg_schedule* out);
+/* Sets the condensation schedule (condense_seg 0 = narrowest width,
+   condense_grid 0 = 8192; HDG_ERR_ARG for a width below the element's rows
+   or not 16/32/64) until the next hdg_setup; e.g. a schedule tuned before. */
+int hdg_set_schedule(hdg_ctx* ctx, const hdg_schedule* sched);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Average device duration (microseconds) of one launch of an internal

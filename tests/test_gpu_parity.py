@@ -470,3 +470,40 @@ def test_graph_replay_equals_eager(hdg, ctx, k, n, precision, precond):
         assert it == eager_it
         assert np.array_equal(u, eager_u)
     other.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_condense_schedules_same_bits(hdg, ctx, k):
+    """Every condensation schedule (lanes per element, grid cap) gives the
+    same condensed operators and recovery operators bit for bit, and the
+    autotuned solve equals the default one."""
+    m = hdg.rectangle_mesh((37, 29), jitter=0.2, seed=31)
+    ctx.set_mesh(m)
+    ctx.setup(k, quad_degree=QD[k], rtol=1e-12)
+    ctx.set_source(kind=hdg._lib.SRC_KAPPA, kappa=(2.5, 6.0))
+    ctx.set_dirichlet(m.boundary_dofs(k + 1), np.zeros(len(m.boundary_dofs(k + 1))))
+    ref = None
+    nat = {1: 16, 2: 32, 3: 64, 4: 64}[k]
+    seg = nat
+    while seg <= 64:
+        for grid in (512, 8192):
+            ctx.set_schedule(seg, grid)
+            ctx.assemble_condense()
+            got = ctx.get_condensed() + ctx.get_local_ops()
+            if ref is None:
+                ref = got
+            else:
+                for a, b in zip(ref, got):
+                    assert np.array_equal(a, b), (seg, grid)
+        seg *= 2
+    with pytest.raises(hdg.HDGError):
+        ctx.set_schedule(nat // 2 if nat > 16 else 8, 0)
+    ctx.set_schedule(0, 0)
+    ctx.run()
+    u_default = ctx.trace()
+    sc = ctx.autotune()
+    print(f"k={k}: tuned {sc}", flush=True)
+    assert sc["condense_seg"] in (16, 32, 64) and sc["condense_seg"] >= nat
+    assert 0 < sc["condense_us"] <= sc["default_us"] * 1.0001
+    ctx.run()
+    assert np.array_equal(ctx.trace(), u_default)
