@@ -38,6 +38,7 @@ sys.path.insert(0, REPO)
 
 METRIC = "elements/s (assemble+condense+solve), p50/p99 latency vs FP64 plain, same L2"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and MFMA)
 SEED = 20261202
 # name: (cells per side, k, quad_degree, description)
 WORKLOADS = {
@@ -57,6 +58,22 @@ def spmv_bytes(ctx):
     nblocks = s.nnz // (nt * nt)
     N = s.n_dofs
     return 8 * s.nnz + 4 * nblocks + 16 * N + 16 * N + N
+
+
+def condense_work(ctx):
+    """Algorithmic bytes and FP64 flops of one condensation launch.  Bytes:
+    per cell 3 node ids + 3 face ids + 3 node coordinates read (72 B) and
+    Ate (ntr^2), bte (ntr), K_e (m ntr), b_e (m) written.  Flops: the
+    Gauss-Jordan elimination of the bordered system, 2 per fma: at pivot
+    step j every other row updates its columns j+1 .. m+ntr (rows = m + ntr),
+    i.e. 2 (rows - 1) sum_j (m + ntr - j) over j < m; building the rows and
+    the quadrature sums are not counted."""
+    s = ctx.sizes
+    m, ntr, nc = s.m, s.ntr, s.n_cells
+    rows = m + ntr
+    fma = (rows - 1) * sum(m + ntr - j for j in range(m))
+    byts = nc * (72 + 8 * (ntr * ntr + ntr + m * ntr + m))
+    return byts, 2 * fma * nc
 
 
 def measured_traffic(workload):
@@ -288,7 +305,7 @@ def main():
                     help="untimed steps (the first solve after a set-up runs eagerly, the second captures "
                          "the CG graphs that later steps replay)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + list(STREAM_WORKLOADS), default="p2-512")
-    ap.add_argument("--slots", type=int, default=8, help="batched/serving: concurrent solves")
+    ap.add_argument("--slots", type=int, default=16, help="serving / --via-server: concurrent solves")
     ap.add_argument("--hbm-gib", type=float, default=64.0, help="batched/serving: device-memory budget")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--seconds", type=float, default=10.0, help="serving: length of the arrival stream")
@@ -348,6 +365,8 @@ def main():
 
     # dominant kernel: one CG trace-SpMV launch, HIP events on ctx's stream
     spmv_us = ctx.time_kernel(0, reps=200)
+    cond_us = ctx.time_kernel(2, reps=5)
+    cond_bytes, cond_flops = condense_work(ctx)
     ab = spmv_bytes(ctx)
     achieved = ab / (spmv_us * 1e-6) / 1e9
     copy_us = ctx.time_kernel(3, reps=3) if args.calibrate else None
@@ -380,6 +399,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"cg_spmv_kernel<{nt}>", "avg_us": spmv_us, "bytes_per_launch": int(ab)},
+        # the condensation kernel against both ceilings: it is bound by neither (the pivot
+        # chain's cross-lane latency bounds it); FP64 dense MFMA peaks at the FP64 vector rate
+        "condense_roofline": {"kernel": f"condense_kernel<{k}>", "avg_us": cond_us,
+                              "bytes_per_launch": int(cond_bytes), "flops_per_launch": int(cond_flops),
+                              "achieved_GBps": cond_bytes / (cond_us * 1e-6) / 1e9,
+                              "hbm_frac": cond_bytes / (cond_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                              "achieved_TFLOPs_fp64": cond_flops / (cond_us * 1e-6) / 1e12,
+                              "fp64_frac": cond_flops / (cond_us * 1e-6) / 1e12 / FP64_PEAK_TFLOPS},
     }
     if copy_us is not None:
         copy_bytes = 2 * 8 * 5 * nt * ndofs

@@ -34,7 +34,8 @@ EXPORTS = (
     "hdg_server_create", "hdg_server_submit", "hdg_server_wait", "hdg_server_trace", "hdg_server_footprint",
     "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error", "hdg_server_set_batching",
     "hdg_batch_create", "hdg_batch_run", "hdg_batch_trace", "hdg_batch_info", "hdg_batch_destroy",
-    "hdg_batch_last_error", "hdg_autotune", "hdg_set_schedule",
+    "hdg_batch_last_error", "hdg_autotune", "hdg_set_schedule", "hdg_set_rectangle_mesh",
+    "hdg_set_dirichlet_boundary",
 )
 
 
@@ -159,6 +160,9 @@ def _declare(L):
         "hdg_server_last_error": (C.c_char_p, [C.c_void_p]),
         "hdg_server_set_batching": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
         "hdg_autotune": (C.c_int, [C.c_void_p, _P(Schedule)]),
+        "hdg_set_rectangle_mesh": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, C.c_uint64]),
+        "hdg_set_dirichlet_boundary": (C.c_int, [C.c_void_p, C.c_double]),
         "hdg_set_schedule": (C.c_int, [C.c_void_p, _P(Schedule)]),
         "hdg_batch_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int, _P(Request)]),
         "hdg_batch_run": (C.c_int, [C.c_void_p, _P(Result)]),

@@ -48,6 +48,16 @@ class Context:
         return n.value
 
     # -- set-up
+    def set_rectangle_mesh(self, nx, ny, ll=(0.0, 0.0), ur=(1.0, 1.0), jitter=0.0, seed=0):
+        """rectangle_mesh generated on the device (hdg_set_rectangle_mesh):
+        the mesh rectangle_mesh((nx, ny), ...) builds, without host arrays."""
+        self._chk(lib().hdg_set_rectangle_mesh(self._h, int(nx), int(ny), float(ll[0]), float(ll[1]),
+                                               float(ur[0]), float(ur[1]), float(jitter), int(seed)))
+
+    def set_dirichlet_boundary(self, value=0.0):
+        """u = value on every dof of every boundary face."""
+        self._chk(lib().hdg_set_dirichlet_boundary(self._h, float(value)))
+
     def set_mesh(self, mesh):
         faces_cm = np.asfortranarray(mesh.faces, dtype=np.int64)
         self._chk(lib().hdg_set_mesh(self._h, mesh.nodes.shape[0], ptr(mesh.nodes), mesh.cells.shape[0],

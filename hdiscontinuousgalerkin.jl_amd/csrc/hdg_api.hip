@@ -707,6 +707,93 @@ int hdg_set_mesh(hdg_ctx* c, int64_t nn, const double* xy, int64_t nc, const int
   return HDG_OK;
 }
 
+int hdg_set_rectangle_mesh(hdg_ctx* c, int64_t nx, int64_t ny, double x0, double y0, double x1, double y1,
+                           double jitter, uint64_t seed) {
+  if (!c) return HDG_ERR_ARG;
+  if (nx < 1 || ny < 1) return fail(c, HDG_ERR_ARG, "rectangle mesh needs nx, ny >= 1");
+  if (!(x1 > x0) || !(y1 > y0)) return fail(c, HDG_ERR_ARG, "the box must be positively oriented");
+  const int64_t nn = (nx + 1) * (ny + 1), nc = 2 * nx * ny, nf = hdg_rectangle_nfaces(nx, ny);
+  if (nn > INT32_MAX || 2 * nf > INT32_MAX || nc > INT32_MAX / 9)
+    return fail(c, HDG_ERR_ARG, "mesh too large for 32-bit device indices");
+  TRY(select_device(c));
+  free_mesh(c);
+  double* dxy;
+  int *dcn, *dcf, *dfc;
+  TRY(dalloc(c, &dxy, 2 * nn));
+  TRY(dalloc(c, &dcn, 3 * nc));
+  TRY(dalloc(c, &dcf, 3 * nc));
+  TRY(dalloc(c, &dfc, 2 * nf));
+  MgData& M = c->mg;
+  TRY(dalloc(c, &M.fv, 2 * nf));
+  TRY(dalloc(c, &M.vf_ptr, nn + 1));
+  TRY(dalloc(c, &M.vf_idx, 2 * nf));
+  TRY(dalloc(c, &M.fq4, 4 * nf));
+  HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  RectMeshArgs a;
+  a.nx = nx;
+  a.ny = ny;
+  a.nf = nf;
+  a.x0 = x0;
+  a.y0 = y0;
+  a.x1 = x1;
+  a.y1 = y1;
+  a.jitter = jitter;
+  a.seed = seed;
+  a.xy = dxy;
+  a.cn = dcn;
+  a.cf = dcf;
+  a.fcell = dfc;
+  a.fv = M.fv;
+  a.vf_ptr = M.vf_ptr;
+  a.vf_idx = M.vf_idx;
+  a.fq4 = M.fq4;
+  a.err = c->d_err;
+  LAUNCHCHK(c, launch_rect_mesh(a, c->stream));
+  c->mesh.nn = nn;
+  c->mesh.nc = nc;
+  c->mesh.nf = nf;
+  c->mesh.xy = dxy;
+  c->mesh.cn = dcn;
+  c->mesh.cf = dcf;
+  c->mesh.fcell = dfc;
+  TRY(dalloc(c, &c->pat.bcol, kSlots * nf));
+  TRY(dalloc(c, &c->pat.src, 2 * kSlots * nf));
+  TRY(dalloc(c, &c->pat.nslot, nf));
+  TRY(dalloc(c, &c->pat.self, nf));
+  TRY(dalloc(c, &c->pat.tslot, kSlots * nf));
+  LAUNCHCHK(c, launch_pattern(c->mesh, c->pat, c->stream));
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flag & 4) {
+    free_mesh(c);
+    return fail(c, HDG_ERR_BOUNDARY, "a boundary face lies on no side of the box (generate_mesh.jl:80)");
+  }
+  c->h_cf.clear();  // downloaded when an export needs it
+  const int64_t nbdry = 2 * nx + 2 * ny;
+  c->nblock_cols = 5 * (nf - nbdry) + 3 * nbdry;
+  c->structured = true;
+  M.nv = nn;
+  mg_plan_levels((int)nx, (int)ny, M);
+  for (int l = 1; l < M.nlev; ++l) {
+    MgLevel& L = M.lev[l];
+    TRY(dalloc(c, &L.a, kStencil * L.n));
+    TRY(dalloc(c, &L.dinv, L.n));
+    TRY(dalloc(c, &L.mask, L.n));
+    TRY(dalloc(c, &L.r, L.n));
+    TRY(dalloc(c, &L.e, L.n));
+    TRY(dalloc(c, &L.t, L.n));
+  }
+  TRY(dalloc(c, &M.cinv, (size_t)kCoarseDirect * kCoarseDirect));
+  if (M.tail < M.nlev) {
+    int64_t tn = 0;
+    for (int t = M.tail; t < M.nlev; ++t) tn += M.lev[t].n;
+    TRY(dalloc(c, &M.tailpack, (kStencil + 1) * tn));
+  }
+  c->has_mesh = true;
+  return HDG_OK;
+}
+
 int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   if (!c || !opt) return HDG_ERR_ARG;
   if (!c->has_mesh) return fail(c, HDG_ERR_STATE, "hdg_set_mesh has not been called");
@@ -843,6 +930,17 @@ int hdg_set_dirichlet(hdg_ctx* c, int64_t n, const int64_t* dofs, const double* 
   return HDG_OK;
 }
 
+int hdg_set_dirichlet_boundary(hdg_ctx* c, double value) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  TRY(select_device(c));
+  LAUNCHCHK(c, launch_boundary_dirichlet(c->mesh.fcell, 0, c->mesh.nf, c->nt, c->d_dmask, c->d_uD, value,
+                                         c->stream));
+  c->jacobi_dirty = true;
+  c->solved = c->recovered = false;
+  return HDG_OK;
+}
+
 int hdg_query_sizes(const hdg_ctx* c, hdg_sizes* s) {
   if (!c || !s) return HDG_ERR_ARG;
   std::memset(s, 0, sizeof(*s));
@@ -920,6 +1018,10 @@ int hdg_recover(hdg_ctx* c, double* q, double* u, double* uhat_cell) {
     }
   }
   if (uhat_cell) {
+    if ((int64_t)c->h_cf.size() != 3 * nc) {
+      c->h_cf.resize(3 * nc);
+      HIPCHK(c, hipMemcpyAsync(c->h_cf.data(), c->mesh.cf, sizeof(int) * 3 * nc, hipMemcpyDeviceToHost, c->stream));
+    }
     std::vector<double> uh(c->T.N);
     HIPCHK(c, hipMemcpyAsync(uh.data(), c->d_uhat, sizeof(double) * uh.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -10,7 +10,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -70,6 +69,7 @@ struct Group {
   int* err = nullptr;  // [ninst] condensation error flags
   SrcParams* srcs = nullptr;
   int* cell_inst = nullptr;
+  int *fv = nullptr, *vf_ptr = nullptr, *vf_idx = nullptr, *fq4 = nullptr;  // face end points, vertex -> faces
   // launch tables
   CgState* st = nullptr;
   Seg *segs = nullptr, *csegs = nullptr;
@@ -201,53 +201,6 @@ SrcParams src_params(const hdg_source& s) {
   return p;
 }
 
-// Host arrays of a group's packed mesh, filled instance by instance from
-// several threads.
-struct PackedHost {
-  std::vector<double> xy;
-  std::vector<int> cn, cf, fcell, fv, vf_ptr, vf_idx, fq4, cell_inst;
-  std::vector<unsigned char> dmask;
-};
-
-// Generates instance j's rectangle mesh and writes it, shifted by the
-// instance's offsets, into the packed arrays.
-bool pack_instance(const hdg_request& q, const Group& g, int j, PackedHost& H) {
-  const Inst& I = g.in[j];
-  const int64_t nn = I.nn, nc = I.nc, nf = I.nf;
-  std::vector<double> xy(2 * nn);
-  std::vector<int64_t> cells(6 * nc), faces(4 * nf);
-  std::vector<int8_t> side(nf);
-  if (hdg_rectangle_mesh(I.nx, I.ny, 0.0, 0.0, 1.0, 1.0, q.jitter, q.seed, xy.data(), cells.data(), faces.data(),
-                         side.data()) != HDG_OK)
-    return false;
-  std::vector<int> fv, ptr, idx, fq4;
-  if (!face_vertex_maps(nn, nf, faces.data(), cells.data(), fv, ptr, idx, g.mg ? &fq4 : nullptr)) return false;
-  std::copy(xy.begin(), xy.end(), H.xy.begin() + 2 * I.node0);
-  for (int64_t e = 0; e < nc; ++e) {
-    for (int l = 0; l < 3; ++l) {
-      H.cn[3 * (I.cell0 + e) + l] = (int)(I.node0 + cells[6 * e + l] - 1);
-      H.cf[3 * (I.cell0 + e) + l] = (int)(I.face0 + cells[6 * e + 3 + l] - 1);
-    }
-    H.cell_inst[I.cell0 + e] = j;
-  }
-  for (int64_t f = 0; f < nf; ++f) {
-    const int64_t c1 = faces[f + 2 * nf], c2 = faces[f + 3 * nf];
-    H.fcell[2 * (I.face0 + f)] = (int)(I.cell0 + c1 - 1);
-    H.fcell[2 * (I.face0 + f) + 1] = c2 > 0 ? (int)(I.cell0 + c2 - 1) : -1;
-    H.fv[2 * (I.face0 + f)] = (int)(I.node0 + fv[2 * f]);
-    H.fv[2 * (I.face0 + f) + 1] = (int)(I.node0 + fv[2 * f + 1]);
-    if (g.mg)
-      for (int u = 0; u < 4; ++u) H.fq4[4 * (I.face0 + f) + u] = (int)(I.node0 + fq4[4 * f + u]);
-    for (int t = 0; t < g.nt; ++t) H.dmask[I.row0 + f * g.nt + t] = side[f] ? 1 : 0;
-  }
-  // vertex -> face lists: every face has two end points, so instance j's
-  // entries start at 2 * face0
-  for (int64_t v = 0; v < nn; ++v) H.vf_ptr[I.node0 + v] = (int)(2 * I.face0 + ptr[v]);
-  if (j + 1 == (int)g.in.size()) H.vf_ptr[I.node0 + nn] = (int)(2 * I.face0 + ptr[nn]);
-  for (size_t u = 0; u < idx.size(); ++u) H.vf_idx[2 * I.face0 + u] = (int)(I.face0 + idx[u]);
-  return true;
-}
-
 CondenseArgs group_condense_args(const Group& g) {
   CondenseArgs a;
   a.mesh = g.mesh;
@@ -309,39 +262,55 @@ int build_group(hdg_batch* b, Group& g) {
   BHIP(b, hipHostMalloc((void**)&g.h_st, 3 * (size_t)n * sizeof(CgState), hipHostMallocDefault));
   BHIP(b, hipHostMalloc((void**)&g.h_res, (3 * (size_t)n + 1) * sizeof(double), hipHostMallocDefault));
 
-  // host packing, in parallel over the instances
-  PackedHost H;
-  H.xy.assign(2 * g.nn, 0.0);
-  H.cn.assign(3 * g.nc, 0);
-  H.cf.assign(3 * g.nc, 0);
-  H.cell_inst.assign(g.nc, 0);
-  H.fcell.assign(2 * g.nf, 0);
-  H.fv.assign(2 * g.nf, 0);
-  H.vf_ptr.assign(g.nn + 1, 0);
-  H.vf_idx.assign(2 * g.nf, 0);
-  if (g.mg) H.fq4.assign(4 * g.nf, 0);
-  H.dmask.assign(g.N, 0);
-  {
-    std::atomic<int> next{0}, bad{0};
-    const int nth = std::max(1, std::min<int>(16, std::min<int>(n, (int)std::thread::hardware_concurrency())));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; ++t)
-      th.emplace_back([&] {
-        for (int j = next++; j < n; j = next++)
-          if (!pack_instance(b->reqs[g.in[j].req], g, j, H)) bad++;
-      });
-    for (auto& t : th) t.join();
-    if (bad) return fail(b, HDG_ERR_ARG, "mesh generation failed for a request of order %d", g.k);
-  }
-  // packed mesh, pattern, trace system
+  // packed mesh, generated on the device instance by instance at its offsets
   double* dxy;
-  int *dcn, *dcf, *dfc;
-  BTRY(upload(b, g, &dxy, H.xy));
-  BTRY(upload(b, g, &dcn, H.cn));
-  BTRY(upload(b, g, &dcf, H.cf));
-  BTRY(upload(b, g, &dfc, H.fcell));
-  BTRY(upload(b, g, &g.cell_inst, H.cell_inst));
-  BTRY(upload(b, g, &g.dmask, H.dmask));
+  int *dcn, *dcf, *dfc, *merr;
+  BTRY(galloc(b, g, &dxy, 2 * g.nn));
+  BTRY(galloc(b, g, &dcn, 3 * g.nc));
+  BTRY(galloc(b, g, &dcf, 3 * g.nc));
+  BTRY(galloc(b, g, &dfc, 2 * g.nf));
+  BTRY(galloc(b, g, &g.cell_inst, g.nc));
+  BTRY(galloc(b, g, &g.dmask, g.N));
+  BTRY(galloc(b, g, &g.fv, 2 * g.nf));
+  BTRY(galloc(b, g, &g.vf_ptr, g.nn + 1));
+  BTRY(galloc(b, g, &g.vf_idx, 2 * g.nf));
+  if (g.mg) BTRY(galloc(b, g, &g.fq4, 4 * g.nf));
+  BTRY(galloc(b, g, &merr, 1));
+  BHIP(b, hipMemsetAsync(merr, 0, sizeof(int), g.s));
+  for (int j = 0; j < n; ++j) {
+    const Inst& I = g.in[j];
+    const hdg_request& q = b->reqs[I.req];
+    RectMeshArgs a;
+    a.nx = I.nx;
+    a.ny = I.ny;
+    a.nf = I.nf;
+    a.jitter = q.jitter;
+    a.seed = q.seed;
+    a.node0 = I.node0;
+    a.cell0 = I.cell0;
+    a.face0 = I.face0;
+    a.inst = j;
+    a.last = j + 1 == n;
+    a.xy = dxy;
+    a.cn = dcn;
+    a.cf = dcf;
+    a.fcell = dfc;
+    a.fv = g.fv;
+    a.vf_ptr = g.vf_ptr;
+    a.vf_idx = g.vf_idx;
+    a.fq4 = g.fq4;
+    a.cell_inst = g.cell_inst;
+    a.err = merr;
+    BLAUNCH(b, launch_rect_mesh(a, g.s));
+  }
+  // u = 0 on every boundary face of every instance
+  BLAUNCH(b, launch_boundary_dirichlet(dfc, 0, g.nf, g.nt, g.dmask, nullptr, 0.0, g.s));
+  {
+    int flag = 0;
+    BHIP(b, hipMemcpyAsync(&flag, merr, sizeof(int), hipMemcpyDeviceToHost, g.s));
+    BHIP(b, hipStreamSynchronize(g.s));
+    if (flag) return fail(b, HDG_ERR_BOUNDARY, "a boundary face lies on no side of the box (order %d group)", g.k);
+  }
   g.mesh = DevMesh{g.nn, g.nc, g.nf, dxy, dcn, dcf, dfc};
   BTRY(galloc(b, g, &g.pat.bcol, kSlots * g.nf));
   BTRY(galloc(b, g, &g.pat.src, 2 * kSlots * g.nf));
@@ -446,18 +415,13 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &cellA, 9 * nc));
   BTRY(galloc(b, g, &cinv, (size_t)n * kCoarseDirect * kCoarseDirect));
   BTRY(galloc(b, g, &tailpack, tailn));
-  int *dfv, *dvp, *dvi, *dfq4;
-  BTRY(upload(b, g, &dfv, H.fv));
-  BTRY(upload(b, g, &dvp, H.vf_ptr));
-  BTRY(upload(b, g, &dvi, H.vf_idx));
-  BTRY(upload(b, g, &dfq4, H.fq4));
   MgData& U = g.Mu;
   U.nlev = 2;
   U.nv = g.nn;
-  U.fv = dfv;
-  U.vf_ptr = dvp;
-  U.vf_idx = dvi;
-  U.fq4 = dfq4;
+  U.fv = g.fv;
+  U.vf_ptr = g.vf_ptr;
+  U.vf_idx = g.vf_idx;
+  U.fq4 = g.fq4;
   U.cellA = cellA;
   {
     MgLevel& L1 = U.lev[1];

@@ -369,4 +369,25 @@ int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Kryl
 // float copy of n doubles
 int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
 
+// rectangle_mesh generated on the device into (possibly packed) arrays:
+// nodes, cells and faces land at node0 / cell0 / face0; ids written are
+// global (shifted); cell_inst (optional) receives inst for every cell;
+// vf_ptr / vf_idx / fq4 optional; err bit 4: a one-cell face off the box.
+struct RectMeshArgs {
+  int64_t nx = 0, ny = 0, nf = 0;
+  double x0 = 0, y0 = 0, x1 = 1, y1 = 1, jitter = 0;
+  uint64_t seed = 0;
+  int64_t node0 = 0, cell0 = 0, face0 = 0;
+  int inst = 0;
+  bool last = true;  // writes the closing entry of vf_ptr
+  double* xy = nullptr;
+  int *cn = nullptr, *cf = nullptr, *fcell = nullptr, *fv = nullptr;
+  int *vf_ptr = nullptr, *vf_idx = nullptr, *fq4 = nullptr, *cell_inst = nullptr;
+  int* err = nullptr;
+};
+int launch_rect_mesh(const RectMeshArgs& a, hipStream_t s);
+// dmask / uD (uD may be null) of the faces [face0, face0 + nf): u = value on faces with one cell
+int launch_boundary_dirichlet(const int* fcell, int64_t face0, int64_t nf, int nt, unsigned char* dmask, double* uD,
+                              double value, hipStream_t s);
+
 }  // namespace hdg

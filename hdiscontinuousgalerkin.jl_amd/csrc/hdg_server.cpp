@@ -43,7 +43,6 @@ struct Slot {
   uint64_t seed = 0;
   int8_t nt = 0;
   hdg_options opt{};
-  std::vector<int64_t> bfaces;  // 0-based boundary faces of the mesh
 };
 
 bool same_options(const hdg_options& a, const hdg_options& b) {
@@ -99,21 +98,12 @@ hdg_result run_request(hdg_server* S, Slot& sl, const Pending& p, std::vector<do
                         sl.seed == q.seed;
   if (st == HDG_OK && !mesh_hit) {
     sl.has_mesh = sl.has_setup = false;
-    const int64_t nx = q.nx, ny = q.ny, nn = (nx + 1) * (ny + 1), nc = 2 * nx * ny;
-    const int64_t nf = hdg_rectangle_nfaces(nx, ny);
-    std::vector<double> xy(2 * nn);
-    std::vector<int64_t> cells(6 * nc), faces(4 * nf);
-    std::vector<int8_t> side(nf);
-    st = hdg_rectangle_mesh(nx, ny, 0.0, 0.0, 1.0, 1.0, q.jitter, q.seed, xy.data(), cells.data(), faces.data(),
-                            side.data());
-    if (st == HDG_OK) st = hdg_set_mesh(sl.ctx, nn, xy.data(), nc, cells.data(), nf, faces.data());
+    // generated on the device: no host mesh arrays, no upload
+    st = hdg_set_rectangle_mesh(sl.ctx, q.nx, q.ny, 0.0, 0.0, 1.0, 1.0, q.jitter, q.seed);
     if (st == HDG_OK) {
-      sl.bfaces.clear();
-      for (int64_t f = 0; f < nf; ++f)
-        if (side[f]) sl.bfaces.push_back(f);
       sl.has_mesh = true;
-      sl.nx = nx;
-      sl.ny = ny;
+      sl.nx = q.nx;
+      sl.ny = q.ny;
       sl.jitter = q.jitter;
       sl.seed = q.seed;
     }
@@ -129,17 +119,8 @@ hdg_result run_request(hdg_server* S, Slot& sl, const Pending& p, std::vector<do
   hdg_sizes sz{};
   if (st == HDG_OK) st = hdg_query_sizes(sl.ctx, &sz);
   if (st == HDG_OK) st = hdg_set_source(sl.ctx, &q.src);
-  if (st == HDG_OK) {
-    // u = 0 on the boundary (every serving problem): all trace dofs of the
-    // boundary faces, 1-based
-    const int nt = sz.nt;
-    std::vector<int64_t> dofs;
-    dofs.reserve(sl.bfaces.size() * nt);
-    for (int64_t f : sl.bfaces)
-      for (int j = 0; j < nt; ++j) dofs.push_back(f * nt + j + 1);
-    std::vector<double> zero(dofs.size(), 0.0);
-    st = hdg_set_dirichlet(sl.ctx, (int64_t)dofs.size(), dofs.data(), zero.data());
-  }
+  // u = 0 on the boundary (every serving problem)
+  if (st == HDG_OK) st = hdg_set_dirichlet_boundary(sl.ctx, 0.0);
   hdg_solve_info info{};
   if (st == HDG_OK) st = hdg_run(sl.ctx, &info);
   if (st == HDG_OK && q.uex_kind >= 0) st = hdg_errornorm(sl.ctx, q.uex_kind, nullptr, &r.errornorm);

@@ -140,6 +140,12 @@ int hdg_rectangle_mesh(int64_t nx, int64_t ny, double x0, double y0, double x1,
                        double y1, double jitter, uint64_t seed, double* xy,
                        int64_t* cells, int64_t* faces, int8_t* face_side);
 
+/* hdg_rectangle_mesh + hdg_set_mesh in one step, generated on the device:
+   the same mesh, numbering and node coordinates bit for bit (so the same
+   solution), without host arrays or their upload. */
+int hdg_set_rectangle_mesh(hdg_ctx* ctx, int64_t nx, int64_t ny, double x0, double y0, double x1, double y1,
+                           double jitter, uint64_t seed);
+
 /* ---- problem set-up ----------------------------------------------------- */
 /* Uploads a PolygonalMesh (src/mesh.jl:43-49) and builds the face-level
    sparsity pattern of the trace matrix on the device. */
@@ -156,6 +162,10 @@ int hdg_set_source(hdg_ctx* ctx, const hdg_source* src);
    trace dofs (1-based) and values.  n = 0 clears. */
 int hdg_set_dirichlet(hdg_ctx* ctx, int64_t n, const int64_t* dofs,
                       const double* vals);
+/* u = value on every trace dof of every boundary face (a face with one
+   cell), free elsewhere; the device-side form of hdg_set_dirichlet for a
+   constant on the whole boundary. */
+int hdg_set_dirichlet_boundary(hdg_ctx* ctx, double value);
 int hdg_query_sizes(const hdg_ctx* ctx, hdg_sizes* sizes);
 
 /* ---- the pipeline -------------------------------------------------------- */

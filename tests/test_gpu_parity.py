@@ -507,3 +507,42 @@ def test_condense_schedules_same_bits(hdg, ctx, k):
     assert 0 < sc["condense_us"] <= sc["default_us"] * 1.0001
     ctx.run()
     assert np.array_equal(ctx.trace(), u_default)
+
+
+@pytest.mark.parametrize("nx,ny,k,box", [(37, 29, 2, ((0.0, 0.0), (1.0, 1.0))), (16, 16, 1, ((0.0, 0.0), (1.0, 1.0))),
+                                         (64, 33, 3, ((-1.0, 0.5), (2.0, 3.25))), (1, 1, 1, ((0.0, 0.0), (1.0, 1.0))),
+                                         (5, 100, 4, ((0.0, 0.0), (1.0, 1.0)))])
+def test_device_rectangle_mesh_same_bits(hdg, ctx, nx, ny, k, box):
+    """The mesh generated on the device (hdg_set_rectangle_mesh) with u = 0
+    on the boundary (hdg_set_dirichlet_boundary) gives the trace system,
+    solution, recovery and error norm of the host-generated mesh with
+    host-listed Dirichlet dofs, bit for bit."""
+    m = hdg.rectangle_mesh((nx, ny), LL=box[0], UR=box[1], jitter=0.2, seed=77)
+    out = []
+    for dev in (False, True):
+        if dev:
+            ctx.set_rectangle_mesh(nx, ny, ll=box[0], ur=box[1], jitter=0.2, seed=77)
+        else:
+            ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=QD[k], rtol=1e-12)
+        ctx.set_source(kind=hdg._lib.SRC_KAPPA, kappa=(2.5, 6.0))
+        if dev:
+            ctx.set_dirichlet_boundary(0.0)
+        else:
+            d = m.boundary_dofs(k + 1)
+            ctx.set_dirichlet(d, np.zeros(len(d)))
+        info = ctx.run()
+        colptr, rowval, nzval, rhs = ctx.get_system()
+        q, u, uc = ctx.recover()
+        out.append((ctx.trace(), info.iters, ctx.errornorm(), colptr, rowval, nzval, rhs, q, u, uc))
+    for a, b in zip(*out):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_device_rectangle_mesh_boundary_error(hdg, ctx):
+    """A box whose interpolated edge nodes miss its sides fails on the
+    device as on the host (HDG_ERR_BOUNDARY, generate_mesh.jl:80)."""
+    with pytest.raises(hdg.ArgumentError, match="BOUNDARY"):
+        hdg.rectangle_mesh((64, 33), LL=(-2.0, -1.0), UR=(3.0, 1.5), jitter=0.2, seed=77)
+    with pytest.raises(hdg.ArgumentError, match="BOUNDARY"):
+        ctx.set_rectangle_mesh(64, 33, ll=(-2.0, -1.0), ur=(3.0, 1.5), jitter=0.2, seed=77)
