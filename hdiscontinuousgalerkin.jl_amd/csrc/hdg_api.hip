@@ -502,14 +502,14 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
     LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, c->f_sp, c->stream));
     LAUNCHCHK(c, launch_mg_power(c->pat, op32(c), c->f_xa, c->f_xb, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
-                                 10, c->stream));
+                                 c->mg.pow_steps, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
     LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
   } else if (use_mg(c)) {
     LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
     // the CG vectors x, r serve as scratch before the solve initialises them
     LAUNCHCHK(c, launch_mg_power(c->pat, op64(c), c->cg.x, c->cg.r, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
-                                 10, c->stream));
+                                 c->mg.pow_steps, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
     LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
   } else {
@@ -816,9 +816,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   free_problem(c);
   c->opt = o;
   c->k = o.k;
-  // vertex-level smoothing: V(1,1) for k <= 2, V(2,2) above (fewest CG
-  // iterations x cost per iteration measured on 256..1024 meshes)
-  c->mg.vsweeps = o.k <= 2 ? 1 : 2;
+  mg_order_choices(o.k, c->mg);
   c->nu = tab.lay.nu;
   c->nq = 2 * c->nu;
   c->m = 3 * c->nu;

@@ -446,14 +446,16 @@ int build_group(hdg_batch* b, Group& g) {
     BTRY(galloc(b, g, &g.f_xa, N));
     BTRY(galloc(b, g, &g.f_xb, N));
   }
-  const int vsweeps = g.k <= 2 ? 1 : 2;  // as hdg_setup chooses for a single problem
+  MgData choice;
+  mg_order_choices(g.k, choice);  // as hdg_setup chooses for a single problem
   g.Mi.assign(n, MgData{});
   for (int j = 0; j < n; ++j) {
     const Inst& I = g.in[j];
     MgData& M = g.Mi[j];
     M = I.plan;
     M.nv = I.nn;
-    M.vsweeps = vsweeps;
+    M.vsweeps = choice.vsweeps;
+    M.pow_steps = choice.pow_steps;
     M.cellA = cellA + 9 * I.cell0;
     M.cinv = cinv + (size_t)j * kCoarseDirect * kCoarseDirect;
     M.tailpack = tailpack + I.tail_off;
@@ -476,7 +478,7 @@ int build_group(hdg_batch* b, Group& g) {
   G.dM = g.dMi;
   G.st = g.st;
   G.ndepth = ndepth;
-  G.vsweeps = vsweeps;
+  G.vsweeps = choice.vsweeps;
   G.omega_v = g.Mi[0].omega_v;
   for (int d = 1; d < ndepth; ++d) {
     std::vector<VRec> rec;
@@ -575,16 +577,16 @@ int enqueue_setup(hdg_batch* b, Group& g) {
     BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)g.nt * g.N, g.s));
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, g.f_sp, g.s));
     BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
-    BLAUNCH(b, launch_mg_power_group(g.pat, op32(g), g.f_xa, g.f_xb, g.cg.part, gr, g.nbt, g.segs, n, factor, 10,
-                                     g.s));
+    BLAUNCH(b, launch_mg_power_group(g.pat, op32(g), g.f_xa, g.f_xb, g.cg.part, gr, g.nbt, g.segs, n, factor,
+                                     g.Mi[0].pow_steps, g.s));
     BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op32(g), kv(g), mv32(g), gr, g.nbt, g.s));
     BLAUNCH(b, launch_mg_precond_group(g.pat, op32(g), kv(g), g.Mu, g.G, mv32(g), gr, g.nbt, g.s));
   } else {
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, nullptr, g.s));
     BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
     // the CG vectors x, r serve as scratch before the solve initialises them
-    BLAUNCH(b, launch_mg_power_group(g.pat, op64(g), g.cg.x, g.cg.r, g.cg.part, gr, g.nbt, g.segs, n, factor, 10,
-                                     g.s));
+    BLAUNCH(b, launch_mg_power_group(g.pat, op64(g), g.cg.x, g.cg.r, g.cg.part, gr, g.nbt, g.segs, n, factor,
+                                     g.Mi[0].pow_steps, g.s));
     BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op64(g), kv(g), mv64(g), gr, g.nbt, g.s));
     BLAUNCH(b, launch_mg_precond_group(g.pat, op64(g), kv(g), g.Mu, g.G, mv64(g), gr, g.nbt, g.s));
   }

@@ -201,7 +201,17 @@ struct MgData {
   double omega_factor = 1.5;     // trace-level weight = factor / lambda_max(Dinv S)
   int coarse_sweeps = 20;
   int vsweeps = 2;               // pre- and post-smoothing sweeps on the vertex levels (1 or 2)
+  int pow_steps = 10;            // power steps behind omega
 };
+// per-order choices of the V-cycle, shared by single problems and batches:
+// V(1,1) on the vertex levels for k <= 2, V(2,2) above; 10 power steps for
+// the trace-level omega at k = 1, 6 above (fewest CG iterations x cost per
+// step measured on 97^2 .. 1024^2 meshes: fewer steps underestimate
+// lambda_max, which helps at k >= 2 and costs 3 iterations at k = 1)
+inline void mg_order_choices(int k, MgData& M) {
+  M.vsweeps = k <= 2 ? 1 : 2;
+  M.pow_steps = k == 1 ? 10 : 6;
+}
 
 // A fine level, the next coarser one and the CG state of one instance: the
 // unit of a vertex-level kernel.  VOne launches one instance over the whole
