@@ -63,10 +63,9 @@ struct hdg_ctx {
   double* d_part2 = nullptr;
 
   // mixed precision (HDG_MIXED): the multigrid preconditioner's trace-level
-  // data in float: trace matrix, Jacobi blocks, S P0, e0, res0, power-step
-  // scratch
+  // data in float: trace matrix, Jacobi blocks, S P0, e0, res0
   float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr;
-  float *f_e0 = nullptr, *f_res0 = nullptr, *f_xa = nullptr, *f_xb = nullptr;
+  float *f_e0 = nullptr, *f_res0 = nullptr;
 
   // captured chunks of CG iterations, one per read-back slot (valid until
   // the next hdg_setup / hdg_set_mesh); solves since the set-up
@@ -184,8 +183,6 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->f_sp);
   dfree(c, c->f_e0);
   dfree(c, c->f_res0);
-  dfree(c, c->f_xa);
-  dfree(c, c->f_xb);
   c->has_setup = c->assembled = c->solved = c->recovered = false;
   c->csched = CondenseSched{};
   c->tuned = false;
@@ -501,15 +498,15 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
     LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
     LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, c->f_sp, c->stream));
-    LAUNCHCHK(c, launch_mg_power(c->pat, op32(c), c->f_xa, c->f_xb, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
-                                 c->mg.pow_steps, c->stream));
+    LAUNCHCHK(c, launch_lanczos_omega(c->pat, c->T, c->cg, Whole{c->T.N, c->cg.st}, c->cg.nblocks, nullptr, 1,
+                                      c->mg.omega_factor, c->mg.lanczos_steps, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
     LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
   } else if (use_mg(c)) {
     LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
-    // the CG vectors x, r serve as scratch before the solve initialises them
-    LAUNCHCHK(c, launch_mg_power(c->pat, op64(c), c->cg.x, c->cg.r, c->cg.part, c->cg.nblocks, c->cg.st, c->mg,
-                                 c->mg.pow_steps, c->stream));
+    // the CG vectors serve as scratch before the solve initialises them
+    LAUNCHCHK(c, launch_lanczos_omega(c->pat, c->T, c->cg, Whole{c->T.N, c->cg.st}, c->cg.nblocks, nullptr, 1,
+                                      c->mg.omega_factor, c->mg.lanczos_steps, c->stream));
     LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
     LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
   } else {
@@ -872,8 +869,6 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     TRY(dalloc(c, &c->f_sp, 4 * N));
     TRY(dalloc(c, &c->f_e0, N));
     TRY(dalloc(c, &c->f_res0, N));
-    TRY(dalloc(c, &c->f_xa, N));
-    TRY(dalloc(c, &c->f_xb, N));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->src = SrcParams{};

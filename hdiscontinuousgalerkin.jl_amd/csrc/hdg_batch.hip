@@ -85,8 +85,7 @@ struct Group {
   MgGroup G;
   MgSetupGroup S;
   CondenseSched csched;  // condensation schedule (autotuned at create with HDG_SCHEDULE_AUTOTUNE)
-  float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr,
-        *f_xa = nullptr, *f_xb = nullptr;
+  float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr;
   // read-back
   CgState* h_st = nullptr;  // pinned, 2 x ninst (chunk slots) + ninst (final)
   double* h_res = nullptr;  // pinned, 2 x ninst, then ninst error flags (ints)
@@ -443,8 +442,6 @@ int build_group(hdg_batch* b, Group& g) {
     BTRY(galloc(b, g, &g.f_sp, 4 * N));
     BTRY(galloc(b, g, &g.f_e0, N));
     BTRY(galloc(b, g, &g.f_res0, N));
-    BTRY(galloc(b, g, &g.f_xa, N));
-    BTRY(galloc(b, g, &g.f_xb, N));
   }
   MgData choice;
   mg_order_choices(g.k, choice);  // as hdg_setup chooses for a single problem
@@ -455,7 +452,6 @@ int build_group(hdg_batch* b, Group& g) {
     M = I.plan;
     M.nv = I.nn;
     M.vsweeps = choice.vsweeps;
-    M.pow_steps = choice.pow_steps;
     M.cellA = cellA + 9 * I.cell0;
     M.cinv = cinv + (size_t)j * kCoarseDirect * kCoarseDirect;
     M.tailpack = tailpack + I.tail_off;
@@ -577,16 +573,14 @@ int enqueue_setup(hdg_batch* b, Group& g) {
     BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)g.nt * g.N, g.s));
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, g.f_sp, g.s));
     BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
-    BLAUNCH(b, launch_mg_power_group(g.pat, op32(g), g.f_xa, g.f_xb, g.cg.part, gr, g.nbt, g.segs, n, factor,
-                                     g.Mi[0].pow_steps, g.s));
+    BLAUNCH(b, launch_lanczos_omega(g.pat, g.T, g.cg, gr, g.nbt, g.segs, n, factor, g.Mi[0].lanczos_steps, g.s));
     BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op32(g), kv(g), mv32(g), gr, g.nbt, g.s));
     BLAUNCH(b, launch_mg_precond_group(g.pat, op32(g), kv(g), g.Mu, g.G, mv32(g), gr, g.nbt, g.s));
   } else {
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, nullptr, g.s));
     BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
-    // the CG vectors x, r serve as scratch before the solve initialises them
-    BLAUNCH(b, launch_mg_power_group(g.pat, op64(g), g.cg.x, g.cg.r, g.cg.part, gr, g.nbt, g.segs, n, factor,
-                                     g.Mi[0].pow_steps, g.s));
+    // the CG vectors serve as scratch before the solve initialises them
+    BLAUNCH(b, launch_lanczos_omega(g.pat, g.T, g.cg, gr, g.nbt, g.segs, n, factor, g.Mi[0].lanczos_steps, g.s));
     BLAUNCH(b, launch_mg_init_group(g.pat, g.T, op64(g), kv(g), mv64(g), gr, g.nbt, g.s));
     BLAUNCH(b, launch_mg_precond_group(g.pat, op64(g), kv(g), g.Mu, g.G, mv64(g), gr, g.nbt, g.s));
   }

@@ -68,6 +68,9 @@ struct CondenseArgs {
          *bev = nullptr;
 };
 
+// PCG coefficients kept for the Lanczos estimate of lambda_max(Dinv S)
+constexpr int kLanczos = 16;
+
 // PCG scalar state, resident on the device; written by block 0 of the
 // kernels that own each field, read by later launches.
 struct CgState {
@@ -86,6 +89,11 @@ struct CgState {
   int i_spmv;    // for the next CG product (written by the update / smoother)
   int i_upd;     // for the update (written by the product)
   int i_pre;     // for the smoother (written by the multigrid update)
+  // iteration i <= kLanczos of a run: (r, z) at its product and (p, S p) at
+  // its update; block-Jacobi PCG's coefficients are those of the Lanczos
+  // process of Dinv S
+  double lz[kLanczos + 1];
+  double lpq[kLanczos + 1];
 };
 
 struct TraceSystem {
@@ -198,20 +206,15 @@ struct MgData {
   double* tailpack = nullptr;    // stencils (7 n) and dinv (n) of levels tail.. back to back,
                                  // the LDS image the tail kernel loads
   double omega_v = 0.8;          // vertex-level Jacobi weight
-  double omega_factor = 1.5;     // trace-level weight = factor / lambda_max(Dinv S)
+  double omega_factor = 1.6;     // trace-level weight = factor / lambda_max(Dinv S) (Lanczos estimate)
   int coarse_sweeps = 20;
   int vsweeps = 2;               // pre- and post-smoothing sweeps on the vertex levels (1 or 2)
-  int pow_steps = 10;            // power steps behind omega
+  int lanczos_steps = 8;         // block-Jacobi PCG (Lanczos) steps behind omega
 };
 // per-order choices of the V-cycle, shared by single problems and batches:
-// V(1,1) on the vertex levels for k <= 2, V(2,2) above; 10 power steps for
-// the trace-level omega at k = 1, 6 above (fewest CG iterations x cost per
-// step measured on 97^2 .. 1024^2 meshes: fewer steps underestimate
-// lambda_max, which helps at k >= 2 and costs 3 iterations at k = 1)
-inline void mg_order_choices(int k, MgData& M) {
-  M.vsweeps = k <= 2 ? 1 : 2;
-  M.pow_steps = k == 1 ? 10 : 6;
-}
+// V(1,1) on the vertex levels for k <= 2, V(2,2) above (fewest CG
+// iterations x cost per iteration measured on 256..1024 meshes)
+inline void mg_order_choices(int k, MgData& M) { M.vsweeps = k <= 2 ? 1 : 2; }
 
 // A fine level, the next coarser one and the CG state of one instance: the
 // unit of a vertex-level kernel.  VOne launches one instance over the whole
@@ -313,10 +316,14 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, const MgData* dM, TraceSyst
 int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
                   const double* Ate, float* sp32, hipStream_t s);
 int mg_setup_levels(const MgData& M, const MgData* dM, hipStream_t s);
-// omega = factor / lambda_max(Dinv S) by power steps on the operator as stored
-template <class V>
-int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
-                    const MgData& M, int steps, hipStream_t s);
+// omega = factor / lambda_max(Dinv S), lambda_max the largest Ritz value of
+// `steps` block-Jacobi PCG iterations from a random right-hand side (the
+// Lanczos tridiagonal matrix of Dinv S built from their coefficients); uses
+// the CG vectors as scratch.  Whole: one problem (st); Grouped: one estimate
+// per instance (seg[0 .. ninst)).
+template <class G>
+int launch_lanczos_omega(const DevPattern& P, const TraceSystem& T, CgVectors& v, G g, int grid, const Seg* seg,
+                         int ninst, double factor, int steps, hipStream_t s);
 // r = b~, x = p = 0, e0 = omega Dinv r (preconditioner storage W)
 template <class W>
 int launch_mg_init(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
@@ -364,9 +371,6 @@ int mg_setup_levels_group(const MgSetupGroup& S, hipStream_t s);
 // grouped forms: rows of each instance from the Grouped selector (grid
 // blocks); Mu describes the packed mesh (its level 1 spans every
 // instance's vertices)
-template <class V>
-int launch_mg_power_group(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, Grouped g, int grid,
-                          const Seg* seg, int ninst, double factor, int steps, hipStream_t s);
 template <class W>
 int launch_mg_init_group(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
                          const MgVecs<W>& mv, Grouped g, int grid, hipStream_t s);

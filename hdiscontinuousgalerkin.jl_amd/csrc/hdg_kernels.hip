@@ -16,6 +16,8 @@
 //   errornorm  squared L2 error (src/DiscreteFunctions.jl:97-120)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "hdg_internal.h"
 
 namespace hdg {
@@ -619,6 +621,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
       st->rr = rr;
       if (i == 1) st->bb = rr;
       st->i_upd = i;
+      if (i <= kLanczos) st->lz[i] = rz;
     }
   }
   const double beta = (i == 1) ? 0.0 : rz / st->rz[(i - 1) & 1];
@@ -670,6 +673,7 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
     }
     return;
   }
+  if (!force && i <= kLanczos && lb == 0 && threadIdx.x == 0) st->lpq[i] = pq;
   const double alpha = rz / pq;
   const double* __restrict__ p = (i & 1) ? v.p1 : v.p0;
   double prz = 0.0, prr = 0.0;
@@ -707,7 +711,18 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
 
 // update(0): r0 = b~ = -(rhs - K uD) on free dofs, x = 0, z = Dinv r0,
 // p = 0; partials of (r0, z0) and (r0, r0).
-template <int NT, class G>
+__device__ inline double hash01_k(uint64_t i) {
+  uint64_t z = i * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// RAND: the right-hand side is a fixed pseudo-random vector in [-1/2, 1/2)
+// on the free dofs (indexed from the instance's first row), for the Lanczos
+// estimate.
+template <int NT, class G, bool RAND = false>
 __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSystem T, CgVectors v, G gsel) {
   constexpr int NTH = 64 * NT;
   __shared__ double rs[NTH];
@@ -719,7 +734,9 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
     const int64_t r = r0 + threadIdx.x;
     double b = 0.0;
     if (r < S.hi) {
-      if (!T.dmask[r]) {
+      if (RAND) {
+        b = T.dmask[r] ? 0.0 : hash01_k((uint64_t)(r - S.lo)) - 0.5;
+      } else if (!T.dmask[r]) {
         const int64_t f = r / NT;
         double s = T.rhs[r];
 #pragma unroll
@@ -881,6 +898,59 @@ __global__ __launch_bounds__(256) void errornorm_kernel(DevMesh m, const double*
   }
   const double bs = block_sum<4>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// Largest eigenvalue of the Lanczos tridiagonal matrix of one instance's
+// run: alpha_j = lz[j] / lpq[j], beta_j = lz[j+1] / lz[j], diagonal
+// 1/alpha_j + beta_{j-1}/alpha_{j-1}, off-diagonal sqrt(beta_j)/alpha_j,
+// over the m iterations that completed; by bisection on Sturm counts.
+// omega = factor / lambda (0.5 when no iteration completed).
+__device__ void lanczos_omega(CgState* st, int m, double factor) {
+  if (st->done) m = min(m, st->it);
+  double d[kLanczos], e2[kLanczos];  // diagonal, squared off-diagonal
+  int n = 0;
+  for (int j = 1; j <= m; ++j) {
+    const double a = st->lz[j] / st->lpq[j];
+    double dj = 1.0 / a;
+    if (j > 1) dj += (st->lz[j] / st->lz[j - 1]) / (st->lz[j - 1] / st->lpq[j - 1]);
+    if (!(dj > 0.0) || !isfinite(dj)) break;
+    d[n] = dj;
+    e2[n] = j < m ? (st->lz[j + 1] / st->lz[j]) / (a * a) : 0.0;
+    if (!isfinite(e2[n])) break;
+    ++n;
+  }
+  if (n == 0) {
+    st->omega = 0.5;
+    return;
+  }
+  double hi = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double r = (j > 0 ? sqrt(e2[j - 1]) : 0.0) + (j + 1 < n ? sqrt(e2[j]) : 0.0);
+    hi = fmax(hi, d[j] + r);
+  }
+  double lo = 0.0;
+  for (int it = 0; it < 200 && hi - lo > 1e-13 * hi; ++it) {
+    const double x = 0.5 * (lo + hi);
+    int below = 0;  // eigenvalues below x
+    double q = 1.0;
+    for (int j = 0; j < n; ++j) {
+      q = d[j] - x - (j > 0 ? e2[j - 1] / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      below += q < 0.0;
+    }
+    if (below < n) lo = x;  // the largest eigenvalue is above x
+    else hi = x;
+  }
+  st->omega = factor / hi;
+}
+
+__global__ void lanczos_omega_kernel(CgState* st, int m, double factor) {
+  if (threadIdx.x == 0) lanczos_omega(st, m, factor);
+}
+
+__global__ void lanczos_omega_seg_kernel(const Seg* seg, int ninst, int m, double factor) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < ninst) lanczos_omega(seg[j].st, m, factor);
 }
 
 int grid_for(int64_t n, int threads, int cap) {
@@ -1091,6 +1161,32 @@ int launch_errornorm_t(const DevMesh& m, const double* tables, const TabLayoutDe
 GROUP_INSTANTIATE(Whole)
 GROUP_INSTANTIATE(Grouped)
 #undef GROUP_INSTANTIATE
+
+template <class G>
+int launch_lanczos_omega(const DevPattern& P, const TraceSystem& T, CgVectors& v, G g, int grid, const Seg* seg,
+                         int ninst, double factor, int steps, hipStream_t s) {
+  steps = std::min(std::max(steps, 1), kLanczos);
+#define LINIT(NT) \
+  hipLaunchKernelGGL((cg_init_kernel<NT, G, true>), dim3(grid), dim3(64 * NT), 0, s, P, T, v, g);
+  HDG_NT_DISPATCH(T.nt, LINIT)
+#undef LINIT
+  const TraceOp<double> op{T.nt, T.nf, T.N, T.vals, T.Dinv, T.dmask};
+  const Krylov<double> kv{v.r, v.z, v.p0, v.p1, v.q, v.x, v.part, v.st, v.nblocks};
+  for (int i = 0; i < steps; ++i) {
+    int e = launch_cg_spmv_t(P, op, kv, g, grid, 0, 0.0, steps, 0, s);
+    if (!e) e = launch_cg_update_t(T, v, g, grid, 0, 0, s);
+    if (e) return e;
+  }
+  if (seg)
+    hipLaunchKernelGGL(lanczos_omega_seg_kernel, dim3((ninst + 63) / 64), dim3(64), 0, s, seg, ninst, steps, factor);
+  else
+    hipLaunchKernelGGL(lanczos_omega_kernel, dim3(1), dim3(64), 0, s, v.st, steps, factor);
+  return (int)hipGetLastError();
+}
+template int launch_lanczos_omega<Whole>(const DevPattern&, const TraceSystem&, CgVectors&, Whole, int, const Seg*,
+                                         int, double, int, hipStream_t);
+template int launch_lanczos_omega<Grouped>(const DevPattern&, const TraceSystem&, CgVectors&, Grouped, int,
+                                           const Seg*, int, double, int, hipStream_t);
 
 int launch_reduce_segs(const double* part, const Seg* seg, int nseg, double* out, hipStream_t s) {
   hipLaunchKernelGGL(reduce_seg_kernel, dim3(nseg), dim3(256), 0, s, part, seg, out);

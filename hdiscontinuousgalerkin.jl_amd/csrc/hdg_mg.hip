@@ -418,75 +418,6 @@ __global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T,
   }
 }
 
-// ---------------------------------------------- power iteration for omega
-__device__ inline double hash01(uint64_t i) {
-  uint64_t z = i * 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  z ^= z >> 31;
-  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
-}
-
-template <int NT, class V, class G>
-__global__ __launch_bounds__(64 * NT) void pow_init_kernel(TraceOp<V> T, V* x, double* part, G gsel) {
-  __shared__ double sh[NT];
-  const Seg S = seg_of(gsel);
-  const int lb = blockIdx.x - S.b0;
-  double s2 = 0.0;
-  for (int64_t r = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
-    const V v = (V)(T.dmask[r] ? 0.0 : 0.5 + hash01((uint64_t)(r - S.lo)));
-    x[r] = v;
-    s2 = fma((double)v, (double)v, s2);
-  }
-  const double b = block_sum_mg<NT>(s2, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = b;
-}
-
-// y = Dinv S (x / |x|), |x|^2 from the previous step's partials.
-template <int NT, class V, class G>
-__global__ __launch_bounds__(64 * NT) void pow_step_kernel(DevPattern P, TraceOp<V> T, const V* x, V* y,
-                                                           const double* part_in, double* part_out, G gsel) {
-  constexpr int NTH = 64 * NT;
-  __shared__ double rs[NTH];
-  __shared__ double sh[NT];
-  const Seg S = seg_of(gsel);
-  const int lb = blockIdx.x - S.b0;
-  const double nx = sqrt(sum_partials_mg<NT>(part_in + S.b0, S.nb, sh));
-  const double inv = nx > 0.0 ? 1.0 / nx : 0.0;
-  double s2 = 0.0;
-  for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
-    const int64_t r = r0 + threadIdx.x;
-    rs[threadIdx.x] = r < S.hi ? inv * s_row<NT, V>(P, T, x, r) : 0.0;
-    __syncthreads();
-    if (r < S.hi) {
-      const int lr = (threadIdx.x / NT) * NT;
-      double z = 0.0;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
-      const V zv = (V)z;
-      y[r] = zv;
-      s2 = fma((double)zv, (double)zv, s2);
-    }
-    __syncthreads();
-  }
-  const double b = block_sum_mg<NT>(s2, sh);
-  if (threadIdx.x == 0) part_out[blockIdx.x] = b;
-}
-
-__global__ __launch_bounds__(256) void pow_final_kernel(const double* part, int nb, CgState* st, double cfac) {
-  __shared__ double sh[4];
-  const double lam = sqrt(sum_partials_mg<4>(part, nb, sh));
-  if (threadIdx.x == 0) st->omega = lam > 0.0 ? cfac / lam : 0.5;
-}
-
-// one block per instance of a group
-__global__ __launch_bounds__(256) void pow_final_seg_kernel(const double* part, const Seg* seg, double cfac) {
-  __shared__ double sh[4];
-  const Seg S = seg[blockIdx.x];
-  const double lam = sqrt(sum_partials_mg<4>(part + S.b0, S.nb, sh));
-  if (threadIdx.x == 0) S.st->omega = lam > 0.0 ? cfac / lam : 0.5;
-}
-
 // ------------------------------------------------------- level 0 (trace)
 // update(i) with the multigrid preconditioner: x += alpha p; r -= alpha q;
 // e0 = omega Dinv r (first smoothing step, preconditioner storage W);
@@ -1083,48 +1014,6 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, const MgData* dM, TraceSyst
   return mg_setup_levels(M, dM, s);
 }
 
-// omega = factor / lambda_max(Dinv S) by power steps on free dofs; xa, xb
-// are scratch vectors of N entries, part holds 2 nb partials.
-template <class V, class G>
-int mg_power_steps(const DevPattern& P, const TraceOp<V>& T, V*& xa, V*& xb, double*& pa, double*& pb, G g,
-                   int grid, int steps, hipStream_t s) {
-#define PINIT(NT) hipLaunchKernelGGL((pow_init_kernel<NT, V, G>), dim3(grid), dim3(64 * NT), 0, s, T, xa, pa, g);
-  MG_NT_DISPATCH(T.nt, PINIT)
-#undef PINIT
-  for (int k = 0; k < steps; ++k) {
-#define PSTEP(NT)                                                                                              \
-    hipLaunchKernelGGL((pow_step_kernel<NT, V, G>), dim3(grid), dim3(64 * NT), 0, s, P, T, (const V*)xa, xb, pa, \
-                       pb, g);
-    MG_NT_DISPATCH(T.nt, PSTEP)
-#undef PSTEP
-    std::swap(xa, xb);
-    std::swap(pa, pb);
-  }
-  return (int)hipGetLastError();
-}
-
-template <class V>
-int launch_mg_power(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, int nb, CgState* st,
-                    const MgData& M, int steps, hipStream_t s) {
-  double* pa = part;
-  double* pb = part + nb;
-  const int e = mg_power_steps(P, T, xa, xb, pa, pb, Whole{T.N, st}, nb, steps, s);
-  if (e) return e;
-  hipLaunchKernelGGL(pow_final_kernel, dim3(1), dim3(256), 0, s, pa, nb, st, M.omega_factor);
-  return (int)hipGetLastError();
-}
-
-template <class V>
-int launch_mg_power_group(const DevPattern& P, const TraceOp<V>& T, V* xa, V* xb, double* part, Grouped g, int grid,
-                          const Seg* seg, int ninst, double factor, int steps, hipStream_t s) {
-  double* pa = part;
-  double* pb = part + grid;
-  const int e = mg_power_steps(P, T, xa, xb, pa, pb, g, grid, steps, s);
-  if (e) return e;
-  hipLaunchKernelGGL(pow_final_seg_kernel, dim3(ninst), dim3(256), 0, s, pa, seg, factor);
-  return (int)hipGetLastError();
-}
-
 template <class W, class G>
 int mg_init_t(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
               const MgVecs<W>& mv, G g, int grid, hipStream_t s) {
@@ -1217,10 +1106,6 @@ int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Kryl
 }
 
 #define MG_INSTANTIATE(W)                                                                                      \
-  template int launch_mg_power<W>(const DevPattern&, const TraceOp<W>&, W*, W*, double*, int, CgState*,          \
-                                  const MgData&, int, hipStream_t);                                             \
-  template int launch_mg_power_group<W>(const DevPattern&, const TraceOp<W>&, W*, W*, double*, Grouped, int,     \
-                                        const Seg*, int, double, int, hipStream_t);                              \
   template int launch_mg_init<W>(const DevPattern&, const TraceSystem&, const TraceOp<W>&, const Krylov<double>&, \
                                  const MgVecs<W>&, hipStream_t);                                                \
   template int launch_mg_init_group<W>(const DevPattern&, const TraceSystem&, const TraceOp<W>&,                 \
