@@ -139,6 +139,28 @@ struct Dims {
   static constexpr int EPW = 64 / SEG;  // elements per wavefront
 };
 
+// Max of an int over each segment of SEG lanes, in every lane: DPP stages
+// inside each row of 16 lanes (pairs by xor 1 and 2 within quads, then
+// mirrored halves and mirrored rows: after each stage every lane holds the
+// max of a group twice as large), then one readlane per row combines the
+// rows of a 32- or 64-lane segment.  Every lane of the wavefront must be
+// active.
+template <int SEG>
+__device__ inline int seg_max_int(int k, int seg) {
+  k = max(k, __builtin_amdgcn_update_dpp(k, k, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  k = max(k, __builtin_amdgcn_update_dpp(k, k, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  k = max(k, __builtin_amdgcn_update_dpp(k, k, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  k = max(k, __builtin_amdgcn_update_dpp(k, k, 0x140, 0xF, 0xF, false));  // row_mirror
+  if constexpr (SEG == 16) {
+    return k;
+  } else {
+    const int r0 = __builtin_amdgcn_readlane(k, 0), r1 = __builtin_amdgcn_readlane(k, 16);
+    const int r2 = __builtin_amdgcn_readlane(k, 32), r3 = __builtin_amdgcn_readlane(k, 48);
+    if constexpr (SEG == 32) return seg ? max(r2, r3) : max(r0, r1);
+    else return max(max(r0, r1), max(r2, r3));
+  }
+}
+
 // Broadcast of lane `src` (absolute lane in the wavefront) within a segment.
 template <int SEG>
 __device__ inline double seg_bcast(double v, int src) {
@@ -363,8 +385,7 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
 #pragma unroll
       for (int j = 0; j < M; ++j) {
         int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~63u) | (unsigned)(63 - ls));
-#pragma unroll
-        for (int off = SEG / 2; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off, SEG));
+        key = seg_max_int<SEG>(key, seg);
         const int bl = 63 - (key & 63);
         const int src = sbase + bl;
         const double pv = seg_bcast<SEG>(row[j], src);
