@@ -285,7 +285,13 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
   using D = Dims<K>;
   static_assert(SEG >= D::ROWS && 64 % SEG == 0, "segment too small");
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, EPW = 64 / SEG;
+  // border columns of the bordered system (trace block and rhs) and the
+  // stride of a segment's pivot-row buffer (16-byte aligned, and an odd
+  // number of 16-byte units so the segments of a wavefront start in
+  // different banks)
+  constexpr int CS = M, NB = D::NC - CS, PBS = ((NB + 1) / 2 | 1) * 2;
   extern __shared__ double T[];
+  __shared__ __align__(16) double prow[4 * EPW * PBS];
   const TabLayoutDev L = a.lay;
   for (int i = threadIdx.x; i < L.total; i += blockDim.x) T[i] = __ldg(a.tables + i);
   __syncthreads();
@@ -382,6 +388,14 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
       // a 32-bit key: the upper word of |a_ij| with its low 6 bits replaced
       // by 63 - (row index) (ties and near-ties go to the lower row, the
       // same choice for every SEG).
+      // The pivot row reaches the other lanes in two ways: columns below M
+      // by cross-lane moves (readlane or bpermute, VALU / LDS-permute work
+      // per column), the border columns (trace block and rhs, present in
+      // every step) through the segment's row in LDS: the pivot lane writes
+      // them, every lane reads them back as broadcast 16-byte loads.  LDS
+      // instructions of one wavefront execute in order, so the reads see
+      // the write and the next step's write follows these reads.
+      double* pb = prow + ((threadIdx.x >> 6) * EPW + seg) * PBS;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
         int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~63u) | (unsigned)(63 - ls));
@@ -392,8 +406,31 @@ __global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
         bad |= !(fabs(pv) > thr);
         const bool me = (ls == bl);
         const double f = me ? 0.0 : row[j] / pv;
+        // LDS part of the pivot row: columns c0.. of the buffer (pairs)
+        const int c0 = (j + 1 > CS ? j + 1 - CS : 0) & ~1;
+        if (me) {
 #pragma unroll
-        for (int c = j + 1; c < D::NC; ++c) row[c] = fma(-f, seg_bcast<SEG>(row[c], src), row[c]);
+          for (int c = 0; c < NB; c += 2) {
+            if (c < c0) continue;
+            double2 v;
+            v.x = row[CS + c];
+            v.y = c + 1 < NB ? row[CS + c + 1] : 0.0;
+            *reinterpret_cast<double2*>(pb + c) = v;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int c = j + 1; c < CS; ++c) row[c] = fma(-f, seg_bcast<SEG>(row[c], src), row[c]);
+#pragma unroll
+        for (int c = 0; c < NB; c += 2) {
+          if (c < c0) continue;
+          const double2 v = *reinterpret_cast<const double2*>(pb + c);
+          if (CS + c > j) row[CS + c] = fma(-f, v.x, row[CS + c]);
+          if (c + 1 < NB) row[CS + c + 1] = fma(-f, v.y, row[CS + c + 1]);
+        }
+        __builtin_amdgcn_wave_barrier();
         if (me) {
           used = true;
           mycol = j;
