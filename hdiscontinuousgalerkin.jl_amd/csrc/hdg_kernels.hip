@@ -280,8 +280,15 @@ __device__ __forceinline__ const SrcParams& src_of(const CondenseArgs& a, int64_
 // SEG: lanes per element (a power of two >= the rows of the bordered
 // system); EPW = 64 / SEG elements per wavefront.  The result does not
 // depend on SEG (the pivot key below is the same for every SEG).
+// Occupancy: left alone the compiler keeps two waves per SIMD (175-230
+// VGPRs); the kernel issues VALU work back to back and a third wave hides
+// more of its latency than the few spilled registers cost at k <= 3
+// (condensation of 512^2 cells: k=1 0.53 -> 0.46 ms, k=2 2.04 -> 1.88 ms,
+// k=3 6.07 -> 5.31 ms).  At k = 4 the row holds 56 doubles and the spills
+// of a third wave cost more than it hides (10.2 -> 11.6 ms).
 template <int K, bool BLOCKS, bool GROUPED, int SEG>
-__global__ __launch_bounds__(256) void condense_kernel(CondenseArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!BLOCKS && K <= 3 ? 3 : 1)))
+void condense_kernel(CondenseArgs a) {
   using D = Dims<K>;
   static_assert(SEG >= D::ROWS && 64 % SEG == 0, "segment too small");
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, EPW = 64 / SEG;
