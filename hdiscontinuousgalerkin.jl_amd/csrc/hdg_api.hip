@@ -6,21 +6,69 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
+#include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/hdg.h"
+#include "hdg_arena.h"
 #include "hdg_host.h"
 #include "hdg_internal.h"
 #include "hdg_tables.h"
 
 using namespace hdg;
 
+// First-fit sub-allocator over one device allocation; free pieces are kept
+// by offset and merged with their neighbours when returned.
+struct hdg_arena {
+  std::mutex mu;
+  int device = 0;
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes
+  static size_t round(size_t b) { return (b + 255) & ~size_t(255); }
+  bool owns(const void* p) const { return p >= base && p < base + size; }
+  void* take(size_t bytes) {
+    bytes = round(bytes);
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + bytes] = rest;
+      return base + off;
+    }
+    return nullptr;
+  }
+  void give(void* p, size_t bytes) {
+    bytes = round(bytes);
+    size_t off = (size_t)((char*)p - base);
+    std::lock_guard<std::mutex> lk(mu);
+    auto next = free_.lower_bound(off);
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        off = prev->first;
+        bytes += prev->second;
+        free_.erase(prev);
+      }
+    }
+    if (next != free_.end() && off + bytes == next->first) {
+      bytes += next->second;
+      free_.erase(next);
+    }
+    free_[off] = bytes;
+  }
+};
+
 struct hdg_ctx {
   int device = 0;
   size_t budget = 0, used = 0;
   std::unordered_map<void*, size_t> allocs;
+  hdg_arena* arena = nullptr;  // serving loop's arena, or null
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -111,12 +159,14 @@ int dalloc(hdg_ctx* c, T** p, size_t count) {
   if (c->budget && c->used + bytes > c->budget)
     return fail(c, HDG_ERR_HBM_BUDGET, "device allocation of %zu bytes exceeds the budget (%zu of %zu used)",
                 bytes, c->used, c->budget);
-  // stream-ordered allocation from the device's pool: unlike hipMalloc /
-  // hipFree it does not synchronise the device, so contexts working on
-  // other streams (the serving loop's slots) keep running
-  void* q = nullptr;
-  hipError_t e = hipMallocAsync(&q, bytes, c->stream);
-  if (e != hipSuccess) return fail(c, HDG_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+  // the serving loop's arena first; else stream-ordered allocation from the
+  // device's pool: unlike hipMalloc / hipFree it does not synchronise the
+  // device, so contexts working on other streams keep running
+  void* q = c->arena ? c->arena->take(bytes) : nullptr;
+  if (!q) {
+    hipError_t e = hipMallocAsync(&q, bytes, c->stream);
+    if (e != hipSuccess) return fail(c, HDG_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
   c->allocs[q] = bytes;
   c->used += bytes;
   *p = static_cast<T*>(q);
@@ -127,11 +177,19 @@ template <class T>
 void dfree(hdg_ctx* c, T*& p) {
   if (!p) return;
   auto it = c->allocs.find((void*)p);
+  size_t bytes = 0;
   if (it != c->allocs.end()) {
+    bytes = it->second;
     c->used -= it->second;
     c->allocs.erase(it);
   }
-  (void)hipFreeAsync((void*)p, c->stream);
+  if (c->arena && c->arena->owns((const void*)p)) {
+    // work queued on the stream may still read the buffer
+    (void)hipStreamSynchronize(c->stream);
+    c->arena->give((void*)p, bytes);
+  } else {
+    (void)hipFreeAsync((void*)p, c->stream);
+  }
   p = nullptr;
 }
 
@@ -623,6 +681,11 @@ void hdg_destroy(hdg_ctx* c) {
   dfree(c, c->d_err);
   dfree(c, c->cg.st);
   dfree(c, c->d_scal);
+  // whatever is still allocated (arena pieces must go back before the arena)
+  while (!c->allocs.empty()) {
+    void* p = c->allocs.begin()->first;
+    dfree(c, p);
+  }
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1247,3 +1310,33 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- serving arena
+int hdg_arena_create(hdg_arena** out, int device, size_t bytes) {
+  if (!out || bytes == 0) return HDG_ERR_ARG;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return HDG_ERR_HIP;
+  hdg_arena* a = new hdg_arena();
+  a->device = device;
+  a->size = hdg_arena::round(bytes);
+  if (hipMalloc((void**)&a->base, a->size) != hipSuccess) {
+    delete a;
+    return HDG_ERR_HIP;
+  }
+  a->free_[0] = a->size;
+  *out = a;
+  return HDG_OK;
+}
+
+void hdg_arena_destroy(hdg_arena* a) {
+  if (!a) return;
+  (void)hipSetDevice(a->device);
+  (void)hipFree(a->base);
+  delete a;
+}
+
+int hdg_attach_arena(hdg_ctx* c, hdg_arena* a) {
+  if (!c || !a || a->device != c->device) return HDG_ERR_ARG;
+  c->arena = a;
+  return HDG_OK;
+}

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/hdg.h"
+#include "hdg_arena.h"
 
 namespace {
 
@@ -60,6 +61,7 @@ bool batchable(const hdg_request& q) {
 struct hdg_server {
   int device = 0;
   size_t budget = 0;
+  hdg_arena* arena = nullptr;  // the budget as one device allocation (null: device pool)
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Pending> queue;
@@ -93,7 +95,10 @@ hdg_result run_request(hdg_server* S, Slot& sl, const Pending& p, std::vector<do
   r.t_start = S->now();
   const hdg_request& q = p.req;
   int st = HDG_OK;
-  if (!sl.ctx) st = hdg_create(&sl.ctx, S->device, 0);
+  if (!sl.ctx) {
+    st = hdg_create(&sl.ctx, S->device, 0);
+    if (st == HDG_OK && S->arena) st = hdg_attach_arena(sl.ctx, S->arena);
+  }
   const bool mesh_hit = sl.has_mesh && sl.nx == q.nx && sl.ny == q.ny && sl.jitter == q.jitter &&
                         sl.seed == q.seed;
   if (st == HDG_OK && !mesh_hit) {
@@ -339,6 +344,9 @@ int hdg_server_create(hdg_server** out, int device, size_t hbm_budget, int slots
   S->device = device;
   S->budget = hbm_budget;
   S->t0 = std::chrono::steady_clock::now();
+  // the slots' contexts allocate from one piece of the budget (a failed
+  // allocation of it leaves them on the device pool)
+  if (hbm_budget > 0 && hdg_arena_create(&S->arena, device, hbm_budget) != HDG_OK) S->arena = nullptr;
   S->slots.resize(slots);
   for (int w = 0; w < slots; ++w) S->threads.emplace_back(worker, S, w);
   *out = S;
@@ -439,6 +447,7 @@ void hdg_server_destroy(hdg_server* S) {
   }
   S->cv_work.notify_all();
   for (auto& t : S->threads) t.join();
+  hdg_arena_destroy(S->arena);
   delete S;
 }
 

@@ -76,6 +76,34 @@ def test_server_budget_admission(hdg):
     assert all(r["device_bytes"] <= foot for r in res)
 
 
+def test_server_arena_equals_pool(hdg):
+    """A server with a budget carves its contexts' buffers out of one device
+    allocation of that size (freed pieces are reused by the next set-ups of
+    other sizes); the results are bitwise those of contexts on the device
+    pool, including when a slot gives its memory back under budget pressure."""
+    specs = [(40, 40, 1), (96, 64, 3), (17, 33, 2), (128, 128, 1), (64, 64, 4), (50, 20, 2),
+             (72, 72, 3), (24, 90, 1), (110, 110, 2), (31, 31, 4), (80, 40, 3), (60, 60, 2),
+             (128, 96, 1), (45, 45, 3), (20, 20, 2), (100, 50, 4)]
+    reqs = [_req(hdg, nx, ny, k, 900 + i, want_trace=True) for i, (nx, ny, k) in enumerate(specs)]
+    with hdg.Server(slots=1) as s1:
+        ref = []
+        for r in reqs:
+            res = s1.wait(s1.submit(r))
+            assert res["status"] == 0
+            ref.append((res, s1.trace(res["id"], res["n_dofs"])))
+    foot = max(hdg.footprint(r) for r in reqs)
+    for budget in (int(3.5 * foot), int(1.2 * foot)):
+        with hdg.Server(slots=3, hbm_budget=budget) as s:
+            ids = [s.submit(r) for r in reqs]
+            got = [s.wait(i) for i in ids]
+            tr = [s.trace(i, g["n_dofs"]) for i, g in zip(ids, got)]
+            st = s.stats()
+        assert st["failed"] == 0 and st["peak_reserved"] <= budget
+        for (ra, ta), rb, tb in zip(ref, got, tr):
+            assert rb["status"] == 0 and rb["iters"] == ra["iters"]
+            assert np.array_equal(ta, tb)
+
+
 def test_server_rejects_and_isolates_failures(hdg):
     with hdg.Server(slots=2, hbm_budget=64 << 20) as s:
         with pytest.raises(hdg.HDGError, match="HBM_BUDGET"):
