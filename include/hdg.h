@@ -298,7 +298,10 @@ typedef struct {
   size_t peak_reserved;      /* largest sum of admitted footprints */
 } hdg_server_summary;
 
-/* hbm_budget 0 = no cap; slots >= 1. */
+/* hbm_budget 0 = no cap; slots >= 1.  With a cap, the server allocates
+   hbm_budget bytes of device memory at creation, as one piece held until
+   hdg_server_destroy, and the slots' contexts sub-allocate from it (if that
+   allocation fails they use the device's pool). */
 int hdg_server_create(hdg_server** srv, int device, size_t hbm_budget, int slots);
 /* Continuous batching: a free slot that takes a request also takes up to
    max_requests - 1 more queued requests with the same options (order,
