@@ -21,49 +21,6 @@
 
 using namespace hdg;
 
-// First-fit sub-allocator over one device allocation; free pieces are kept
-// by offset and merged with their neighbours when returned.
-struct hdg_arena {
-  std::mutex mu;
-  int device = 0;
-  char* base = nullptr;
-  size_t size = 0;
-  std::map<size_t, size_t> free_;  // offset -> bytes
-  static size_t round(size_t b) { return (b + 255) & ~size_t(255); }
-  bool owns(const void* p) const { return p >= base && p < base + size; }
-  void* take(size_t bytes) {
-    bytes = round(bytes);
-    std::lock_guard<std::mutex> lk(mu);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < bytes) continue;
-      const size_t off = it->first, rest = it->second - bytes;
-      free_.erase(it);
-      if (rest) free_[off + bytes] = rest;
-      return base + off;
-    }
-    return nullptr;
-  }
-  void give(void* p, size_t bytes) {
-    bytes = round(bytes);
-    size_t off = (size_t)((char*)p - base);
-    std::lock_guard<std::mutex> lk(mu);
-    auto next = free_.lower_bound(off);
-    if (next != free_.begin()) {
-      auto prev = std::prev(next);
-      if (prev->first + prev->second == off) {
-        off = prev->first;
-        bytes += prev->second;
-        free_.erase(prev);
-      }
-    }
-    if (next != free_.end() && off + bytes == next->first) {
-      bytes += next->second;
-      free_.erase(next);
-    }
-    free_[off] = bytes;
-  }
-};
-
 struct hdg_ctx {
   int device = 0;
   size_t budget = 0, used = 0;

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/hdg.h"
+#include "hdg_arena.h"
 #include "hdg_host.h"
 #include "hdg_internal.h"
 #include "hdg_tables.h"
@@ -53,7 +54,7 @@ struct Group {
   std::vector<Inst> in;
   int64_t nn = 0, nc = 0, nf = 0, N = 0;
   hipStream_t s = nullptr;
-  std::vector<void*> owned;
+  std::vector<std::pair<void*, size_t>> owned;  // device buffers and their sizes
   size_t bytes = 0;
   // packed mesh and trace system
   DevMesh mesh;
@@ -100,6 +101,7 @@ struct Group {
 struct hdg_batch {
   int device = 0;
   size_t budget = 0, used = 0;
+  hdg_arena* arena = nullptr;  // serving loop's arena, or null
   std::vector<hdg_request> reqs;
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> where;  // request -> (group, instance)
@@ -148,10 +150,12 @@ int galloc(hdg_batch* b, Group& g, T** p, size_t count) {
   if (b->budget && b->used + bytes > b->budget)
     return fail(b, HDG_ERR_HBM_BUDGET, "batch needs more device memory than the budget (%zu of %zu bytes used)",
                 b->used, b->budget);
-  void* q = nullptr;
-  hipError_t e = hipMallocAsync(&q, bytes, g.s);
-  if (e != hipSuccess) return fail(b, HDG_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
-  g.owned.push_back(q);
+  void* q = b->arena ? b->arena->take(bytes) : nullptr;
+  if (!q) {
+    hipError_t e = hipMallocAsync(&q, bytes, g.s);
+    if (e != hipSuccess) return fail(b, HDG_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  g.owned.push_back({q, bytes});
   g.bytes += bytes;
   b->used += bytes;
   *p = static_cast<T*>(q);
@@ -173,7 +177,13 @@ void free_group(hdg_batch* b, Group& g) {
   if (!g.s) return;
   for (auto& x : g.gexec)
     if (x) (void)hipGraphExecDestroy(x);
-  for (void* p : g.owned) (void)hipFreeAsync(p, g.s);
+  (void)hipStreamSynchronize(g.s);  // queued work may still read the buffers
+  for (auto& [p, bytes] : g.owned) {
+    if (b->arena && b->arena->owns(p))
+      b->arena->give(p, bytes);
+    else
+      (void)hipFreeAsync(p, g.s);
+  }
   (void)hipStreamSynchronize(g.s);
   b->used -= std::min(b->used, g.bytes);
   g.owned.clear();
@@ -673,15 +683,24 @@ int enqueue_finish(hdg_batch* b, Group& g) {
 extern "C" {
 
 int hdg_batch_create(hdg_batch** out, int device, size_t hbm_budget, int n, const hdg_request* reqs) {
+  return hdg_batch_create_in_arena(out, device, hbm_budget, n, reqs, nullptr);
+}
+
+}  // extern "C"
+
+int hdg_batch_create_in_arena(hdg_batch** out, int device, size_t hbm_budget, int n, const hdg_request* reqs,
+                              hdg_arena* arena) {
   if (!out) return HDG_ERR_ARG;
   *out = nullptr;
   if (n < 1 || !reqs) return HDG_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HDG_ERR_HIP;
   if (device < 0 || device >= ndev) return HDG_ERR_ARG;
+  if (arena && arena->device != device) return HDG_ERR_ARG;
   hdg_batch* b = new hdg_batch();
   b->device = device;
   b->budget = hbm_budget;
+  b->arena = arena;
   b->reqs.assign(reqs, reqs + n);
   b->where.assign(n, {0, 0});
   auto bail = [&](int rc) {
@@ -738,8 +757,6 @@ int hdg_batch_create(hdg_batch** out, int device, size_t hbm_budget, int n, cons
   *out = b;
   return HDG_OK;
 }
-
-}  // extern "C"
 
 namespace {
 

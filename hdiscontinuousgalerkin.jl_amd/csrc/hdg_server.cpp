@@ -172,7 +172,7 @@ void run_batch(hdg_server* S, int w, const std::vector<Pending>& batch) {
   std::vector<hdg_result> res(n);
   std::vector<std::vector<double>> traces(n);
   hdg_batch* b = nullptr;
-  int st = hdg_batch_create(&b, S->device, 0, n, rq.data());
+  int st = hdg_batch_create_in_arena(&b, S->device, 0, n, rq.data(), S->arena);
   double t_run = S->now();
   if (st == HDG_OK) {
     hdg_batch_run(b, res.data());  // per-request statuses in res
