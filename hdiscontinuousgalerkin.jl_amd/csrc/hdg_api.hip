@@ -403,6 +403,7 @@ CondenseArgs condense_args(const hdg_ctx* c) {
   a.Ke = c->d_Ke;
   a.be = c->d_be;
   a.err = c->d_err;
+  a.natural = c->tab.mass_spd() ? 1 : 0;
   return a;
 }
 

@@ -223,6 +223,7 @@ CondenseArgs group_condense_args(const Group& g) {
   a.Ke = g.Ke;
   a.be = g.be;
   a.err = g.err;
+  a.natural = g.tab.mass_spd() ? 1 : 0;
   return a;
 }
 

@@ -62,6 +62,7 @@ struct CondenseArgs {
   double* Ke = nullptr;
   double* be = nullptr;
   int* err = nullptr;  // bit 1: det(J) <= 0; bit 2: singular local matrix (grouped: one int per instance)
+  int natural = 0;     // the tables' mass is SPD (RefTables::mass_spd): eliminate in natural order
   // element-block mode
   int64_t first = 0, count = 0;
   double *A = nullptr, *B = nullptr, *C = nullptr, *E = nullptr, *F = nullptr, *H = nullptr,

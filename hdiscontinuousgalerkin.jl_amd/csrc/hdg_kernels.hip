@@ -286,7 +286,12 @@ __device__ __forceinline__ const SrcParams& src_of(const CondenseArgs& a, int64_
 // (condensation of 512^2 cells: k=1 0.53 -> 0.46 ms, k=2 2.04 -> 1.88 ms,
 // k=3 6.07 -> 5.31 ms).  At k = 4 the row holds 56 doubles and the spills
 // of a third wave cost more than it hides (10.2 -> 11.6 ms).
-template <int K, bool BLOCKS, bool GROUPED, int SEG>
+// NAT: the tables' mass is SPD (CondenseArgs::natural), so A is SPD on every
+// cell and Gauss-Jordan runs in natural order: row j is the pivot of column
+// j, no pivot search, and a q pivot row (all zero in the other vector
+// component's q columns, which stay zero: A is block diagonal and B^T rows
+// are never pivots before the q columns are done) skips those columns.
+template <int K, bool BLOCKS, bool GROUPED, int SEG, bool NAT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(!BLOCKS && K <= 3 ? 3 : 1)))
 void condense_kernel(CondenseArgs a) {
   using D = Dims<K>;
@@ -405,9 +410,12 @@ void condense_kernel(CondenseArgs a) {
       double* pb = prow + ((threadIdx.x >> 6) * EPW + seg) * PBS;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~63u) | (unsigned)(63 - ls));
-        key = seg_max_int<SEG>(key, seg);
-        const int bl = 63 - (key & 63);
+        int bl = j;
+        if constexpr (!NAT) {
+          int key = used ? -1 : (int)(((unsigned)__double2hiint(fabs(row[j])) & ~63u) | (unsigned)(63 - ls));
+          key = seg_max_int<SEG>(key, seg);
+          bl = 63 - (key & 63);
+        }
         const int src = sbase + bl;
         const double pv = seg_bcast<SEG>(row[j], src);
         bad |= !(fabs(pv) > thr);
@@ -429,7 +437,10 @@ void condense_kernel(CondenseArgs a) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int c = j + 1; c < CS; ++c) row[c] = fma(-f, seg_bcast<SEG>(row[c], src), row[c]);
+        for (int c = j + 1; c < CS; ++c) {
+          if (NAT && c < NQ && c / NU != j / NU) continue;
+          row[c] = fma(-f, seg_bcast<SEG>(row[c], src), row[c]);
+        }
 #pragma unroll
         for (int c = 0; c < NB; c += 2) {
           if (c < c0) continue;
@@ -1032,12 +1043,19 @@ int condense_launch_seg(const CondenseArgs& a, bool blocks, int grid_cap, hipStr
   const int64_t per_block = 4 * (64 / SEG);
   const int grid = grid_for(n, (int)per_block, grid_cap > 0 ? grid_cap : 8192);
   const size_t lds = sizeof(double) * a.lay.total;
+  // natural order only where a rule can make the mass SPD: at k = 4 the
+  // rules with positive weights have too few points
+  const bool nat = a.natural && K <= 3;
   if (blocks)
-    hipLaunchKernelGGL((condense_kernel<K, true, false, SEG>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, true, false, SEG, false>), dim3(grid), dim3(256), lds, s, a);
+  else if (a.srcs && nat)
+    hipLaunchKernelGGL((condense_kernel<K, false, true, SEG, K <= 3>), dim3(grid), dim3(256), lds, s, a);
   else if (a.srcs)
-    hipLaunchKernelGGL((condense_kernel<K, false, true, SEG>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, false, true, SEG, false>), dim3(grid), dim3(256), lds, s, a);
+  else if (nat)
+    hipLaunchKernelGGL((condense_kernel<K, false, false, SEG, K <= 3>), dim3(grid), dim3(256), lds, s, a);
   else
-    hipLaunchKernelGGL((condense_kernel<K, false, false, SEG>), dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL((condense_kernel<K, false, false, SEG, false>), dim3(grid), dim3(256), lds, s, a);
   return (int)hipGetLastError();
 }
 

@@ -312,4 +312,27 @@ bool RefTables::build(int k_, int qdeg_) {
   return true;
 }
 
+bool RefTables::mass_spd() const {
+  for (double w : qw)
+    if (!(w > 0.0)) return false;
+  const int n = lay.nu;
+  std::vector<double> a(data.begin() + lay.Mref, data.begin() + lay.Mref + (size_t)n * n);
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i) dmax = std::max(dmax, a[i * n + i]);
+  if (!(dmax > 0.0)) return false;
+  for (int j = 0; j < n; ++j) {  // Cholesky, lower triangle in place
+    double d = a[j * n + j];
+    for (int t = 0; t < j; ++t) d -= a[j * n + t] * a[j * n + t];
+    if (!(d > 1e-10 * dmax)) return false;
+    const double l = std::sqrt(d);
+    a[j * n + j] = l;
+    for (int i = j + 1; i < n; ++i) {
+      double v = a[i * n + j];
+      for (int t = 0; t < j; ++t) v -= a[i * n + t] * a[j * n + t];
+      a[i * n + j] = v / l;
+    }
+  }
+  return true;
+}
+
 }  // namespace hdg

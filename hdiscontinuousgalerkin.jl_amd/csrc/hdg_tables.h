@@ -46,6 +46,14 @@ struct RefTables {
   // builds the tables; returns false when the quadrature order is
   // unavailable (src/quadrature.jl:24)
   bool build(int k, int qdeg);
+  // True when every cell weight is positive and the reference mass
+  // sum_q w N_i N_j is positive definite (Cholesky pivots above 1e-10 of
+  // the largest diagonal entry): then the q block A = (kappa^-1 v_j, v_i)
+  // of every cell with det J > 0 and kappa > 0 is symmetric positive
+  // definite, and so is the Schur complement C + B^T A^-1 B of a
+  // nonsingular local matrix.  False for the default rules at k = 2
+  // (6 points on a conic) and k = 4 (12 points for 15 functions).
+  bool mass_spd() const;
 };
 
 }  // namespace hdg
