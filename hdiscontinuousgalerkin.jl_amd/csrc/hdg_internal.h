@@ -118,8 +118,7 @@ struct CgVectors {
 // Several independent problems (instances) packed into one mesh share every
 // launch.  Instance j owns rows (or cells) [lo, hi) and the blocks
 // [b0, b0 + nb) of the launch; its blocks visit its rows with stride
-// nb * blockDim (in the order xcd_slot gives them, where a kernel uses it),
-// and its reductions run over its own blocks' partial sums
+// nb * blockDim, and its reductions run over its own blocks' partial sums
 // only, so an instance gets the bits it would get alone with nb blocks.
 // A selector turns the block index into the instance's segment: Whole is
 // one instance spanning the grid, Grouped looks the block up in a table.
@@ -138,23 +137,6 @@ struct Grouped {
 };
 __device__ __forceinline__ Seg seg_of(const Whole& w) { return Seg{0, w.n, 0, (int)gridDim.x, w.st}; }
 __device__ __forceinline__ Seg seg_of(const Grouped& g) { return g.seg[g.binst[blockIdx.x]]; }
-
-// Slot of block b of nb in a grid-stride sweep over tiles, for kernels that
-// gather neighbours of their rows: the blocks that share an XCD (b mod 8:
-// the dispatcher deals workgroups to the eight XCDs in turn) take
-// consecutive slots, so each XCD covers one contiguous eighth of every
-// window of nb tiles that the grid works on at a time.  The neighbour
-// entries a row gathers (faces of the adjacent cells, up to a mesh row
-// away) are then mostly in that XCD's L2 instead of fetched again by
-// another XCD, while the window itself stays one contiguous range.
-// Trace SpMV of 512^2 P2 cells (1024 blocks, all resident): HBM traffic
-// 465 -> 408 MB per launch, 91.3 -> 89.8 us.  A grid larger than what is
-// resident at once (the 4096-block level-0 residual) got slower with it
-// (64 -> 76 us), so only the SpMV uses it.
-__device__ __forceinline__ int xcd_slot(int b, int nb) {
-  const int x = b & 7, q = nb >> 3, rem = nb & 7;
-  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + (b >> 3);
-}
 
 // Views for the CG kernels.  TraceOp<W> is the trace operator stored as W:
 // FP64 for the CG product, FP64 or float for the multigrid preconditioner

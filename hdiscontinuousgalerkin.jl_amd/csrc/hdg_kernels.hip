@@ -706,8 +706,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
   const double* __restrict__ z = v.z;
   double* __restrict__ qo = v.q;
   double part = 0.0;
-  const int slot = xcd_slot(lb, S.nb);
-  for (int64_t r = S.lo + (int64_t)slot * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
+  for (int64_t r = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
     const int64_t f = r / NT;
     const int* bc = P.bcol + f * kSlots;
     double acc = 0.0;
@@ -1077,12 +1076,9 @@ int condense_launch(const CondenseArgs& a, bool blocks, const CondenseSched& sc,
 
 }  // namespace
 
-// A multiple of 8, so that the blocks of an instance in a grouped launch
-// (which gets the blocks it would get alone, one instance after another)
-// start on the first XCD and xcd_slot's XCD is the one a block runs on.
 int cg_blocks(int64_t N, int nt) {
   (void)nt;
-  return (grid_for(N, kCgThreads, 1024) + 7) & ~7;
+  return grid_for(N, kCgThreads, 1024);
 }
 
 int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s) {
