@@ -297,11 +297,16 @@ void condense_kernel(CondenseArgs a) {
   using D = Dims<K>;
   static_assert(SEG >= D::ROWS && 64 % SEG == 0, "segment too small");
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NTR = D::NTR, EPW = 64 / SEG;
-  // border columns of the bordered system (trace block and rhs) and the
-  // stride of a segment's pivot-row buffer (16-byte aligned, and an odd
+  // columns from CS on reach the other lanes through LDS (see below), and
+  // the stride of a segment's pivot-row buffer (16-byte aligned, and an odd
   // number of 16-byte units so the segments of a wavefront start in
-  // different banks)
-  constexpr int CS = M, NB = D::NC - CS, PBS = ((NB + 1) / 2 | 1) * 2;
+  // different banks).  CS = M puts the whole border (trace block and rhs)
+  // there.  Natural order on 64-lane segments has no pivot search, and LDS
+  // issue becomes the limit, so 8 border columns go back to readlanes
+  // (k = 3 on 512^2 cells: 4.52 -> 3.88 ms; the pivoting kernels and
+  // 16/32-lane segments are fastest with the whole border in LDS).
+  constexpr int CS = (NAT && SEG == 64 && M + 8 < D::NC) ? M + 8 : M;
+  constexpr int NB = D::NC - CS, PBS = ((NB + 1) / 2 | 1) * 2;
   extern __shared__ double T[];
   __shared__ __align__(16) double prow[4 * EPW * PBS];
   const TabLayoutDev L = a.lay;
@@ -400,13 +405,13 @@ void condense_kernel(CondenseArgs a) {
       // a 32-bit key: the upper word of |a_ij| with its low 6 bits replaced
       // by 63 - (row index) (ties and near-ties go to the lower row, the
       // same choice for every SEG).
-      // The pivot row reaches the other lanes in two ways: columns below M
+      // The pivot row reaches the other lanes in two ways: columns below CS
       // by cross-lane moves (readlane or bpermute, VALU / LDS-permute work
-      // per column), the border columns (trace block and rhs, present in
-      // every step) through the segment's row in LDS: the pivot lane writes
-      // them, every lane reads them back as broadcast 16-byte loads.  LDS
-      // instructions of one wavefront execute in order, so the reads see
-      // the write and the next step's write follows these reads.
+      // per column), the border columns from CS on (present in every step)
+      // through the segment's row in LDS: the pivot lane writes them, every
+      // lane reads them back as broadcast 16-byte loads.  LDS instructions
+      // of one wavefront execute in order, so the reads see the write and
+      // the next step's write follows these reads.
       double* pb = prow + ((threadIdx.x >> 6) * EPW + seg) * PBS;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
