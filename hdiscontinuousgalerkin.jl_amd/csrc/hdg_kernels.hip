@@ -174,6 +174,17 @@ __device__ inline double seg_bcast(double v, int src) {
   }
 }
 
+// Broadcast for a 32-lane segment by readlanes: the value of lane s0 to
+// the lower half-wave and of lane s1 to the upper one (s0, s1 wave-uniform),
+// four readlanes and a select per double, VALU work instead of the two LDS
+// permutes of seg_bcast.  Every lane of the wavefront must be active.
+__device__ inline double halves_bcast(double v, int s0, int s1, int seg) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int l0 = __builtin_amdgcn_readlane(lo, s0), l1 = __builtin_amdgcn_readlane(lo, s1);
+  const int h0 = __builtin_amdgcn_readlane(hi, s0), h1 = __builtin_amdgcn_readlane(hi, s1);
+  return __hiloint2double(seg ? h1 : h0, seg ? l1 : l0);
+}
+
 // ------------------------------------------------------------- condense
 // v0 or v1 (i = 0, 1) and v0, v1 or v2 (i = 0, 1, 2).  BLEND forms them as
 // sums of products with 0/1 weights, so that the choice stays a choice
@@ -441,10 +452,21 @@ void condense_kernel(CondenseArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // pivoting on 32-lane segments: the first kRl32 (q,u) columns by
+        // readlane pairs, the others by permutes, which balances VALU and
+        // LDS issue (k = 2 on 512^2 cells: 1.91 -> 1.75 ms; 9 columns 1.79,
+        // 16 columns 1.83, all 18 1.95)
+        constexpr int kRl32 = (!NAT && SEG == 32) ? 12 : 0;
+        int s0 = 0, s1 = 0;
+        if constexpr (kRl32 > 0) {
+          s0 = __builtin_amdgcn_readlane(src, 0);
+          s1 = __builtin_amdgcn_readlane(src, 32);
+        }
 #pragma unroll
         for (int c = j + 1; c < CS; ++c) {
           if (NAT && c < NQ && c / NU != j / NU) continue;
-          row[c] = fma(-f, seg_bcast<SEG>(row[c], src), row[c]);
+          const double b = c < kRl32 ? halves_bcast(row[c], s0, s1, seg) : seg_bcast<SEG>(row[c], src);
+          row[c] = fma(-f, b, row[c]);
         }
 #pragma unroll
         for (int c = 0; c < NB; c += 2) {
