@@ -141,9 +141,10 @@ void dfree(hdg_ctx* c, T*& p) {
     c->allocs.erase(it);
   }
   if (c->arena && c->arena->owns((const void*)p)) {
-    // work queued on the stream may still read the buffer
+    // work queued on the stream may still read the buffer; a pointer this
+    // context did not allocate is left alone
     (void)hipStreamSynchronize(c->stream);
-    c->arena->give((void*)p, bytes);
+    if (bytes) c->arena->give((void*)p, bytes);
   } else {
     (void)hipFreeAsync((void*)p, c->stream);
   }
