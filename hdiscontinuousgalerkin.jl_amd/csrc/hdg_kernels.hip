@@ -410,7 +410,8 @@ void condense_kernel(CondenseArgs a) {
       int mycol = 0;
       double mypiv = 1.0;
       // Gauss-Jordan on the first M columns with partial pivoting among
-      // the rows not yet used as pivots (implicit row exchanges).  Pivot rows
+      // the rows not yet used as pivots (implicit row exchanges), or with
+      // NAT row j as the pivot of column j.  Pivot rows
       // stay unscaled (each row is divided by its own pivot at the end), so
       // an elimination step is one fma per column.  The pivot search reduces
       // a 32-bit key: the upper word of |a_ij| with its low 6 bits replaced
