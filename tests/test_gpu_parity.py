@@ -328,6 +328,34 @@ def test_full_size_c2_properties(hdg, O, ctx):
     assert 40.0 < errs[0] / errs[1] < 90.0, errs
 
 
+def test_full_size_p3_mixed_properties(hdg, ctx):
+    """1024x1024 cells (2,097,152 triangles), P3 at quadrature degree 6,
+    jittered: the mixed-precision configuration of the benchmark, 12.6M trace
+    dofs.  Size-independent checks: FP64 and HDG_MIXED solves reach the
+    recursive tolerance, their recomputed FP64 residuals stay at the
+    attainable floor, the two solutions agree within 1e-10 relative (the
+    mixed form's target against FP64), the L2 error against the manufactured
+    solution sits at the FP64 rounding floor, and nnz = (5 n_int + 3 n_bdry) nt^2."""
+    n, k = 1024, 3
+    m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=20261202)
+    ctx.set_mesh(m)
+    d = m.boundary_dofs(k + 1)
+    sols = []
+    for prec in (hdg.HDG_FP64, hdg.HDG_MIXED):
+        ctx.setup(k, quad_degree=6, rtol=1e-12, precision=prec, precond=hdg.PC_MULTIGRID)
+        ctx.set_source()
+        ctx.set_dirichlet(d, np.zeros(len(d)))
+        info = ctx.run()
+        print(f"precision {prec}: iters {info.iters} relres {info.relres:.3g} true {info.true_relres:.3g}", flush=True)
+        assert info.relres <= 1e-12
+        assert info.true_relres < 5e-9, info.true_relres
+        assert 0.0 < ctx.errornorm() < 1e-20  # L2 error < 1e-10: the rounding floor at this size
+        sols.append(ctx.trace())
+    nb = 4 * n
+    assert ctx.sizes.nnz == (5 * (m.faces.shape[0] - nb) + 3 * nb) * (k + 1) ** 2
+    assert rel(sols[1], sols[0]) < 1e-10
+
+
 # ------------------------------------------------------- multigrid path
 @pytest.mark.parametrize("k,n", [(1, 32), (2, 32), (3, 16), (4, 16), (2, 64)])
 def test_multigrid_matches_block_jacobi_and_direct(hdg, O, ctx, k, n):
