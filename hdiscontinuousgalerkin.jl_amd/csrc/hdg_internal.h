@@ -206,7 +206,9 @@ struct MgData {
   double* cinv = nullptr;        // [kCoarseDirect^2] inverse of the coarsest operator (row-major)
   double* tailpack = nullptr;    // stencils (7 n) and dinv (n) of levels tail.. back to back,
                                  // the LDS image the tail kernel loads
-  double omega_v = 0.8;          // vertex-level Jacobi weight
+  // vertex-level Jacobi weight: 0.6-0.7 take one CG iteration fewer than
+  // 0.8 at 512^2 P2 (37 vs 38), 0.9 four more; 1024^2 P3 stays at 29
+  double omega_v = 0.7;
   double omega_factor = 1.6;     // trace-level weight = factor / lambda_max(Dinv S) (Lanczos estimate)
   int coarse_sweeps = 20;
   int vsweeps = 2;               // pre- and post-smoothing sweeps on the vertex levels (1 or 2)
@@ -351,7 +353,7 @@ struct MgGroup {
   VGroup res[kMaxLevels];
   int gres[kMaxLevels];
   int vsweeps = 2;
-  double omega_v = 0.8;
+  double omega_v = 0.7;
 };
 // Set-up tables of a group's vertex hierarchies: level-1 assembly blocks
 // (instance, block) and, per depth d, the instances with a level d+1, with
