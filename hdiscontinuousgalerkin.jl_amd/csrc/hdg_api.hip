@@ -28,6 +28,9 @@ struct hdg_ctx {
   hdg_arena* arena = nullptr;  // serving loop's arena, or null
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream for the multigrid set-up beside the Lanczos steps
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
 
   // mesh
@@ -510,22 +513,32 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   const double rtol = c->opt.rtol > 0 ? c->opt.rtol : 1e-12;
   const int maxit = c->opt.maxit > 0 ? c->opt.maxit : 200000;
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  if (mixed(c)) {
-    const int64_t N = c->T.N;
-    LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
-    LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, c->f_sp, c->stream));
+  if (use_mg(c)) {
+    const bool mx = mixed(c);
+    if (mx) {
+      const int64_t N = c->T.N;
+      LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
+      LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
+    }
+    // The multigrid set-up (small, latency-bound kernels writing the level
+    // arrays and S P0) and the Lanczos estimate of the smoothing weight
+    // (bandwidth-bound, writing the CG vectors, which serve as its scratch,
+    // and the CG state) only read the trace system in common: the set-up
+    // runs on the second stream beside the estimate.
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, mx ? c->f_sp : nullptr, c->aux));
+    HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
     LAUNCHCHK(c, launch_lanczos_omega(c->pat, c->T, c->cg, Whole{c->T.N, c->cg.st}, c->cg.nblocks, nullptr, 1,
                                       c->mg.omega_factor, c->mg.lanczos_steps, c->stream));
-    LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
-  } else if (use_mg(c)) {
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, nullptr, c->stream));
-    // the CG vectors serve as scratch before the solve initialises them
-    LAUNCHCHK(c, launch_lanczos_omega(c->pat, c->T, c->cg, Whole{c->T.N, c->cg.st}, c->cg.nblocks, nullptr, 1,
-                                      c->mg.omega_factor, c->mg.lanczos_steps, c->stream));
-    LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
-    LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (mx) {
+      LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
+    } else {
+      LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op64(c), kv64(c), mv64(c), c->stream));
+      LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
+    }
   } else {
     LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   }
@@ -614,6 +627,9 @@ int hdg_create(hdg_ctx** out, int device, size_t hbm_budget) {
     }
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipHostMalloc((void**)&c->h_st, 2 * sizeof(CgState), hipHostMallocDefault) != hipSuccess) {
     delete c;
@@ -648,6 +664,12 @@ void hdg_destroy(hdg_ctx* c) {
   if (c->h_st) (void)hipHostFree(c->h_st);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream) {
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
