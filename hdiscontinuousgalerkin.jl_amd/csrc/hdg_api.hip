@@ -525,13 +525,25 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
     // (bandwidth-bound, writing the CG vectors, which serve as its scratch,
     // and the CG state) only read the trace system in common: the set-up
     // runs on the second stream beside the estimate.
-    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, mx ? c->f_sp : nullptr, c->aux));
-    HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
+    // A serving slot (a context on the server's arena) stays on one stream:
+    // the other slots keep the device busy, and a second stream per slot
+    // would outnumber the hardware queues.
+    const bool fork = c->arena == nullptr;
+    if (fork && !c->aux) {
+      HIPCHK(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+      HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+      HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    hipStream_t ss = fork ? c->aux : c->stream;
+    if (fork) {
+      HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+      HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    }
+    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, mx ? c->f_sp : nullptr, ss));
+    if (fork) HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
     LAUNCHCHK(c, launch_lanczos_omega(c->pat, c->T, c->cg, Whole{c->T.N, c->cg.st}, c->cg.nblocks, nullptr, 1,
                                       c->mg.omega_factor, c->mg.lanczos_steps, c->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (fork) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (mx) {
       LAUNCHCHK(c, launch_mg_init(c->pat, c->T, op32(c), kv64(c), mv32(c), c->stream));
       LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
@@ -627,9 +639,6 @@ int hdg_create(hdg_ctx** out, int device, size_t hbm_budget) {
     }
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipHostMalloc((void**)&c->h_st, 2 * sizeof(CgState), hipHostMallocDefault) != hipSuccess) {
     delete c;
