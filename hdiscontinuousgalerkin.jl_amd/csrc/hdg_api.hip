@@ -382,10 +382,17 @@ int select_device(hdg_ctx* c) {
   return HDG_OK;
 }
 
+int err_bits(hdg_ctx* c, int flag);
+
 int check_err_flag(hdg_ctx* c) {
   int flag = 0;
   HIPCHK(c, hipMemcpyAsync(&flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return err_bits(c, flag);
+}
+
+// the condensation's error bits as a status
+int err_bits(hdg_ctx* c, int flag) {
   if (flag & 1) return fail(c, HDG_ERR_NEG_JACOBIAN, "det(J) is not positive for at least one cell");
   if (flag & 2)
     return fail(c, HDG_ERR_LOCAL_SINGULAR,
@@ -502,9 +509,12 @@ int launch_chunk(hdg_ctx* c, int slot, double rtol2, int maxit) {
   return HDG_OK;
 }
 
-int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit);
+int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit, int* err_flag);
 
-int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
+// err_flag non-null: the recovery and the read-back of the condensation's
+// error bits are queued with the solve's own read-back (one synchronisation
+// for hdg_run); the recovery counts only if the solve converged.
+int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
   if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
   if (c->jacobi_dirty) {
     LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
@@ -574,14 +584,18 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info) {
   (void)hipEventDestroy(evs[0]);
   (void)hipEventDestroy(evs[1]);
   ++c->solves;
-  return finish_solve(c, info, maxit);
+  return finish_solve(c, info, maxit, err_flag);
 }
 
 // uhat = x (+ Dirichlet values), the FP64 residual, the solve record
-int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit) {
+int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit, int* err_flag) {
   LAUNCHCHK(c, launch_cg_finish(c->T, c->cg, c->d_uhat, c->stream));
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   LAUNCHCHK(c, launch_true_residual(c->pat, c->T, c->d_uhat, c->d_scal + 1, c->cg, c->stream));
+  if (err_flag) {
+    LAUNCHCHK(c, launch_recover(c->k, c->mesh, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream));
+    HIPCHK(c, hipMemcpyAsync(err_flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  }
   CgState st;
   double tr2 = 0.0;
   HIPCHK(c, hipMemcpyAsync(&st, c->cg.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
@@ -597,7 +611,7 @@ int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit) {
     info->seconds = ms * 1e-3;
   }
   c->solved = true;
-  c->recovered = false;
+  c->recovered = err_flag != nullptr && st.done == 1;
   if (st.done == 3)
     return fail(c, HDG_ERR_NOT_CONVERGED, "trace solve broke down at iteration %d (non-positive curvature or NaN)",
                 st.it);
@@ -1096,16 +1110,17 @@ int hdg_run(hdg_ctx* c, hdg_solve_info* info) {
   if (!c) return HDG_ERR_ARG;
   TRY(select_device(c));
   TRY(assemble_impl(c));
-  int r = solve_impl(c, info);
-  if (r != HDG_OK) {
-    // a singular local matrix or a negative Jacobian is reported as such,
-    // not as the failed solve it causes
+  int flag = 0;
+  const int r = solve_impl(c, info, &flag);
+  // a singular local matrix or a negative Jacobian is reported as such, not
+  // as the failed solve it causes (the flag was read with the solve's
+  // read-back, unless the solve stopped before it)
+  if (r != HDG_OK && r != HDG_ERR_NOT_CONVERGED) {
     TRY(check_err_flag(c));
     return r;
   }
-  TRY(recover_impl(c));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return check_err_flag(c);
+  TRY(err_bits(c, flag));
+  return r;
 }
 
 int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs) {
