@@ -212,7 +212,10 @@ struct MgData {
   double omega_factor = 1.6;     // trace-level weight = factor / lambda_max(Dinv S) (Lanczos estimate)
   int coarse_sweeps = 20;
   int vsweeps = 2;               // pre- and post-smoothing sweeps on the vertex levels (1 or 2)
-  int lanczos_steps = 8;         // block-Jacobi PCG (Lanczos) steps behind omega
+  // block-Jacobi PCG (Lanczos) steps behind omega: 6 estimate lambda_max
+  // a little lower than 8, which lands omega * lambda_max between 1.6 and
+  // 1.7, still short of the 1.8 where iterations grow
+  int lanczos_steps = 6;
 };
 // per-order choices of the V-cycle, shared by single problems and batches:
 // V(1,1) on the vertex levels for k <= 2, V(2,2) above (fewest CG
