@@ -153,7 +153,7 @@ def _declare(L):
         "hdg_server_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int]),
         "hdg_server_submit": (C.c_int, [C.c_void_p, _P(Request), _i64]),
         "hdg_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, _P(Result)]),
-        "hdg_server_trace": (C.c_int, [C.c_void_p, C.c_int64, _d]),
+        "hdg_server_trace": (C.c_int, [C.c_void_p, C.c_int64, _d, C.c_int64]),
         "hdg_server_footprint": (C.c_size_t, [_P(Request)]),
         "hdg_server_stats": (C.c_int, [C.c_void_p, _P(ServerSummary)]),
         "hdg_server_destroy": (None, [C.c_void_p]),
@@ -166,7 +166,7 @@ def _declare(L):
         "hdg_set_schedule": (C.c_int, [C.c_void_p, _P(Schedule)]),
         "hdg_batch_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int, _P(Request)]),
         "hdg_batch_run": (C.c_int, [C.c_void_p, _P(Result)]),
-        "hdg_batch_trace": (C.c_int, [C.c_void_p, C.c_int, _d]),
+        "hdg_batch_trace": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_int64]),
         "hdg_batch_info": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_size_t)]),
         "hdg_batch_destroy": (None, [C.c_void_p]),
         "hdg_batch_last_error": (C.c_char_p, [C.c_void_p]),

@@ -21,6 +21,11 @@
 
 using namespace hdg;
 
+std::mutex& hdg::graph_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 struct hdg_ctx {
   int device = 0;
   size_t budget = 0, used = 0;
@@ -161,6 +166,9 @@ void dfree(hdg_ctx* c, T*& p) {
   } while (0)
 
 void drop_graphs(hdg_ctx* c) {
+  // a replay may still be queued on the stream
+  if ((c->gexec[0] || c->gexec[1]) && c->stream) (void)hipStreamSynchronize(c->stream);
+  std::lock_guard<std::mutex> lk(graph_mutex());
   for (auto& g : c->gexec)
     if (g) {
       (void)hipGraphExecDestroy(g);
@@ -240,7 +248,7 @@ void free_mesh(hdg_ctx* c) {
 }
 
 // Cells numbered as rectangle_mesh(TriangleCell, (nx, ny)) numbers them
-// (generate_mesh.jl:290-304): lower triangle (n(i,j), n(i+1,j), n(i,j+1)),
+// (generate_mesh.jl:130-138): lower triangle (n(i,j), n(i+1,j), n(i,j+1)),
 // then upper (n(i+1,j), n(i+1,j+1), n(i,j+1)), n(i,j) = i + (j-1)(nx+1).
 bool detect_structured(int64_t nn, int64_t nc, const int64_t* cells, int* nx_out, int* ny_out) {
   const int64_t nx = cells[2] - 2;
@@ -488,6 +496,7 @@ int enqueue_chunk(hdg_ctx* c, double rtol2, int maxit) {
 int launch_chunk(hdg_ctx* c, int slot, double rtol2, int maxit) {
   const bool eager = (c->opt.schedule & HDG_SCHEDULE_EAGER) != 0 || c->solves == 0;
   if (!eager && !c->gexec[slot]) {
+    std::lock_guard<std::mutex> lk(graph_mutex());
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int r = enqueue_chunk(c, rtol2, maxit);
@@ -775,7 +784,12 @@ int hdg_set_rectangle_mesh(hdg_ctx* c, int64_t nx, int64_t ny, double x0, double
                            double jitter, uint64_t seed) {
   if (!c) return HDG_ERR_ARG;
   if (nx < 1 || ny < 1) return fail(c, HDG_ERR_ARG, "rectangle mesh needs nx, ny >= 1");
-  if (!(x1 > x0) || !(y1 > y0)) return fail(c, HDG_ERR_ARG, "the box must be positively oriented");
+  // the device generator writes the counter-clockwise numbering; a box given
+  // with decreasing sides (whose triangles rectangle_mesh reorders) goes
+  // through the host generator and hdg_set_mesh
+  if (!(x1 > x0) || !(y1 > y0))
+    return fail(c, HDG_ERR_UNSUPPORTED,
+                "the device generator needs x1 > x0 and y1 > y0; use hdg_rectangle_mesh + hdg_set_mesh");
   const int64_t nn = (nx + 1) * (ny + 1), nc = 2 * nx * ny, nf = hdg_rectangle_nfaces(nx, ny);
   if (nn > INT32_MAX || 2 * nf > INT32_MAX || nc > INT32_MAX / 9)
     return fail(c, HDG_ERR_ARG, "mesh too large for 32-bit device indices");

@@ -175,9 +175,12 @@ int upload(hdg_batch* b, Group& g, T** p, const std::vector<T>& h) {
 
 void free_group(hdg_batch* b, Group& g) {
   if (!g.s) return;
-  for (auto& x : g.gexec)
-    if (x) (void)hipGraphExecDestroy(x);
-  (void)hipStreamSynchronize(g.s);  // queued work may still read the buffers
+  (void)hipStreamSynchronize(g.s);  // queued work may still read the buffers and replay the graphs
+  {
+    std::lock_guard<std::mutex> lk(graph_mutex());
+    for (auto& x : g.gexec)
+      if (x) (void)hipGraphExecDestroy(x);
+  }
   for (auto& [p, bytes] : g.owned) {
     if (b->arena && b->arena->owns(p))
       b->arena->give(p, bytes);
@@ -624,6 +627,7 @@ int launch_chunk(hdg_batch* b, Group& g, int slot) {
   CgState* dst = g.h_st + slot * n;
   const bool eager = g.eager || g.runs == 0;
   if (!eager && !g.gexec[slot]) {
+    std::lock_guard<std::mutex> lk(graph_mutex());
     hipGraph_t gr = nullptr;
     BHIP(b, hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal));
     int r = enqueue_chunk(b, g);
@@ -841,12 +845,15 @@ int hdg_batch_run(hdg_batch* b, hdg_result* res) {
   return first_bad;
 }
 
-int hdg_batch_trace(hdg_batch* b, int i, double* uhat) {
+int hdg_batch_trace(hdg_batch* b, int i, double* uhat, int64_t cap) {
   if (!b || !uhat || i < 0 || i >= (int)b->reqs.size()) return HDG_ERR_ARG;
   if (!b->ran) return fail(b, HDG_ERR_STATE, "hdg_batch_run has not been called");
   if (hipSetDevice(b->device) != hipSuccess) return fail(b, HDG_ERR_HIP, "hipSetDevice failed");
   Group& g = b->groups[b->where[i].first];
   const Inst& I = g.in[b->where[i].second];
+  if (cap < I.N)
+    return fail(b, HDG_ERR_ARG, "the trace of instance %d has %lld doubles, more than the buffer's %lld", i,
+                (long long)I.N, (long long)cap);
   BHIP(b, hipMemcpyAsync(uhat, g.uhat + I.row0, sizeof(double) * I.N, hipMemcpyDeviceToHost, g.s));
   BHIP(b, hipStreamSynchronize(g.s));
   return HDG_OK;

@@ -117,7 +117,7 @@ __device__ __forceinline__ void cell_geometry(const DevMesh& m, int64_t c, Geo& 
   g.Ji[1] = -J1 * di;
   g.Ji[2] = -J2 * di;
   g.Ji[3] = J0 * di;
-  const double wn[6] = {-(J1 - J3), J0 - J2, -J3, J2, J1, -J0};  // src/shapes.jl:169-176
+  const double wn[6] = {-(J1 - J3), J0 - J2, -J3, J2, J1, -J0};  // src/shapes.jl:80-87
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     const double l = sqrt(wn[2 * f] * wn[2 * f] + wn[2 * f + 1] * wn[2 * f + 1]);
@@ -212,7 +212,7 @@ __device__ __forceinline__ double pick3(int i, double v0, double v1, double v2) 
 // with A = (kappa^-1 v_j, v_i), B = (u_j, div v_i), C = tau<u_j, w_i>,
 // E = <uhat_j, v_i.n>, F = tau<uhat_j, w_i>, H = <uhat_j, mu_i>
 // (examples/poisson2D_HDG.jl:88-153; the vector basis is block-by-component,
-// VectorFunctionSpaces.jl:189-195).
+// VectorFunctionSpaces.jl:28-57).
 template <int K>
 __device__ __forceinline__ void build_row(int r, const double* T, const TabLayoutDev& L, const Geo& g,
                                  double tau, const double* Aacc, double beacc,

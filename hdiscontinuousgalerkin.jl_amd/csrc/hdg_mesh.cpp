@@ -56,9 +56,6 @@ extern "C" int hdg_rectangle_mesh(int64_t nx, int64_t ny, double x0, double y0,
                                   uint64_t seed, double* xy, int64_t* cells,
                                   int64_t* faces, int8_t* face_side) {
   if (nx < 1 || ny < 1 || !xy || !cells || !faces || !face_side) return HDG_ERR_ARG;
-  // the closed-form numbering assumes no vertex swap in _check_node_data
-  // (generate_mesh.jl:215-223), i.e. a positively oriented box
-  if (!(x1 > x0) || !(y1 > y0)) return HDG_ERR_ARG;
   const int64_t nnx = nx + 1, nny = ny + 1, nf = hdg_rectangle_nfaces(nx, ny);
   const double LL[2] = {x0, y0}, UR[2] = {x1, y1}, LR[2] = {x1, y0}, UL[2] = {x0, y1};
   for (int64_t i = 0; i < nny; ++i) {
@@ -75,6 +72,18 @@ extern "C" int hdg_rectangle_mesh(int64_t nx, int64_t ny, double x0, double y0,
   }
   const Sweep S{nx, ny};
   auto node = [&](int64_t i, int64_t j) { return i + (j - 1) * nnx; };
+  // _check_node_data (generate_mesh.jl:49-57): a triangle whose vertices
+  // run clockwise (a box with one side given in decreasing order) has its
+  // vertices 2 and 3 swapped.  The sign is evaluated as the reference
+  // evaluates it, so that degenerate boxes decide the same way.
+  auto clockwise = [&](int64_t n1, int64_t n2, int64_t n3) {
+    const double* p1 = xy + 2 * (n1 - 1);
+    const double* p2 = xy + 2 * (n2 - 1);
+    const double* p3 = xy + 2 * (n3 - 1);
+    const double ax = p2[0] - p1[0], ay = p2[1] - p1[1];
+    const double bx = p3[0] - p1[0], by = p3[1] - p1[1];
+    return (ax * by - ay * bx) < 0;
+  };
   auto setf = [&](int64_t f, int64_t v1, int64_t v2, int64_t c1, int64_t c2, int8_t side) {
     faces[(f - 1)] = v1;
     faces[(f - 1) + nf] = v2;
@@ -85,27 +94,58 @@ extern "C" int hdg_rectangle_mesh(int64_t nx, int64_t ny, double x0, double y0,
   for (int64_t j = 1; j <= ny; ++j)
     for (int64_t i = 1; i <= nx; ++i) {
       const int64_t lo = 2 * ((j - 1) * nx + (i - 1)) + 1, up = lo + 1;
+      const int64_t a = node(i, j), b = node(i + 1, j), c = node(i, j + 1), d = node(i + 1, j + 1);
+      const bool sl = clockwise(a, b, c), su = clockwise(b, d, c);
+      // With vertices 2 and 3 swapped the local edges (2,3),(3,1),(1,2)
+      // meet the faces in another order: lower (D, B, L) instead of
+      // (D, L, B), upper (T, R, D) instead of (T, D, R); every face is
+      // stored in the reverse traversal.  The ids only change where the
+      // lower triangle of grid cell (1, 1) creates both L and B.
+      int64_t fL = S.L(i, j), fB = S.B(i, j);
+      if (sl && i == 1 && j == 1) {
+        const int64_t t = fL;
+        fL = fB;
+        fB = t;
+      }
+      const int64_t fD = S.D(i, j), fT = S.T(i, j), fR = S.R(i, j);
       int64_t* cl = cells + 6 * (lo - 1);
       int64_t* cu = cells + 6 * (up - 1);
-      cl[0] = node(i, j); cl[1] = node(i + 1, j); cl[2] = node(i, j + 1);
-      cl[3] = S.D(i, j); cl[4] = S.L(i, j); cl[5] = S.B(i, j);
-      cu[0] = node(i + 1, j); cu[1] = node(i + 1, j + 1); cu[2] = node(i, j + 1);
-      cu[3] = S.T(i, j); cu[4] = S.D(i, j); cu[5] = S.R(i, j);
+      if (!sl) {
+        cl[0] = a; cl[1] = b; cl[2] = c;
+        cl[3] = fD; cl[4] = fL; cl[5] = fB;
+      } else {
+        cl[0] = a; cl[1] = c; cl[2] = b;
+        cl[3] = fD; cl[4] = fB; cl[5] = fL;
+      }
+      if (!su) {
+        cu[0] = b; cu[1] = d; cu[2] = c;
+        cu[3] = fT; cu[4] = fD; cu[5] = fR;
+      } else {
+        cu[0] = b; cu[1] = c; cu[2] = d;
+        cu[3] = fT; cu[4] = fR; cu[5] = fD;
+      }
       // faces first met by this grid cell, with the first cell's traversal
-      setf(S.D(i, j), node(i + 1, j), node(i, j + 1), lo, up, 0);
-      if (i == 1) setf(S.L(i, j), node(i, j + 1), node(i, j), lo, 0, 4);
-      if (j == 1) setf(S.B(i, j), node(i, j), node(i + 1, j), lo, 0, 1);
-      setf(S.T(i, j), node(i + 1, j + 1), node(i, j + 1), up,
-           j < ny ? 2 * (j * nx + (i - 1)) + 1 : 0, j < ny ? 0 : 3);
-      setf(S.R(i, j), node(i + 1, j), node(i + 1, j + 1), up,
-           i < nx ? 2 * ((j - 1) * nx + i) + 1 : 0, i < nx ? 0 : 2);
+      if (!sl) setf(fD, b, c, lo, up, 0); else setf(fD, c, b, lo, up, 0);
+      if (i == 1) {
+        if (!sl) setf(fL, c, a, lo, 0, 4); else setf(fL, a, c, lo, 0, 4);
+      }
+      if (j == 1) {
+        if (!sl) setf(fB, a, b, lo, 0, 1); else setf(fB, b, a, lo, 0, 1);
+      }
+      const int64_t cT = j < ny ? 2 * (j * nx + (i - 1)) + 1 : 0;
+      const int64_t cR = i < nx ? 2 * ((j - 1) * nx + i) + 1 : 0;
+      if (!su) setf(fT, d, c, up, cT, j < ny ? 0 : 3); else setf(fT, c, d, up, cT, j < ny ? 0 : 3);
+      if (!su) setf(fR, b, d, up, cR, i < nx ? 0 : 2); else setf(fR, d, b, up, cR, i < nx ? 0 : 2);
     }
   // boundary sets are found as the reference finds them: by exact equality
   // of both end points with a side of the box, bottom, right, top, left in
   // that order; a one-cell face on none of them is an error
   // (_get_rectangular_boundary_sets, generate_mesh.jl:60-89)
   for (int64_t f = 0; f < nf; ++f) {
-    if (faces[f + 3 * nf] != 0) continue;
+    if (faces[f + 3 * nf] != 0) {
+      face_side[f] = 0;
+      continue;
+    }
     const double* p1 = xy + 2 * (faces[f] - 1);
     const double* p2 = xy + 2 * (faces[f + nf] - 1);
     int8_t side = 0;

@@ -1,5 +1,5 @@
 // rectangle_mesh(TriangleCell, (nx, ny), LL, UR) generated on the device
-// (src/generate_mesh.jl:1-18,101-143,267-309): the same node coordinates,
+// (src/generate_mesh.jl:1-18,110-143): the same node coordinates,
 // cells, face numbering and boundary classification as the host generator
 // (hdg_mesh.cpp), bit for bit, plus the structures the solver derives from
 // them (face end points, vertex -> face lists, the four vertices around each
@@ -195,7 +195,10 @@ __global__ void rm_vertex_faces_kernel(RectMeshArgs a) {
   if (a.last && v == nnx * nny - 1) a.vf_ptr[a.node0 + v + 1] = (int)(p0 + d);
 }
 
-// u = value on every dof of every face with one cell, 0 and free elsewhere
+// Every dof of every face with one cell is prescribed, the others are free.
+// u = value on the boundary: the trace basis is the orthonormal Legendre
+// basis on [0, 1] with P0 = 1, so the projection of a constant is `value`
+// on mode 0 and 0 on the higher modes.
 __global__ void boundary_dirichlet_kernel(const int* fcell, int64_t face0, int64_t nf, int nt, unsigned char* dmask,
                                           double* uD, double value) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,7 +206,7 @@ __global__ void boundary_dirichlet_kernel(const int* fcell, int64_t face0, int64
   const int64_t F = face0 + r / nt, R = face0 * nt + r;
   const bool b = fcell[2 * F + 1] < 0;
   dmask[R] = b ? 1 : 0;
-  if (uD) uD[R] = b ? value : 0.0;
+  if (uD) uD[R] = (b && r % nt == 0) ? value : 0.0;
 }
 
 int blocks_for(int64_t n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }

@@ -179,7 +179,7 @@ void run_batch(hdg_server* S, int w, const std::vector<Pending>& batch) {
     for (int i = 0; i < n; ++i)
       if (rq[i].want_trace && res[i].status == HDG_OK) {
         traces[i].resize(res[i].n_dofs);
-        if (hdg_batch_trace(b, i, traces[i].data()) != HDG_OK) traces[i].clear();
+        if (hdg_batch_trace(b, i, traces[i].data(), (int64_t)traces[i].size()) != HDG_OK) traces[i].clear();
       }
   } else {
     for (int i = 0; i < n; ++i) {
@@ -406,13 +406,18 @@ int hdg_server_wait(hdg_server* S, int64_t id, double timeout_s, hdg_result* out
   return HDG_OK;
 }
 
-int hdg_server_trace(hdg_server* S, int64_t id, double* uhat) {
+int hdg_server_trace(hdg_server* S, int64_t id, double* uhat, int64_t cap) {
   if (!S || !uhat) return HDG_ERR_ARG;
   std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->done.find(id);
   if (it == S->done.end() || it->second.trace.empty()) {
     S->err = "no kept trace for this request (want_trace = 0, not completed, or already taken)";
     return HDG_ERR_STATE;
+  }
+  if (cap < (int64_t)it->second.trace.size()) {
+    S->err = "the trace of request " + std::to_string(id) + " has " + std::to_string(it->second.trace.size()) +
+             " doubles, more than the buffer's " + std::to_string(cap);
+    return HDG_ERR_ARG;
   }
   std::memcpy(uhat, it->second.trace.data(), sizeof(double) * it->second.trace.size());
   S->done.erase(it);

@@ -268,7 +268,7 @@ bool RefTables::build(int k_, int qdeg_) {
   }
   // face tables: reference edges in CCW order, edge f opposite vertex f,
   // traversed from its first to its second local vertex
-  // ((2,3),(3,1),(1,2) of src/shapes.jl:108-113).
+  // ((2,3),(3,1),(1,2) of src/shapes.jl:19-24).
   const double ev[3][2][2] = {{{1.0, 0.0}, {0.0, 1.0}}, {{0.0, 1.0}, {0.0, 0.0}},
                               {{0.0, 0.0}, {1.0, 0.0}}};
   std::vector<double> E(3 * nu * nfq), Nt(nt * nfq), lt(nt);

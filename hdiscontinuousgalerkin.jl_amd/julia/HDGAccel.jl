@@ -22,6 +22,7 @@ Set ENV["LIBHDG"] to the path of libhdg.so before loading.
 module HDGAccel
 
 using SparseArrays
+import LinearAlgebra
 using HDiscontinuousGalerkin
 
 const libhdg = get(ENV, "LIBHDG", "libhdg.so")
@@ -237,7 +238,7 @@ end
 """Trace solution of request i (1-based) after run!."""
 function trace(b::Batch, i::Integer, n_dofs::Integer)
     out = Vector{Float64}(undef, n_dofs)
-    st = ccall((:hdg_batch_trace, libhdg), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), b.h, i - 1, out)
+    st = ccall((:hdg_batch_trace, libhdg), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Int64), b.h, i - 1, out, length(out))
     st == 0 || error("hdg_batch_trace failed with status $st")
     out
 end

@@ -38,7 +38,7 @@ class PolygonalMesh:
         return ((fs[:, None] - 1) * nt + np.arange(1, nt + 1)[None, :]).ravel()
 
 
-# accessors of src/mesh.jl:56-77 and src/HDiscontinuousGalerkin.jl:215-226
+# accessors of src/mesh.jl:56-77,117-124
 def getncells(m): return m.cells.shape[0]
 def getnfaces(m): return m.faces.shape[0]
 def getnnodes(m): return m.nodes.shape[0]
@@ -63,7 +63,7 @@ def cell_volume(m, cell):
 
 
 def rectangle_mesh(nel, LL=(0.0, 0.0), UR=(1.0, 1.0), jitter=0.0, seed=0):
-    """rectangle_mesh(TriangleCell, nel, LL, UR) (src/generate_mesh.jl:267-309)
+    """rectangle_mesh(TriangleCell, nel, LL, UR) (src/generate_mesh.jl:96-143)
     through libhdg's closed-form generator.  `jitter` > 0 moves interior
     nodes by up to jitter*h per axis after generation (seeded splitmix64)."""
     nx, ny = int(nel[0]), int(nel[1])
@@ -104,7 +104,7 @@ def _rows(path):
 
 def parse_mesh_triangle(root):
     """parse_mesh_triangle(root) for Triangle's .node/.edge/.ele files
-    (src/triangle_mesh.jl:50-175): cells in file order with the vertex
+    (src/triangle_mesh.jl:1-126): cells in file order with the vertex
     order made counter-clockwise, faces numbered by first encounter over
     local edges (2,3),(3,1),(1,2), and the "boundary" faceset from the
     edge markers."""

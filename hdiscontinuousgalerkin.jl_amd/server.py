@@ -59,6 +59,7 @@ class Server:
         if code != L.HDG_OK:
             raise L.HDGError(code, "hdg_server_create failed")
         self._h = h
+        self._ndofs = {}  # request id -> trace length, from its result
         if batch > 1:
             self._chk(lib().hdg_server_set_batching(self._h, int(batch), int(batch_cells)))
 
@@ -79,11 +80,19 @@ class Server:
     def wait(self, rid, timeout=-1.0):
         r = L.Result()
         self._chk(lib().hdg_server_wait(self._h, int(rid), float(timeout), C.byref(r)))
-        return {f: getattr(r, f) for f, _ in L.Result._fields_}
+        out = {f: getattr(r, f) for f, _ in L.Result._fields_}
+        self._ndofs[int(rid)] = int(out["n_dofs"])
+        return out
 
-    def trace(self, rid, n_dofs):
-        out = np.zeros(int(n_dofs))
-        self._chk(lib().hdg_server_trace(self._h, int(rid), out.ctypes.data_as(C.POINTER(C.c_double))))
+    def trace(self, rid):
+        """The kept trace solution of a completed request (want_trace),
+        sized from the request's own result."""
+        n = self._ndofs.get(int(rid))
+        if n is None:
+            raise L.HDGError(8, f"request {rid} has not been waited for")
+        out = np.zeros(n)
+        self._chk(lib().hdg_server_trace(self._h, int(rid), out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        del self._ndofs[int(rid)]
         return out
 
     def stats(self):
@@ -148,7 +157,7 @@ class Batch:
         if self._ndofs[i] is None:
             raise L.HDGError(8, "run the batch first")
         out = np.zeros(int(self._ndofs[i]))
-        self._chk(lib().hdg_batch_trace(self._h, int(i), out.ctypes.data_as(C.POINTER(C.c_double))))
+        self._chk(lib().hdg_batch_trace(self._h, int(i), out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
         return out
 
     def info(self):

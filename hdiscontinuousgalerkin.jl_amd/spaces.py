@@ -54,7 +54,7 @@ class ScalarFunctionSpace:
 
 class VectorFunctionSpace(ScalarFunctionSpace):
     """VectorFunctionSpace: two scalar blocks, component-major
-    (src/VectorFunctionSpaces.jl:171-179)."""
+    (src/VectorFunctionSpaces.jl:10-18,28-34)."""
 
     def __init__(self, mesh, fe, quad_degree=None):
         super().__init__(mesh, fe, quad_degree)
@@ -63,7 +63,7 @@ class VectorFunctionSpace(ScalarFunctionSpace):
 
 class ScalarTraceFunctionSpace:
     """ScalarTraceFunctionSpace(Wh, fe): Legendre on faces, sharing Wh's face
-    rule (src/TraceFunctionSpaces.jl:235-252)."""
+    rule (src/TraceFunctionSpaces.jl:11-28)."""
 
     def __init__(self, Wh, fe):
         self.fs = Wh
@@ -77,8 +77,8 @@ def getnbasefunctions(fs):
 
 
 def getnlocaldofs(fs):
-    """src/ScalarFunctionSpaces.jl:146, VectorFunctionSpaces.jl:185,
-    TraceFunctionSpaces.jl:262."""
+    """src/ScalarFunctionSpaces.jl:146, VectorFunctionSpaces.jl:24,
+    TraceFunctionSpaces.jl:38."""
     return fs.nbase * 3 if isinstance(fs, ScalarTraceFunctionSpace) else fs.nbase
 
 

@@ -128,7 +128,7 @@ const char* hdg_last_error(const hdg_ctx* ctx);
 /* Bytes of device memory the context holds now. */
 int hdg_memory_used(const hdg_ctx* ctx, size_t* bytes);
 
-/* ---- mesh: replaces rectangle_mesh (src/generate_mesh.jl:267-309) ------ */
+/* ---- mesh: replaces rectangle_mesh (src/generate_mesh.jl:96-143) ------- */
 /* Face count of rectangle_mesh(TriangleCell, (nx, ny), ...). */
 int64_t hdg_rectangle_nfaces(int64_t nx, int64_t ny);
 /* rectangle_mesh(TriangleCell, (nx, ny), LL, UR) into caller buffers
@@ -154,7 +154,7 @@ int hdg_set_mesh(hdg_ctx* ctx, int64_t n_nodes, const double* xy,
                  const int64_t* faces);
 /* ScalarFunctionSpace / VectorFunctionSpace / ScalarTraceFunctionSpace
    construction (src/ScalarFunctionSpaces.jl:24-29, VectorFunctionSpaces.jl:
-   171-179, TraceFunctionSpaces.jl:235-252): reference tables for (k, quad
+   10-18, TraceFunctionSpaces.jl:11-28): reference tables for (k, quad
    degree) go to the device. */
 int hdg_setup(hdg_ctx* ctx, const hdg_options* opt);
 int hdg_set_source(hdg_ctx* ctx, const hdg_source* src);
@@ -319,9 +319,10 @@ int hdg_server_submit(hdg_server* srv, const hdg_request* req, int64_t* id);
    `out` and forgets the request unless it kept a trace.  HDG_ERR_STATE on
    an unknown id, HDG_ERR_NOT_CONVERGED on timeout. */
 int hdg_server_wait(hdg_server* srv, int64_t id, double timeout_s, hdg_result* out);
-/* Copies the kept trace solution of a completed request (n_dofs doubles)
-   and releases it. */
-int hdg_server_trace(hdg_server* srv, int64_t id, double* uhat);
+/* Copies the kept trace solution of a completed request (its result's
+   n_dofs doubles) into uhat, a buffer of cap doubles, and releases it;
+   HDG_ERR_ARG (the trace kept) when cap is smaller. */
+int hdg_server_trace(hdg_server* srv, int64_t id, double* uhat, int64_t cap);
 /* Device bytes a request would hold (mesh, per-cell operators, trace system,
    solver and multigrid data). */
 size_t hdg_server_footprint(const hdg_request* req);
@@ -357,8 +358,9 @@ int hdg_batch_create(hdg_batch** out, int device, size_t hbm_budget, int n, cons
    call).  Returns HDG_OK when every instance converged, else the first
    failing status (each res[i].status holds its own). */
 int hdg_batch_run(hdg_batch* b, hdg_result* res);
-/* Trace solution of request i after hdg_batch_run (res[i].n_dofs doubles). */
-int hdg_batch_trace(hdg_batch* b, int i, double* uhat);
+/* Trace solution of request i after hdg_batch_run (res[i].n_dofs doubles)
+   into uhat, a buffer of cap doubles; HDG_ERR_ARG when cap is smaller. */
+int hdg_batch_trace(hdg_batch* b, int i, double* uhat, int64_t cap);
 /* Number of groups and device bytes held. */
 int hdg_batch_info(const hdg_batch* b, int* ngroups, size_t* device_bytes);
 void hdg_batch_destroy(hdg_batch* b);

@@ -9,7 +9,7 @@
  * Each function cites the reference file:line it restates.  Pins (checks
  * that would fail on a plausible mistake) live in tests/test_oracle_*.py:
  *   - reference literals: test/test_FunctionSpace.jl:39-72 (A, B, C, E, H,
- *     face measures), test/test_ScalarFuncSp.jl:61-79 (Jinv, detJ, detJf,
+ *     face measures), test/test_ScalarFuncSp.jl:15-33 (Jinv, detJ, detJf,
  *     normals), test/test_mesh.jl:11-44 (connectivity, orientation, sets),
  *     test/test_quadrature.jl:6-25, test/test_basis.jl:7-93, and the squared
  *     L2 bounds test/test_FunctionSpace.jl:243, examples/poisson2D_HDG.jl:218;
@@ -451,7 +451,7 @@ int orc_quad1d(int order, double* x, double* w) {
 
 /* ========================================================================= */
 /* Reference element tables (src/ScalarFunctionSpaces.jl:31-99,              */
-/* src/TraceFunctionSpaces.jl:235-252, src/quadrature.jl:52-75)              */
+/* src/TraceFunctionSpaces.jl:11-28, src/quadrature.jl:52-75)              */
 /* ========================================================================= */
 typedef struct {
   int k, qdeg, nu, nt, nqp, nfq;
@@ -466,7 +466,7 @@ typedef struct {
   double Nt[MAXNT * MAXQ];     /* Nt[i*nfq+q] trace basis at 1-D qps */
 } refel;
 
-/* src/shapes.jl:108-112 -- reference edges, CCW, edge i opposite vertex i. */
+/* src/shapes.jl:19-24 -- reference edges, CCW, edge i opposite vertex i. */
 static const double ref_edges[3][2][2] = {
     {{1.0, 0.0}, {0.0, 1.0}}, {{0.0, 1.0}, {0.0, 0.0}}, {{0.0, 0.0}, {1.0, 0.0}}};
 
@@ -524,11 +524,11 @@ int orc_sizes(int k, int qdeg, int* nu, int* nt, int* nqp, int* nfq) {
 /* Mesh                                                                       */
 /* ========================================================================= */
 
-/* generate_mesh.jl:287 */
+/* generate_mesh.jl:121 */
 int64_t orc_rect_nfaces(int64_t nx, int64_t ny) { return nx + ny + 3 * nx * ny; }
 
 /* open-addressing map (min,max) node pair -> face id, standing in for the
-   Dict used by _build_cells (generate_mesh.jl:186-212). */
+   Dict used by _build_cells (generate_mesh.jl:20-46). */
 typedef struct { int64_t* keys; int64_t* vals; int64_t cap; } pairmap;
 static uint64_t mix64(uint64_t z) {
   z ^= z >> 33; z *= 0xff51afd7ed558ccdULL; z ^= z >> 33;
@@ -544,7 +544,7 @@ static int64_t* pm_slot(pairmap* m, int64_t a, int64_t b) {
   }
 }
 
-/* generate_mesh.jl:215-223 */
+/* generate_mesh.jl:49-57 */
 static void check_node_data(const double* xy, int64_t* n) {
   double ax = xy[2 * (n[1] - 1)] - xy[2 * (n[0] - 1)];
   double ay = xy[2 * (n[1] - 1) + 1] - xy[2 * (n[0] - 1) + 1];
@@ -553,7 +553,7 @@ static void check_node_data(const double* xy, int64_t* n) {
   if ((ax * by - ay * bx) < 0) { int64_t t = n[1]; n[1] = n[2]; n[2] = t; }
 }
 
-/* generate_mesh.jl:186-212 */
+/* generate_mesh.jl:20-46 */
 static int64_t build_cell(int64_t face_idx, int64_t n_el, const int64_t* el_nodes,
                           int64_t* cells, int64_t* faces, int64_t nf, pairmap* dict) {
   static const int ren[3][2] = {{1, 2}, {2, 0}, {0, 1}};
@@ -699,7 +699,7 @@ static int reinit(const refel* R, const double* xy, const int64_t* cells,
   G->Jinv[1] = -J[1] * dinv;
   G->Jinv[2] = -J[2] * dinv;
   G->Jinv[3] = J[0] * dinv;
-  /* shapes.jl:169-176 weighted normals */
+  /* shapes.jl:80-87 weighted normals */
   for (int l = 0; l < 3; ++l) {
     double wn[2];
     if (l == 0) { wn[0] = -(J[1] - J[3]); wn[1] = J[0] - J[2]; }
@@ -802,7 +802,7 @@ static int element_blocks_impl(const orc_problem* P, const refel* R,
     }
     double kinv = P->kappa_on ? 1.0 / kappa_at(P, xq, yq) : 1.0;
     for (int i = 0; i < nq; ++i) {
-      int ci = i / nu, ii = i % nu;  /* VectorFunctionSpaces.jl:189-195 */
+      int ci = i / nu, ii = i % nu;  /* VectorFunctionSpaces.jl:28-57 */
       double vh = R->N[ii * nqp + q];
       double div_vh = dNdx[(ii * nqp + q) * 2 + ci];
       for (int j = 0; j < nq; ++j) {
@@ -1077,9 +1077,12 @@ int orc_assemble(int64_t ncells, const int64_t* cells, int64_t nfaces, int nt,
 
 /* ========================================================================= */
 /* Trace Dirichlet data (src/boundary.jl:11-42)                              */
-/* g_kind 0: g = 0; 1: g = g0 + g1 x + g2 y.  quirk = 1 keeps the reference's */
-/* accumulator that is not reset between modes (boundary.jl:27-37); quirk 0  */
-/* is the L2 projection onto the orthonormal trace basis.                    */
+/* g_kind 0: g = 0; 1: g = g0 + g1 x + g2 y.  quirk 0 is the L2 projection   */
+/* onto the orthonormal trace basis.  quirk bit 1 keeps the reference's      */
+/* accumulator, which is not reset between modes (boundary.jl:27-37); bit 2  */
+/* weights the face points with the cell rule's weights fs.fs.qr_weights[q]  */
+/* as the reference does (boundary.jl:33) instead of the face rule's.        */
+/* quirk 3 is the reference's computation.                                   */
 /* ========================================================================= */
 int orc_dirichlet_trace(const orc_problem* P, const double* xy,
                         const int64_t* cells, const int64_t* faces,
@@ -1102,9 +1105,9 @@ int orc_dirichlet_trace(const orc_problem* P, const double* xy,
     int o = orc_face_orientation(cells, cell, lf);
     double N = 0.0;
     for (int i = 0; i < nt; ++i) {
-      if (!quirk) N = 0.0;
+      if (!(quirk & 1)) N = 0.0;
       for (int q = 0; q < nfq; ++q) {
-        int qo = o ? q : nfq - 1 - q; /* TraceFunctionSpaces.jl:271-281 */
+        int qo = o ? q : nfq - 1 - q; /* TraceFunctionSpaces.jl:47-56 */
         double x = 0.0, y = 0.0;
         for (int v = 0; v < 3; ++v) {
           double lv = R.L[(lf * 3 + v) * nfq + qo];
@@ -1112,7 +1115,8 @@ int orc_dirichlet_trace(const orc_problem* P, const double* xy,
           y += lv * xy[2 * (cells[6 * cell + v] - 1) + 1];
         }
         double g = g_kind == 1 ? g_par[0] + g_par[1] * x + g_par[2] * y : 0.0;
-        N += R.fw[q] * g * R.Nt[i * nfq + q];
+        const double wq = (quirk & 2) ? R.qw[q] : R.fw[q];  /* boundary.jl:33 */
+        N += wq * g * R.Nt[i * nfq + q];
       }
       if (vals) vals[kk] = N;
       if (dofs) dofs[kk] = (f + 1) * nt - nt + (i + 1);

@@ -227,7 +227,7 @@ class Mesh:
 
 
 def rectangle_mesh(nx, ny, LL=(0.0, 0.0), UR=(1.0, 1.0), jitter=0.0, seed=0):
-    """rectangle_mesh(TriangleCell, (nx,ny), LL, UR) (generate_mesh.jl:267-309),
+    """rectangle_mesh(TriangleCell, (nx,ny), LL, UR) (generate_mesh.jl:96-143),
     then optional interior jitter (amplitude jitter*h, splitmix64 seed)."""
     L = lib()
     nn = (nx + 1) * (ny + 1)
@@ -270,7 +270,7 @@ def _read_rows(path):
 
 
 def parse_mesh_triangle(root):
-    """parse_mesh_triangle (src/triangle_mesh.jl:50-175), restated."""
+    """parse_mesh_triangle (src/triangle_mesh.jl:1-126), restated."""
     _, node_rows = _read_rows(root + ".node")
     xy = np.array([[float(r[1]), float(r[2])] for r in node_rows])
     _, edge_rows = _read_rows(root + ".edge")
@@ -396,7 +396,12 @@ def assemble(mesh, nt, Ate, bte):
 
 def dirichlet_trace(P, mesh, faceset="boundary", g=None, quirk=False):
     """Dirichlet(u_hat, mesh, faceset, g) (boundary.jl:11-42); g None means
-    g = 0, otherwise (g0, g1, g2) for g = g0 + g1 x + g2 y."""
+    g = 0, otherwise (g0, g1, g2) for g = g0 + g1 x + g2 y.  quirk False is
+    the L2 projection onto the orthonormal trace basis; True is the
+    reference's computation (its accumulator is not reset between modes,
+    boundary.jl:27-37, and the face points are weighted with the cell rule's
+    weights, :33); an int selects those two deviations as bits 1 and 2."""
+    quirk = 3 if quirk is True else int(quirk)
     mask = mesh.boundary_mask(faceset)
     gk, gp = (0, np.zeros(3)) if g is None else (1, np.asarray(g, dtype=np.float64))
     nt = P.k + 1

@@ -24,7 +24,7 @@ def _alone(hdg, reqs):
         for r in reqs:
             res = s.wait(s.submit(r))
             assert res["status"] == 0, res
-            out.append((res, s.trace(res["id"], res["n_dofs"])))
+            out.append((res, s.trace(res["id"])))
     return out
 
 

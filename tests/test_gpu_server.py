@@ -26,7 +26,7 @@ def test_server_alone_equals_mix_and_context(hdg):
     with hdg.Server(slots=1) as s1:
         r1 = s1.wait(s1.submit(x))
         assert r1["status"] == 0
-        t_alone = s1.trace(r1["id"], r1["n_dofs"])
+        t_alone = s1.trace(r1["id"])
     others = [(64, 64, 1), (33, 17, 3), (96, 80, 2), (20, 100, 4), (128, 128, 2), (50, 50, 1),
               (40, 72, 3), (81, 81, 2), (16, 16, 4), (100, 60, 1), (64, 32, 3)]
     with hdg.Server(slots=4) as s4:
@@ -37,7 +37,7 @@ def test_server_alone_equals_mix_and_context(hdg):
                 xid = s4.submit(x)
         res = [s4.wait(i) for i in ids]
         rx = s4.wait(xid)
-        t_mix = s4.trace(xid, rx["n_dofs"])
+        t_mix = s4.trace(xid)
         st = s4.stats()
     assert all(r["status"] == 0 for r in res) and rx["status"] == 0
     assert st["completed"] == len(others) + 1 and st["failed"] == 0
@@ -90,13 +90,13 @@ def test_server_arena_equals_pool(hdg):
         for r in reqs:
             res = s1.wait(s1.submit(r))
             assert res["status"] == 0
-            ref.append((res, s1.trace(res["id"], res["n_dofs"])))
+            ref.append((res, s1.trace(res["id"])))
     foot = max(hdg.footprint(r) for r in reqs)
     for budget in (int(3.5 * foot), int(1.2 * foot)):
         with hdg.Server(slots=3, hbm_budget=budget) as s:
             ids = [s.submit(r) for r in reqs]
             got = [s.wait(i) for i in ids]
-            tr = [s.trace(i, g["n_dofs"]) for i, g in zip(ids, got)]
+            tr = [s.trace(i) for i, g in zip(ids, got)]
             st = s.stats()
         assert st["failed"] == 0 and st["peak_reserved"] <= budget
         for (ra, ta), rb, tb in zip(ref, got, tr):
@@ -141,7 +141,7 @@ def test_server_batching_equals_alone(hdg):
         for r in reqs:
             res = s1.wait(s1.submit(r))
             assert res["status"] == 0
-            alone.append((res, s1.trace(res["id"], res["n_dofs"])))
+            alone.append((res, s1.trace(res["id"])))
     folded = _req(hdg, 16, 16, 1, 799)
     folded.jitter = 0.95
     with hdg.Server(slots=2, batch=16) as sb:
@@ -150,7 +150,7 @@ def test_server_batching_equals_alone(hdg):
         ids = [sb.submit(r) for r in reqs]
         bad = sb.submit(folded)
         res = [sb.wait(i) for i in ids]
-        tr = [sb.trace(i, r["n_dofs"]) for i, r in zip(ids, res)]
+        tr = [sb.trace(i) for i, r in zip(ids, res)]
         rbad = sb.wait(bad)
         assert all(sb.wait(i)["status"] == 0 for i in busy)
     for (ra, ta), rb, tb in zip(alone, res, tr):

@@ -41,6 +41,31 @@ def test_rectangle_mesh_bit_exact(hdg, O, nx, ny, jit):
     assert m.nodesets == o.nodesets
 
 
+@pytest.mark.parametrize("n", [(1, 1), (2, 3), (5, 3), (16, 9)])
+@pytest.mark.parametrize("LL,UR", [((1.0, 0.0), (0.0, 1.0)), ((0.0, 1.0), (1.0, 0.0)), ((1.0, 1.0), (0.0, 0.0)),
+                                   ((2.0, -1.0), (-1.0, 3.0)), ((0.5, 0.0), (0.5, 1.0)), ((0.0, 0.25), (1.0, 0.25))])
+def test_rectangle_mesh_inverted_box(hdg, O, n, LL, UR):
+    """Boxes given with a decreasing side: every triangle comes out
+    clockwise and rectangle_mesh swaps its vertices 2 and 3
+    (_check_node_data, generate_mesh.jl:49-57), which reorders its local
+    faces, reverses every stored face traversal and renumbers the first two
+    boundary faces; both sides decreasing is a rotation and swaps nothing.
+    Degenerate boxes (zero width or height) give a mesh too (the reference
+    fails later, at det(J) <= 0).  Bit-exact against the oracle, which
+    follows the reference's hash-map sweep."""
+    try:
+        o = O.rectangle_mesh(n[0], n[1], LL=LL, UR=UR)
+    except RuntimeError:
+        with pytest.raises(hdg.ArgumentError, match="HDG_ERR_BOUNDARY"):
+            hdg.rectangle_mesh(n, LL=LL, UR=UR)
+        return
+    m = hdg.rectangle_mesh(n, LL=LL, UR=UR)
+    assert np.array_equal(m.cells, o.cells)
+    assert np.array_equal(m.faces, o.faces)
+    assert np.array_equal(m.nodes.view(np.uint64), o.xy.view(np.uint64))
+    assert m.facesets == o.facesets and m.nodesets == o.nodesets
+
+
 @pytest.mark.parametrize("n", [(5, 3), (7, 9), (16, 16)])
 @pytest.mark.parametrize("LL,UR", [((-1.0, 0.25), (2.5, 3.0)), ((-1.0, -2.0), (3.0, 2.0)),
                                    ((0.0, 0.0), (3.0, 7.0)), ((-0.5, 0.25), (1.5, 4.25))])
@@ -64,7 +89,7 @@ def test_rectangle_mesh_rejects_bad_args(hdg):
     with pytest.raises(hdg.ArgumentError):
         hdg.rectangle_mesh((0, 3))
     with pytest.raises(hdg.ArgumentError):
-        hdg.rectangle_mesh((2, 2), LL=(1.0, 0.0), UR=(0.0, 1.0))
+        hdg.rectangle_mesh((3, 0))
 
 
 def test_jittered_mesh_stays_positive(hdg):

@@ -138,14 +138,15 @@ def test_assembly_pattern_and_sparse_semantics(O, k):
 
 def test_dirichlet_projection_and_quirk(O):
     """Trace Dirichlet (boundary.jl:11-42): quirk=0 is the projection onto
-    the orthonormal trace basis; quirk=1 keeps the reference's accumulator
-    that is not reset between modes, i.e. the running sum of quirk=0."""
+    the orthonormal trace basis; quirk bit 1 keeps the reference's
+    accumulator that is not reset between modes, i.e. the running sum of
+    quirk=0."""
     m = O.rectangle_mesh(3, 3, jitter=0.2, seed=2)
     k = 2
     P = O.problem(k=k, qdeg=3)
     g = [0.3, 1.7, -0.9]
     d0, v0 = O.dirichlet_trace(P, m, g=g)
-    d1, v1 = O.dirichlet_trace(P, m, g=g, quirk=True)
+    d1, v1 = O.dirichlet_trace(P, m, g=g, quirk=1)
     full = linear_trace(O, m, k, g)
     np.testing.assert_array_equal(d0, d1)
     np.testing.assert_allclose(v0, full[d0 - 1], atol=1e-15)
@@ -154,6 +155,34 @@ def test_dirichlet_projection_and_quirk(O):
     assert np.all(np.diff(d0) > 0)
     dz, vz = O.dirichlet_trace(P, m)
     np.testing.assert_array_equal(vz, 0.0)
+
+
+def test_dirichlet_reference_weights(O):
+    """The reference weights its face points with the cell rule's weights
+    (fs.fs.qr_weights, boundary.jl:33) and does not reset its accumulator
+    (:27-37); quirk=True reproduces both.  Pinned by closed forms:
+    k = 1 (Strang 3 points of weight 1/6 against Gauss-Legendre 2 points of
+    weight 1/2): every term is a third of the projection's, so the values
+    are a third of its running sum; k = 2, constant g (Strang 6 points of
+    weight 1/12, StrangQuad.jl:12-19, at the 3 Gauss points 1/2, 1/2 +- sqrt(15)/10):
+    mode 0 gets 3 g/12, mode 1 adds 0 (P1 is odd about 1/2), mode 2 adds
+    g/12 (P2(1/2) + 2 P2(1/2 + sqrt(15)/10)) = g/12 sqrt(5) (-1/2 + 4/5)."""
+    m = O.rectangle_mesh(4, 3, jitter=0.2, seed=5)
+    P1 = O.problem(k=1, qdeg=2)
+    g = [0.3, 1.7, -0.9]
+    _, v0 = O.dirichlet_trace(P1, m, g=g)
+    _, vr = O.dirichlet_trace(P1, m, g=g, quirk=True)
+    np.testing.assert_allclose(vr, np.cumsum(v0.reshape(-1, 2), axis=1).ravel() / 3.0, rtol=1e-14, atol=1e-15)
+    # weights alone, accumulator reset: a third of the projection
+    _, vw = O.dirichlet_trace(P1, m, g=g, quirk=2)
+    np.testing.assert_allclose(vw, v0 / 3.0, rtol=1e-14, atol=1e-15)
+    P2 = O.problem(k=2, qdeg=3)
+    c = 1.37
+    _, w0 = O.dirichlet_trace(P2, m, g=[c, 0.0, 0.0])
+    _, wr = O.dirichlet_trace(P2, m, g=[c, 0.0, 0.0], quirk=True)
+    np.testing.assert_allclose(w0.reshape(-1, 3), np.tile([c, 0.0, 0.0], (len(w0) // 3, 1)), atol=1e-15)
+    want = [c / 4, c / 4, c / 4 + c * np.sqrt(5.0) * 0.3 / 12]
+    np.testing.assert_allclose(wr.reshape(-1, 3), np.tile(want, (len(wr) // 3, 1)), rtol=1e-14, atol=1e-15)
 
 
 def test_apply_and_direct_solve_vs_scipy(O):
@@ -223,3 +252,50 @@ def test_triangle_figure1_mesh(O):
     P = O.problem(k=2, qdeg=3)
     r = O.solve_pipeline(P, m)
     assert O.errornorm(P, m, r["u"]) < 1e-4
+
+
+def _sine_series_error(O, P, m, u, coef):
+    """Squared L2 error (errornorm's quadrature: the space's cell rule) of u
+    against u_ex = sum c_pq sin(p pi x) sin(q pi y) / (pi^2 (p^2 + q^2))."""
+    pts, w = O.quad2d(P.qdeg)
+    nu = (P.k + 1) * (P.k + 2) // 2
+    N = np.array([[O.dubiner_value(j + 1, r, s) for j in range(nu)] for r, s in pts])  # (nqp, nu)
+    x = m.xy[m.cells[:, :3] - 1]  # (nc, 3, 2)
+    J = np.stack([x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]], axis=2)
+    detJ = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+    lam = np.stack([1.0 - pts[:, 0] - pts[:, 1], pts[:, 0], pts[:, 1]], axis=1)
+    xq = np.einsum("qv,cvd->cqd", lam, x)
+    ue = np.zeros(xq.shape[:2])
+    for p in range(1, 5):
+        for q in range(1, 5):
+            ue += (coef[(p - 1) * 4 + (q - 1)] / (np.pi ** 2 * (p * p + q * q))
+                   * np.sin(p * np.pi * xq[..., 0]) * np.sin(q * np.pi * xq[..., 1]))
+    uh = u @ N.T
+    return float((((uh - ue) ** 2) * w[None, :]).sum(axis=1) @ detJ)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_sine_series_source_manufactured(O, k):
+    """The serving workload's source f = sum_{p,q<=4} c_pq sin(p pi x)
+    sin(q pi y) (coefficients p-major) has the exact solution
+    u = sum c_pq sin(p pi x) sin(q pi y) / (pi^2 (p^2 + q^2)) with u = 0 on
+    the boundary of the unit square; the discrete u converges to it at rate
+    k+1.  A transposed coefficient index, a wrong frequency or a sign fails
+    (the error then stalls at O(1))."""
+    coef = np.random.default_rng(11).standard_normal(16)
+    P = O.problem(k=k, qdeg=QD[k], src_kind=2, coef=coef)
+    errs = []
+    sizes = (8, 16, 32) if k == 1 else (8, 16)
+    for n in sizes:
+        m = O.rectangle_mesh(n, n, jitter=0.1, seed=3)
+        r = O.solve_pipeline(P, m)
+        errs.append(np.sqrt(_sine_series_error(O, P, m, r["u"], coef)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] > k + 1 - 0.35, (errs, rates)
+    scale = np.sqrt(np.sum((coef / (np.pi ** 2 * 2)) ** 2))
+    assert errs[-1] < 0.05 * scale, errs
+    # the coefficient index is p-major: transposing it changes the source
+    Pt = O.problem(k=k, qdeg=QD[k], src_kind=2, coef=coef.reshape(4, 4).T.ravel())
+    m = O.rectangle_mesh(8, 8, jitter=0.1, seed=3)
+    rt = O.solve_pipeline(Pt, m)
+    assert np.sqrt(_sine_series_error(O, Pt, m, rt["u"], coef)) > 10 * errs[0]

@@ -51,7 +51,7 @@ def test_rectangle_mesh_2x2(O):
 
 @pytest.mark.parametrize("n", [1, 3, 7, 16])
 def test_rectangle_face_count_and_first_encounter(O, n):
-    """nfaces = nx + ny + 3 nx ny (generate_mesh.jl:287); faces are numbered
+    """nfaces = nx + ny + 3 nx ny (generate_mesh.jl:121); faces are numbered
     by first encounter, so face ids along the cell sweep never skip."""
     m = O.rectangle_mesh(n, n + 1)
     assert m.nfaces == n + (n + 1) + 3 * n * (n + 1)
@@ -131,7 +131,7 @@ def test_dubiner_orthonormal(O, k):
 
 
 def test_lagrange_and_legendre(O):
-    """test/test_basis.jl:140-201"""
+    """test/test_basis.jl:32-93"""
     v, g = O.lagrange2d_p1(0.5, 0.5)
     np.testing.assert_allclose(v, [0.0, 0.5, 0.5], atol=np.finfo(float).eps)
     np.testing.assert_allclose(g, [[-1, -1], [1, 0], [0, 1]], atol=np.finfo(float).eps)
@@ -146,7 +146,7 @@ def test_lagrange_and_legendre(O):
 
 # ------------------------------------------------------ function spaces
 def test_fixture_geometry(O, fixture_mesh_root):
-    """test/test_ScalarFuncSp.jl:61-79 and test/test_FunctionSpace.jl:39-41"""
+    """test/test_ScalarFuncSp.jl:15-33 and test/test_FunctionSpace.jl:39-41"""
     m = O.parse_mesh_triangle(fixture_mesh_root)
     invs = [[[1.0, 1.0], [-2.0, 0.0]], [[-1.0, -1.0], [2.0, 0.0]],
             [[1.0, 1.0], [-1.0, 1.0]], [[1.0, -1.0], [0.0, 2.0]]]
