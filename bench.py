@@ -237,6 +237,12 @@ def run_stream(args):
     H = _import_package()
     rng = np.random.default_rng(SEED)
     reqs = stream_requests(H, args.workload, rng)
+    if args.distinct > 0:
+        # a stream that revisits a few problems: a slot whose next request
+        # repeats its mesh and options keeps its set-up and, from the second
+        # solve on, replays captured CG graphs
+        pick = rng.integers(0, args.distinct, size=len(reqs))
+        reqs = [reqs[int(j)] for j in pick]
     if args.eager:
         for r in reqs:
             r.opt.schedule = H._lib.SCHEDULE_EAGER
@@ -284,7 +290,7 @@ def run_stream(args):
                                    if args.workload == "batched" else
                                    f"Poisson arrivals {args.rate}/s for {args.seconds} s, n ~ U{{32..512}}, "
                                    "k ~ U{1..4}, 4x4 N(0,1) sine-series f")),
-                   "slots": args.slots, "hbm_budget_gib": args.hbm_gib,
+                   "slots": args.slots, "hbm_budget_gib": args.hbm_gib, "distinct": args.distinct,
                    "batch": args.batch, "batch_cells": args.batch_cells,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "requests": len(res), "failed": len(res) - len(ok), "cells": int(cells)},
@@ -327,6 +333,8 @@ def main():
                     help="batched / serving: launch every kernel from the host (no CG graphs)")
     ap.add_argument("--via-server", action="store_true",
                     help="batched: run the instances through the serving loop instead of one batch")
+    ap.add_argument("--distinct", type=int, default=0,
+                    help="serving: draw the stream from this many distinct requests (0: all distinct)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also dispatch the plain copy kernel that calibrates HBM traffic counters")
     args = ap.parse_args()

@@ -510,6 +510,7 @@ int launch_chunk(hdg_ctx* c, int slot, double rtol2, int maxit) {
     HIPCHK(c, e);
   }
   if (!eager) {
+    std::lock_guard<std::mutex> lk(graph_mutex());
     HIPCHK(c, hipGraphLaunch(c->gexec[slot], c->stream));
     return HDG_OK;
   }

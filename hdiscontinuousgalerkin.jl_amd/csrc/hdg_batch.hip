@@ -641,6 +641,7 @@ int launch_chunk(hdg_batch* b, Group& g, int slot) {
     BHIP(b, e);
   }
   if (!eager) {
+    std::lock_guard<std::mutex> lk(graph_mutex());
     BHIP(b, hipGraphLaunch(g.gexec[slot], g.s));
   } else {
     BTRY(enqueue_chunk(b, g));

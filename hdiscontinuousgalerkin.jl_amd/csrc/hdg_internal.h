@@ -8,14 +8,14 @@
 
 namespace hdg {
 
-// Serialises stream capture, graph instantiation and graph destruction
-// across every context and batch of the process.  Contexts and batch groups
-// are driven from their own host threads (serving slots, batch groups);
-// the HIP runtime keeps process-wide state for capturing streams and graph
-// objects, and a serving run with 16 slot threads capturing at once ended in
-// a host segmentation fault inside the runtime's capture path.  The lock
-// covers hipStreamBeginCapture .. hipGraphInstantiate and
-// hipGraphExecDestroy; replays (hipGraphLaunch) run unlocked.
+// Serialises every graph call of the process: stream capture through
+// hipGraphInstantiate, hipGraphLaunch and hipGraphExecDestroy.  Contexts
+// and batch groups are driven from host threads of their own, and the HIP
+// runtime keeps process-wide state for capturing streams, graph objects and
+// their kernel-argument memory.  (Serving slots, where 16 threads captured
+// and replayed at once, ended in a host segmentation fault inside the
+// runtime below these calls even with this lock; the slots no longer
+// capture at all.)
 std::mutex& graph_mutex();
 
 constexpr int kSlots = 5;  // block columns per face row: self + 2 faces of each cell

@@ -378,6 +378,14 @@ int hdg_server_submit(hdg_server* S, const hdg_request* req, int64_t* id) {
     return HDG_ERR_HBM_BUDGET;
   }
   Pending p{S->next_id++, *req, S->now(), foot};
+  // The slots launch every kernel from the host and never capture graphs.
+  // A serving run whose slots repeated their set-ups (so that each slot
+  // captured and replayed CG graphs from its own thread) ended in a host
+  // segmentation fault inside the HIP runtime below hipGraph* calls, also
+  // with capture, instantiation, destruction and replay serialised
+  // process-wide; the cause is not established, so the path is off here.
+  // Results are bitwise those of the captured schedule.
+  p.req.opt.schedule |= HDG_SCHEDULE_EAGER;
   *id = p.id;
   S->open[p.id] = 1;
   S->queue.push_back(p);

@@ -261,7 +261,8 @@ int hdg_time_kernel(hdg_ctx* ctx, int which, int reps, double* avg_us);
    the footprints the other slots hold (continuous admission: a slot takes
    the first of the next few queued requests that fits as soon as it is
    free).  A slot keeps its mesh and set-up while consecutive requests share
-   them, so repeated meshes stay device-resident.  Results are bitwise the
+   them, so repeated meshes stay device-resident.  Slots launch every kernel
+   from the host (HDG_SCHEDULE_EAGER is set on every submitted request).  Results are bitwise the
    same whatever else runs in the other slots (no atomics, fixed reduction
    order). */
 typedef struct hdg_server hdg_server;

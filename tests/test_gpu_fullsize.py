@@ -1,0 +1,148 @@
+"""GPU parity at the benchmark's sizes and at the direct-solve tolerance.
+
+- 512 x 512 cells (524,288 triangles), P2, jittered from seed 20261202 (the
+  FP64 drop-in configuration): mesh, face numbering and the CSC pattern of
+  K bit-exact against the oracle; Ate, bte, K_e, b_e of every cell, the
+  assembled values and the rhs to 1e-12 relative (per cell: the largest
+  deviation over the largest entry of the oracle's block).
+- Trace solutions against the oracle's direct (banded LU) solve to 1e-12
+  relative on meshes up to 32 x 32, the sizes at which a direct solve's own
+  rounding (about cond(K) eps) stays below that.
+- The unstructured Triangle fixture figure.1 (62 triangles, irregular face
+  orientation): operators, pattern, solution and error norm against the
+  oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+QD = {1: 2, 2: 3, 3: 6, 4: 6}
+SEED = 20261202
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def per_cell_rel(a, b):
+    """Largest over cells of max|a_c - b_c| / max|b_c|."""
+    a = np.asarray(a).reshape(len(a), -1)
+    b = np.asarray(b).reshape(len(b), -1)
+    return float((np.abs(a - b).max(axis=1) / np.maximum(np.abs(b).max(axis=1), 1e-300)).max())
+
+
+@pytest.fixture(scope="module")
+def ctx(hdg):
+    c = hdg.Context(0)
+    yield c
+    c.close()
+
+
+def test_full_c2_mesh_pattern_and_operators(hdg, O, ctx):
+    n, k = 512, 2
+    print(f"[full C2] {n}x{n} P{k} jitter 0.2 seed {SEED}", flush=True)
+    m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=SEED)
+    om = O.rectangle_mesh(n, n, jitter=0.2, seed=SEED)
+    assert np.array_equal(m.cells, om.cells)
+    assert np.array_equal(m.faces, om.faces)
+    assert np.array_equal(m.nodes.view(np.uint64), om.xy.view(np.uint64))
+    assert m.facesets == om.facesets
+    ctx.set_mesh(m)
+    ctx.setup(k, quad_degree=3, rtol=1e-12)
+    ctx.set_source()
+    d = m.boundary_dofs(k + 1)
+    ctx.set_dirichlet(d, np.zeros(len(d)))
+    ctx.assemble_condense()
+    P = O.problem(k=k, qdeg=3)
+    oA, ob, oK, obe = O.condense(P, om, 0)
+    Ate, bte = ctx.get_condensed()
+    assert per_cell_rel(Ate, oA) < 1e-12
+    assert per_cell_rel(bte, ob) < 1e-12
+    del Ate, bte
+    Ke, be = ctx.get_local_ops()
+    assert per_cell_rel(Ke, oK) < 1e-12
+    assert per_cell_rel(be, obe) < 1e-12
+    del Ke, be, oK, obe
+    K, orhs = O.assemble(om, k + 1, oA, ob)
+    del oA, ob
+    colptr, rowval, nzval, rhs = ctx.get_system()
+    assert np.array_equal(colptr, K.colptr)
+    assert np.array_equal(rowval, K.rowval)
+    assert rel(nzval, K.nzval) < 1e-12
+    assert rel(rhs, orhs) < 1e-12
+    assert ctx.sizes.nnz == 35_398_656 == len(K.nzval)
+    # the device generator gives the same pattern (and solution) bit for bit
+    ctx.set_rectangle_mesh(n, n, jitter=0.2, seed=SEED)
+    ctx.setup(k, quad_degree=3, rtol=1e-12)
+    ctx.set_source()
+    ctx.set_dirichlet_boundary(0.0)
+    ctx.assemble_condense()
+    c2, r2, nz2, rhs2 = ctx.get_system()
+    assert np.array_equal(c2, colptr) and np.array_equal(r2, rowval)
+    assert np.array_equal(nz2, nzval) and np.array_equal(rhs2, rhs)
+
+
+# attained on an MI355X (multigrid PCG to a recursive residual of 1e-15
+# against the banded LU): see the assertion per case
+@pytest.mark.parametrize("k,n,jit", [(1, 10, 0.0), (1, 32, 0.0), (1, 32, 0.2), (2, 16, 0.2), (2, 32, 0.0),
+                                     (3, 12, 0.2), (3, 32, 0.0), (4, 16, 0.2)])
+def test_solution_parity_direct_1e12(hdg, O, ctx, k, n, jit):
+    """û, u and q against K \\ b (poisson2D_HDG.jl:195) to 1e-12 relative.
+    The GPU solve runs its CG to a recursive residual of 1e-15, where the
+    iterate's error is set by the FP64 rounding of the residual; the
+    oracle's banded LU has a rounding error of the same order, about
+    cond(K) eps, which stays below 1e-12 up to 32 x 32 cells."""
+    print(f"[solution 1e-12] k={k} n={n} jitter={jit}", flush=True)
+    m = hdg.rectangle_mesh((n, n), jitter=jit, seed=11)
+    om = O.rectangle_mesh(n, n, jitter=jit, seed=11)
+    P = O.problem(k=k, qdeg=QD[k])
+    r = O.solve_pipeline(P, om)
+    ctx.set_mesh(m)
+    ctx.setup(k, quad_degree=QD[k], rtol=1e-15, maxit=20000)
+    ctx.set_source()
+    ctx.set_dirichlet(r["dofs"], r["vals"])
+    info = ctx.run()
+    uhat = ctx.trace()
+    q, u, _ = ctx.recover()
+    du, duu, dq = rel(uhat, r["uhat"]), rel(u, r["u"]), rel(q, r["q"])
+    print(f"    iters {info.iters} relres {info.relres:.2e} true {info.true_relres:.2e} "
+          f"uhat {du:.2e} u {duu:.2e} q {dq:.2e}", flush=True)
+    assert du < 1e-12 and duu < 1e-12 and dq < 1e-12
+    assert ctx.errornorm() == pytest.approx(O.errornorm(P, om, r["u"]), rel=1e-10)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_figure1_unstructured_vs_oracle(hdg, O, ctx, k):
+    """test/mesh/figure.1 (Triangle output, 62 cells, faces met in file order
+    with mixed orientations): condensed operators, the CSC pattern of K
+    (bit-exact) and values, the solution against the oracle's direct solve
+    and the squared L2 error."""
+    root = os.path.join(GOLDEN, "figure.1")
+    m = hdg.parse_mesh_triangle(root)
+    om = O.parse_mesh_triangle(root)
+    assert np.array_equal(m.cells, om.cells) and np.array_equal(m.faces, om.faces)
+    P = O.problem(k=k, qdeg=QD[k])
+    r = O.solve_pipeline(P, om)
+    ctx.set_mesh(m)
+    ctx.setup(k, quad_degree=QD[k], rtol=1e-15, maxit=20000)
+    ctx.set_source()
+    ctx.set_dirichlet(r["dofs"], r["vals"])
+    ctx.assemble_condense()
+    Ate, bte = ctx.get_condensed()
+    Ke, be = ctx.get_local_ops()
+    assert per_cell_rel(Ate, r["Ate"]) < 1e-12 and per_cell_rel(bte, r["bte"]) < 1e-12
+    assert per_cell_rel(Ke, r["Ke"]) < 1e-12 and per_cell_rel(be, r["be"]) < 1e-12
+    colptr, rowval, nzval, rhs = ctx.get_system()
+    assert np.array_equal(colptr, r["K"].colptr) and np.array_equal(rowval, r["K"].rowval)
+    assert rel(nzval, r["K"].nzval) < 1e-12 and rel(rhs, r["rhs"]) < 1e-12
+    uhat, info = ctx.solve()
+    q, u, _ = ctx.recover()
+    assert info.relres <= 1e-15
+    assert rel(uhat, r["uhat"]) < 1e-12 and rel(u, r["u"]) < 1e-12 and rel(q, r["q"]) < 1e-12
+    assert ctx.errornorm() == pytest.approx(O.errornorm(P, om, r["u"]), rel=1e-10)
