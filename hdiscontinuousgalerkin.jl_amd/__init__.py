@@ -14,7 +14,7 @@ mirror of the reference's API.
 """
 from ._lib import (ArgumentError, HDGError, SingularException, HDG_FP64, HDG_MIXED, lib,  # noqa: F401
                    LIB_PATH, EXPORTS, PC_AUTO, PC_BLOCK_JACOBI, PC_MULTIGRID, SCHEDULE_EAGER,
-                   SCHEDULE_AUTOTUNE)
+                   SCHEDULE_AUTOTUNE, SCHEDULE_LU)
 from .context import Context  # noqa: F401
 from .server import Batch, Server, footprint, make_request  # noqa: F401
 from .hdg import (apply_bc, default_context, doassemble, errornorm, get_usigma, hdg_solve,  # noqa: F401

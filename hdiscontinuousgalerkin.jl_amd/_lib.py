@@ -21,6 +21,7 @@ HDG_FP64, HDG_MIXED = 0, 1
 PC_AUTO, PC_BLOCK_JACOBI, PC_MULTIGRID = 0, 1, 2
 SCHEDULE_EAGER = 1
 SCHEDULE_AUTOTUNE = 2
+SCHEDULE_LU = 4
 SRC_REFERENCE, SRC_KAPPA, SRC_SINE_SERIES, SRC_VALUES = 0, 1, 2, 3
 UEX_SINSIN, UEX_LINEAR = 0, 1
 

@@ -55,6 +55,9 @@ struct hdg_ctx {
   RefTables tab;
   double* d_tab = nullptr;
   TabLayoutDev lay{};
+  // SPD condensation tables (kappa = 1 and an SPD reference mass), or null
+  double* d_spd = nullptr;
+  SpdLayoutDev spd{};
   SrcParams src;
   double* d_fqp = nullptr;
 
@@ -180,6 +183,7 @@ void drop_graphs(hdg_ctx* c) {
 void free_problem(hdg_ctx* c) {
   drop_graphs(c);
   dfree(c, c->d_tab);
+  dfree(c, c->d_spd);
   dfree(c, c->d_fqp);
   dfree(c, c->d_dmask);
   dfree(c, c->d_uD);
@@ -423,11 +427,27 @@ CondenseArgs condense_args(const hdg_ctx* c) {
   a.be = c->d_be;
   a.err = c->d_err;
   a.natural = c->tab.mass_spd() ? 1 : 0;
+  // the Schur-complement form where A = detJ diag(Mref, Mref): kappa = 1,
+  // an SPD reference mass, unless the schedule asks for the elimination
+  if (c->d_spd && !c->src.kappa_on && !(c->opt.schedule & HDG_SCHEDULE_LU)) {
+    a.spd_tables = c->d_spd;
+    a.spd = c->spd;
+  }
   return a;
 }
 
+// condensation of every cell by the form condense_args selects
+int launch_condense_cells(const hdg_ctx* c, const CondenseArgs& a) {
+  if (a.spd_tables) return launch_condense_spd(c->k, a, c->csched, c->stream);
+  return launch_condense(c->k, a, false, c->stream, c->csched);
+}
+
 int autotune_impl(hdg_ctx* c) {
-  LAUNCHCHK(c, tune_condense(c->k, condense_args(c), c->stream, &c->csched, &c->tuned_us, &c->default_us));
+  const CondenseArgs a = condense_args(c);
+  if (a.spd_tables)
+    LAUNCHCHK(c, tune_condense_spd(c->k, a, c->stream, &c->csched, &c->tuned_us, &c->default_us));
+  else
+    LAUNCHCHK(c, tune_condense(c->k, a, c->stream, &c->csched, &c->tuned_us, &c->default_us));
   c->tuned = true;
   c->assembled = c->solved = c->recovered = false;
   return HDG_OK;
@@ -437,7 +457,7 @@ int assemble_impl(hdg_ctx* c) {
   if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
   if ((c->opt.schedule & HDG_SCHEDULE_AUTOTUNE) && !c->tuned) TRY(autotune_impl(c));
   HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
-  LAUNCHCHK(c, launch_condense(c->k, condense_args(c), false, c->stream, c->csched));
+  LAUNCHCHK(c, launch_condense_cells(c, condense_args(c)));
   LAUNCHCHK(c, launch_fill(c->pat, c->d_Ate, c->d_bte, c->T, c->stream));
   LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
   c->jacobi_dirty = false;
@@ -908,6 +928,18 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   c->lay = TabLayoutDev{L.nu, L.nt, L.nqp, L.nfq, L.Mref, L.Dref, L.Cref, L.G, L.Href, L.Nw, L.N, L.Mg, L.W, L.total};
   TRY(dalloc(c, &c->d_tab, L.total));
   HIPCHK(c, hipMemcpyAsync(c->d_tab, tab.data.data(), sizeof(double) * L.total, hipMemcpyHostToDevice, c->stream));
+  {
+    SpdLayout sl;
+    std::vector<double> sd;
+    if (o.k <= 3 && build_spd_tables(tab, sl, sd)) {
+      c->spd = SpdLayoutDev{sl.nu, sl.nt, sl.nqp, sl.R, sl.C, sl.Gam, sl.Z, sl.V, sl.P, sl.Om, sl.H, sl.Nw, sl.Mg,
+                            sl.total};
+      TRY(dalloc(c, &c->d_spd, sl.total));
+      HIPCHK(c, hipMemcpyAsync(c->d_spd, sd.data(), sizeof(double) * sl.total, hipMemcpyHostToDevice, c->stream));
+      // the upload reads sd: finish it before sd goes out of scope
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+  }
   const int64_t nc = c->mesh.nc, nf = c->mesh.nf, N = nf * c->nt;
   TRY(dalloc(c, &c->d_Ate, nc * c->ntr * c->ntr));
   TRY(dalloc(c, &c->d_bte, nc * c->ntr));
@@ -1312,7 +1344,7 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
     if (which == 0) return launch_cg_spmv(c->pat, op64(c), kv64(c), 2, 0.0, 0, 1, c->stream);
     if (which == 1) return launch_cg_update(c->T, c->cg, 2, 1, c->stream);
     if (which == 3) return launch_copy(c->T.vals, scratch, ncopy, c->stream);
-    return launch_condense(c->k, a, false, c->stream, c->csched);
+    return launch_condense_cells(c, a);
   };
   int rc = once();  // warm-up
   if (rc) { dfree(c, scratch); return fail(c, HDG_ERR_HIP, "kernel launch failed"); }

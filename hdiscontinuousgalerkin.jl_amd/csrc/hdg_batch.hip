@@ -49,6 +49,7 @@ struct LevelBuf {
 
 struct Group {
   hdg_options opt{};
+  int kappa = 0;  // the instances' sources weight (q, v) with kappa (groups never mix)
   int k = 0, nt = 0, ntr = 0, m = 0;
   bool mg = false, mixed = false, eager = false;
   std::vector<Inst> in;
@@ -63,6 +64,8 @@ struct Group {
   CgVectors cg{};
   RefTables tab;
   TabLayoutDev lay{};
+  double* d_spd = nullptr;  // Schur-complement condensation tables (no kappa in the group), or null
+  SpdLayoutDev spd{};
   double *d_tab = nullptr, *Ate = nullptr, *bte = nullptr, *Ke = nullptr, *be = nullptr, *qu = nullptr,
          *uhat = nullptr;
   unsigned char* dmask = nullptr;
@@ -227,7 +230,16 @@ CondenseArgs group_condense_args(const Group& g) {
   a.be = g.be;
   a.err = g.err;
   a.natural = g.tab.mass_spd() ? 1 : 0;
+  if (g.d_spd) {
+    a.spd_tables = g.d_spd;
+    a.spd = g.spd;
+  }
   return a;
+}
+
+int launch_group_condense(const Group& g, const CondenseArgs& a) {
+  if (a.spd_tables) return launch_condense_spd(g.k, a, g.csched, g.s);
+  return launch_condense(g.k, a, false, g.s, g.csched);
 }
 
 // With HDG_SCHEDULE_AUTOTUNE: the condensation schedule that runs fastest
@@ -235,7 +247,11 @@ CondenseArgs group_condense_args(const Group& g) {
 // run).
 int tune_group(hdg_batch* b, Group& g) {
   if (!(g.opt.schedule & HDG_SCHEDULE_AUTOTUNE)) return HDG_OK;
-  BLAUNCH(b, tune_condense(g.k, group_condense_args(g), g.s, &g.csched, nullptr, nullptr));
+  const CondenseArgs a = group_condense_args(g);
+  if (a.spd_tables)
+    BLAUNCH(b, tune_condense_spd(g.k, a, g.s, &g.csched, nullptr, nullptr));
+  else
+    BLAUNCH(b, tune_condense(g.k, a, g.s, &g.csched, nullptr, nullptr));
   return HDG_OK;
 }
 
@@ -337,6 +353,15 @@ int build_group(hdg_batch* b, Group& g) {
     std::vector<SrcParams> sp(n);
     for (int j = 0; j < n; ++j) sp[j] = src_params(b->reqs[g.in[j].req].src);
     BTRY(upload(b, g, &g.srcs, sp));
+    // the Schur-complement condensation where the group has no kappa (as a
+    // context on its own would condense each of its instances)
+    SpdLayout sl;
+    std::vector<double> sd;
+    if (!g.kappa && g.k <= 3 && !(g.opt.schedule & HDG_SCHEDULE_LU) && build_spd_tables(tab, sl, sd)) {
+      g.spd = SpdLayoutDev{sl.nu, sl.nt, sl.nqp, sl.R, sl.C, sl.Gam, sl.Z, sl.V, sl.P, sl.Om, sl.H, sl.Nw, sl.Mg,
+                           sl.total};
+      BTRY(upload(b, g, &g.d_spd, sd));
+    }
   }
   const int64_t nc = g.nc, N = g.N;
   BTRY(galloc(b, g, &g.Ate, nc * g.ntr * g.ntr));
@@ -572,7 +597,7 @@ int maxit_of(const Group& g) { return g.opt.maxit > 0 ? g.opt.maxit : 200000; }
 // condensation, assembly, preconditioner set-up, update(0)
 int enqueue_setup(hdg_batch* b, Group& g) {
   BHIP(b, hipMemsetAsync(g.err, 0, sizeof(int) * g.in.size(), g.s));
-  BLAUNCH(b, launch_condense(g.k, group_condense_args(g), false, g.s, g.csched));
+  BLAUNCH(b, launch_group_condense(g, group_condense_args(g)));
   BLAUNCH(b, launch_fill(g.pat, g.Ate, g.bte, g.T, g.s));
   BLAUNCH(b, launch_jacobi(g.pat, g.T, g.s));
   const Grouped gr = rows(g);
@@ -729,13 +754,17 @@ int hdg_batch_create_in_arena(hdg_batch** out, int device, size_t hbm_budget, in
                        i, q.src.kind));
     if (q.uex_kind != -1 && q.uex_kind != HDG_UEX_SINSIN)
       return bail(fail(b, HDG_ERR_ARG, "request %d: batches report HDG_UEX_SINSIN error norms only", i));
+    // groups never mix sources with and without kappa: without it a group
+    // may condense through the Schur complement, as one context would
+    const int kap = q.src.kappa_on ? 1 : 0;
     int gi = -1;
     for (size_t t = 0; t < b->groups.size(); ++t)
-      if (same_group(b->groups[t].opt, q.opt)) gi = (int)t;
+      if (same_group(b->groups[t].opt, q.opt) && b->groups[t].kappa == kap) gi = (int)t;
     if (gi < 0) {
       b->groups.emplace_back();
       Group& g = b->groups.back();
       g.opt = q.opt;
+      g.kappa = kap;
       g.mg = q.opt.precond != HDG_PC_BLOCK_JACOBI;
       g.mixed = g.mg && q.opt.precision == HDG_MIXED;
       g.eager = (q.opt.schedule & HDG_SCHEDULE_EAGER) != 0;

@@ -55,6 +55,12 @@ struct TabLayoutDev {
   int Mref, Dref, Cref, G, Href, Nw, N, Mg, W, total;
 };
 
+// Offsets of the SPD condensation tables (hdg_tables.h SpdLayout).
+struct SpdLayoutDev {
+  int nu, nt, nqp;
+  int R, C, Gam, Z, V, P, Om, H, Nw, Mg, total;
+};
+
 // Element build + condensation.  Per cell outputs: Ate row-major ntr x ntr,
 // bte[ntr], Ke column-major m x ntr, be[m].  Element blocks mode writes the
 // A..H blocks instead (column-major per cell) for cells [first, first+count).
@@ -74,6 +80,9 @@ struct CondenseArgs {
   double* be = nullptr;
   int* err = nullptr;  // bit 1: det(J) <= 0; bit 2: singular local matrix (grouped: one int per instance)
   int natural = 0;     // the tables' mass is SPD (RefTables::mass_spd): eliminate in natural order
+  // SPD condensation (kappa = 1, SPD reference mass): its tables, or null
+  const double* spd_tables = nullptr;
+  SpdLayoutDev spd{};
   // element-block mode
   int64_t first = 0, count = 0;
   double *A = nullptr, *B = nullptr, *C = nullptr, *E = nullptr, *F = nullptr, *H = nullptr,
@@ -269,10 +278,21 @@ int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s);
 struct CondenseSched {
   int seg = 0;
   int grid = 0;
+  // the Schur-complement kernel: waves per SIMD it is compiled for (2 or 3;
+  // 0 = 3) and its grid cap (0 = 16384)
+  int spd_wpe = 0;
+  int spd_grid = 0;
 };
 int launch_condense(int k, const CondenseArgs& a, bool blocks_mode, hipStream_t s,
                     const CondenseSched& sc = CondenseSched{});
 int condense_native_seg(int k);
+// Condensation through the Schur complement on the u block for kappa = 1
+// and an SPD reference mass (a.spd_tables set, a.src without kappa): same
+// outputs as launch_condense up to rounding.  Its autotuner times the
+// (spd_wpe, spd_grid) schedules and keeps the fastest in *best.
+int launch_condense_spd(int k, const CondenseArgs& a, const CondenseSched& sc, hipStream_t s);
+int tune_condense_spd(int k, const CondenseArgs& a, hipStream_t s, CondenseSched* best, double* best_us,
+                      double* default_us);
 // Autotuner: times every (seg, grid) schedule of the condensation kernel on
 // the cells of a (3 warm launches each, HIP events on stream s) and returns
 // the fastest; the default schedule's time in default_us.  Overwrites the

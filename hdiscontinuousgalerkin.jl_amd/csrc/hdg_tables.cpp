@@ -1,6 +1,7 @@
 // Reference-element tables (host).  See hdg_tables.h.
 #include "hdg_tables.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -332,6 +333,132 @@ bool RefTables::mass_spd() const {
       a[i * n + j] = v / l;
     }
   }
+  return true;
+}
+
+bool build_spd_tables(const RefTables& rt, SpdLayout& L, std::vector<double>& data) {
+  if (!rt.mass_spd()) return false;
+  const TableLayout& T = rt.lay;
+  const int nu = T.nu, nt = T.nt, nqp = T.nqp;
+  const double* d = rt.data.data();
+  // The derived tables enter every cell alike, so their rounding errors do
+  // not average out over the mesh: they perturb the constant-trace identity
+  // Ate 1 = 0 of all cells the same way, an error the trace solve amplifies
+  // by cond(K) ~ h^-2.  They are therefore formed in extended precision
+  // from the base tables and rounded once.
+  using X = long double;
+  // Mi = Mref^-1 by Gauss-Jordan with partial pivoting (Mref is SPD and
+  // close to the identity for an orthonormal basis)
+  std::vector<X> a(d + T.Mref, d + T.Mref + nu * nu), mi(nu * nu, 0.0L);
+  for (int i = 0; i < nu; ++i) mi[i * nu + i] = 1.0L;
+  for (int j = 0; j < nu; ++j) {
+    int p = j;
+    for (int i = j + 1; i < nu; ++i)
+      if (std::fabs(a[i * nu + j]) > std::fabs(a[p * nu + j])) p = i;
+    for (int c = 0; c < nu; ++c) {
+      std::swap(a[j * nu + c], a[p * nu + c]);
+      std::swap(mi[j * nu + c], mi[p * nu + c]);
+    }
+    const X piv = a[j * nu + j];
+    for (int c = 0; c < nu; ++c) {
+      a[j * nu + c] /= piv;
+      mi[j * nu + c] /= piv;
+    }
+    for (int i = 0; i < nu; ++i) {
+      if (i == j) continue;
+      const X f = a[i * nu + j];
+      for (int c = 0; c < nu; ++c) {
+        a[i * nu + c] -= f * a[j * nu + c];
+        mi[i * nu + c] -= f * mi[j * nu + c];
+      }
+    }
+  }
+  auto D = [&](int dd, int i, int j) { return (X)d[T.Dref + dd * nu * nu + i * nu + j]; };
+  auto Gm = [&](int fo, int i, int t) { return (X)d[T.G + (fo * nu + i) * nt + t]; };
+  L.nu = nu;
+  L.nt = nt;
+  L.nqp = nqp;
+  int o = 0;
+  L.R = o; o += 3 * nu * nu;
+  L.C = o; o += 3 * nu * nu;
+  L.Gam = o; o += 6 * nu * nt;
+  L.Z = o; o += 12 * nu * nt;
+  L.V = o; o += 6 * nu * nt;
+  L.P = o; o += 2 * nu * nu;
+  L.Om = o; o += 36 * nt * nt;
+  L.H = o; o += nt * nt;
+  L.Nw = o; o += nu * nqp;
+  L.Mg = o; o += 3 * nqp;
+  L.total = o;
+  data.assign(o, 0.0);
+  // P_d = Mi D_d, V_fo = Mi Gam_fo
+  std::vector<X> P(2 * nu * nu), V(6 * nu * nt);
+  for (int dd = 0; dd < 2; ++dd)
+    for (int i = 0; i < nu; ++i)
+      for (int j = 0; j < nu; ++j) {
+        X v = 0.0L;
+        for (int t = 0; t < nu; ++t) v += mi[i * nu + t] * D(dd, t, j);
+        P[(dd * nu + i) * nu + j] = v;
+        data[L.P + (dd * nu + i) * nu + j] = (double)v;
+      }
+  for (int fo = 0; fo < 6; ++fo)
+    for (int i = 0; i < nu; ++i)
+      for (int t = 0; t < nt; ++t) {
+        X v = 0.0L;
+        for (int j = 0; j < nu; ++j) v += mi[i * nu + j] * Gm(fo, j, t);
+        V[(fo * nu + i) * nt + t] = v;
+        data[L.V + (fo * nu + i) * nt + t] = (double)v;
+      }
+  // R_de = D_d' Mi D_e = D_d' P_e (symmetric up to rounding: symmetrised)
+  for (int i = 0; i < nu; ++i)
+    for (int j = i; j < nu; ++j) {
+      X r[2][2][2];  // [d][e][(i,j) or (j,i)]
+      for (int dd = 0; dd < 2; ++dd)
+        for (int e = 0; e < 2; ++e) {
+          X v = 0.0L, w = 0.0L;
+          for (int t = 0; t < nu; ++t) {
+            v += D(dd, t, i) * P[(e * nu + t) * nu + j];
+            w += D(dd, t, j) * P[(e * nu + t) * nu + i];
+          }
+          r[dd][e][0] = v;
+          r[dd][e][1] = w;
+        }
+      const X m0 = 0.5L * (r[0][0][0] + r[0][0][1]);
+      const X m1 = 0.5L * (r[0][1][0] + r[1][0][0] + r[0][1][1] + r[1][0][1]);
+      const X m2 = 0.5L * (r[1][1][0] + r[1][1][1]);
+      const X mm[3] = {m0, m1, m2};
+      for (int m = 0; m < 3; ++m)
+        data[L.R + (m * nu + i) * nu + j] = data[L.R + (m * nu + j) * nu + i] = (double)mm[m];
+    }
+  for (int x = 0; x < 3 * nu * nu; ++x) data[L.C + x] = d[T.Cref + x];
+  for (int x = 0; x < 6 * nu * nt; ++x) data[L.Gam + x] = d[T.G + x];
+  // Z_dfo = D_d' V_fo
+  for (int dd = 0; dd < 2; ++dd)
+    for (int fo = 0; fo < 6; ++fo)
+      for (int i = 0; i < nu; ++i)
+        for (int t = 0; t < nt; ++t) {
+          X v = 0.0L;
+          for (int j = 0; j < nu; ++j) v += D(dd, j, i) * V[(fo * nu + j) * nt + t];
+          data[L.Z + ((dd * 6 + fo) * nu + i) * nt + t] = (double)v;
+        }
+  // Om_fo,gp = Gam_fo' Mi Gam_gp, exactly symmetric
+  for (int fo = 0; fo < 6; ++fo)
+    for (int gp = fo; gp < 6; ++gp)
+      for (int t = 0; t < nt; ++t)
+        for (int u = 0; u < nt; ++u) {
+          if (gp == fo && u < t) continue;
+          X v = 0.0L, w = 0.0L;
+          for (int i = 0; i < nu; ++i) {
+            v += Gm(fo, i, t) * V[(gp * nu + i) * nt + u];
+            w += Gm(gp, i, u) * V[(fo * nu + i) * nt + t];
+          }
+          const double s = (double)(0.5L * (v + w));
+          data[L.Om + ((fo * 6 + gp) * nt + t) * nt + u] = s;
+          data[L.Om + ((gp * 6 + fo) * nt + u) * nt + t] = s;
+        }
+  for (int x = 0; x < nt * nt; ++x) data[L.H + x] = d[T.Href + x];
+  for (int x = 0; x < nu * nqp; ++x) data[L.Nw + x] = d[T.Nw + x];
+  for (int x = 0; x < 3 * nqp; ++x) data[L.Mg + x] = d[T.Mg + x];
   return true;
 }
 

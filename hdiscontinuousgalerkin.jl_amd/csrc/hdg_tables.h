@@ -56,4 +56,27 @@ struct RefTables {
   bool mass_spd() const;
 };
 
+// Tables of the condensation for kappa = 1 and an SPD reference mass
+// (RefTables::mass_spd).  The q block of an affine cell is then
+// A = detJ diag(Mref, Mref), so every product through A^-1 in the Schur
+// complement S = C + B' A^-1 B, in R = F + B' A^-1 E, in E' A^-1 E and in
+// the recovered q is a combination of constant matrices with a few scalars
+// of the cell's geometry (Mi = Mref^-1, D_d = Dref[d], Gam = G):
+struct SpdLayout {
+  int nu, nt, nqp;
+  int R;    // [3][nu][nu]        D0' Mi D0, D0' Mi D1 + D1' Mi D0, D1' Mi D1
+  int C;    // [3][nu][nu]        Cref
+  int Gam;  // [3][2][nu][nt]     G (face f, orientation o)
+  int Z;    // [2][3][2][nu][nt]  D_d' Mi Gam_fo
+  int V;    // [3][2][nu][nt]     Mi Gam_fo
+  int P;    // [2][nu][nu]        Mi D_d
+  int Om;   // [6][6][nt][nt]     Gam_fo' Mi Gam_gp, exactly symmetric under (fo,t) <-> (gp,s)
+  int H;    // [nt][nt]           Href
+  int Nw;   // [nu][nqp]          w_q N_i(q)
+  int Mg;   // [3][nqp]           P1 geometry basis at the cell points
+  int total;
+};
+// Builds them from rt; false when the reference mass is not SPD.
+bool build_spd_tables(const RefTables& rt, SpdLayout& L, std::vector<double>& data);
+
 }  // namespace hdg

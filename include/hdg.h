@@ -67,8 +67,13 @@ enum { HDG_PC_AUTO = 0, HDG_PC_BLOCK_JACOBI = 1, HDG_PC_MULTIGRID = 2 };
    kernels from the host, and from the second solve on each chunk of CG
    iterations is captured once as a HIP graph and replayed (capture costs
    more than one solve saves); HDG_SCHEDULE_EAGER never captures.  Same
-   results, bit for bit. */
-enum { HDG_SCHEDULE_EAGER = 1, HDG_SCHEDULE_AUTOTUNE = 2 };
+   results, bit for bit.  Where kappa = 1 and the cell rule leaves the
+   reference mass SPD (k = 1 at quad_degree 2, k = 3 at 6, k <= 2 at 4),
+   condensation eliminates the q block analytically (A = detJ diag(Mref,
+   Mref)) and factors the Schur complement on u; HDG_SCHEDULE_LU eliminates
+   the full local matrix as the reference's factorize does instead (same
+   operators to rounding, about cond(Me) eps). */
+enum { HDG_SCHEDULE_EAGER = 1, HDG_SCHEDULE_AUTOTUNE = 2, HDG_SCHEDULE_LU = 4 };
 
 /* Sources.  HDG_SRC_REFERENCE is the reference's f = 2 pi^2 sin(pi x) sin(pi y)
    (examples/poisson2D_HDG.jl:55).  HDG_SRC_VALUES takes f already evaluated
@@ -336,7 +341,8 @@ const char* hdg_server_last_error(const hdg_server* srv);
 /* No counterpart in the reference (one problem per script run).  A batch
    holds n independent requests (same fields as the serving loop's; u = 0 on
    the boundary, f and kappa from src, HDG_SRC_VALUES not accepted).
-   Requests of the same order and options form a group: their rectangle
+   Requests of the same order and options (and with or without kappa) form
+   a group: their rectangle
    meshes are packed into one mesh (a disjoint union), so element build,
    condensation, assembly, recovery and every elementwise kernel of the
    solve run once per group, and the PCG scalars, reductions and multigrid

@@ -74,6 +74,29 @@ def test_constant_and_linear_patch(O, k):
             np.testing.assert_allclose(Ate[c] @ ue, expect, atol=1e-12 * np.abs(Ate[c]).max())
 
 
+@pytest.mark.parametrize("k,tau", [(1, 0.7), (2, 2.5), (3, 2.5), (3, 1.0)])
+def test_constant_trace_with_tau(O, k, tau):
+    """A constant trace c is reproduced by u = c, q = 0 for any tau, and
+    since H carries no tau (poisson2D_HDG.jl:149) the condensed operator maps
+    it to Ate c = (tau - 1) H c = (tau - 1) |e_f| c on mode 0 of each face."""
+    m = O.rectangle_mesh(4, 3, jitter=0.2, seed=7)
+    nt = k + 1
+    P = O.problem(k=k, qdeg=QD[k], tau=tau)
+    Ate, _, Ke, _ = O.condense(P, m)
+    c = np.zeros(3 * nt)
+    c[0::nt] = 1.0
+    for e in range(m.ncells):
+        _, _, _, detJf = O.geometry(m, e + 1)
+        expect = np.zeros(3 * nt)
+        expect[0::nt] = (tau - 1.0) * np.asarray(detJf)
+        np.testing.assert_allclose(Ate[e] @ c, expect, atol=1e-12 * np.abs(Ate[e]).max())
+        # K_e c: q = 0, u = 1 (coefficient 1 / sqrt 2 on phi_1 = sqrt 2)
+        nu = (k + 1) * (k + 2) // 2
+        want = np.zeros(3 * nu)
+        want[2 * nu] = 1.0 / np.sqrt(2.0)
+        np.testing.assert_allclose(Ke[e] @ c, want, atol=1e-11 * np.abs(Ke[e]).max())
+
+
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_trace_operator_symmetric_negative_definite(O, k):
     m = O.rectangle_mesh(5, 4, jitter=0.2, seed=3)
