@@ -24,6 +24,8 @@ SCHEDULE_AUTOTUNE = 2
 SCHEDULE_LU = 4
 SRC_REFERENCE, SRC_KAPPA, SRC_SINE_SERIES, SRC_VALUES = 0, 1, 2, 3
 UEX_SINSIN, UEX_LINEAR = 0, 1
+G_LINEAR, G_SINSIN = 0, 1
+DIRICHLET_REFERENCE = 1
 
 # every symbol include/hdg.h declares
 EXPORTS = (
@@ -36,7 +38,8 @@ EXPORTS = (
     "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error", "hdg_server_set_batching",
     "hdg_batch_create", "hdg_batch_run", "hdg_batch_trace", "hdg_batch_info", "hdg_batch_destroy",
     "hdg_batch_last_error", "hdg_autotune", "hdg_set_schedule", "hdg_set_rectangle_mesh",
-    "hdg_set_dirichlet_boundary",
+    "hdg_set_dirichlet_boundary", "hdg_set_dirichlet_function",
+    "hdg_get_applied_system",
 )
 
 
@@ -164,6 +167,8 @@ def _declare(L):
         "hdg_set_rectangle_mesh": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                              C.c_double, C.c_double, C.c_uint64]),
         "hdg_set_dirichlet_boundary": (C.c_int, [C.c_void_p, C.c_double]),
+        "hdg_set_dirichlet_function": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_int]),
+        "hdg_get_applied_system": (C.c_int, [_ctx, _i64, _i64, _d, _d, _d]),
         "hdg_set_schedule": (C.c_int, [C.c_void_p, _P(Schedule)]),
         "hdg_batch_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int, _P(Request)]),
         "hdg_batch_run": (C.c_int, [C.c_void_p, _P(Result)]),

@@ -91,6 +91,28 @@ class Context:
             s.f_qp = keep.ctypes.data
         self._chk(lib().hdg_set_source(self._h, C.byref(s)))
 
+    def set_dirichlet_function(self, kind=L.G_LINEAR, par=(0.0, 0.0, 0.0), reference=False):
+        """Dirichlet data on every boundary face computed on the device
+        (hdg_set_dirichlet_function): g = par[0] + par[1] x + par[2] y or
+        sin(pi x) sin(pi y); reference=True reproduces the reference's
+        weights and accumulator (boundary.jl:27-37)."""
+        p = np.ascontiguousarray(par, dtype=np.float64)
+        self._chk(lib().hdg_set_dirichlet_function(self._h, int(kind), ptr(p),
+                                                   L.DIRICHLET_REFERENCE if reference else 0))
+
+    def get_applied_system(self):
+        """apply!(K, b, dbc) on the device: (colptr, rowval, nzval, rhs,
+        meandiag) of the applied system, Julia's CSC, 1-based."""
+        s = self.sizes
+        colptr = np.zeros(s.n_dofs + 1, dtype=np.int64)
+        rowval = np.zeros(s.nnz, dtype=np.int64)
+        nzval = np.zeros(s.nnz)
+        rhs = np.zeros(s.n_dofs)
+        md = np.zeros(1)
+        self._chk(lib().hdg_get_applied_system(self._h, ptr(colptr, _i64), ptr(rowval, _i64), ptr(nzval),
+                                               ptr(rhs), ptr(md)))
+        return colptr, rowval, nzval, rhs, float(md[0])
+
     def set_dirichlet(self, dofs, vals):
         d = np.ascontiguousarray(dofs, dtype=np.int64)
         v = np.ascontiguousarray(vals, dtype=np.float64)

@@ -1048,6 +1048,37 @@ int hdg_set_dirichlet_boundary(hdg_ctx* c, double value) {
   return HDG_OK;
 }
 
+int hdg_set_dirichlet_function(hdg_ctx* c, int g_kind, const double* par, int flags) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
+  if (g_kind != HDG_G_LINEAR && g_kind != HDG_G_SINSIN) return fail(c, HDG_ERR_ARG, "unknown g kind %d", g_kind);
+  if (g_kind == HDG_G_LINEAR && !par) return fail(c, HDG_ERR_ARG, "HDG_G_LINEAR needs 3 parameters");
+  if (flags & ~HDG_DIRICHLET_REFERENCE) return fail(c, HDG_ERR_ARG, "unknown flags %d", flags);
+  TRY(select_device(c));
+  const RefTables& t = c->tab;
+  const int nfq = (int)t.fx.size();
+  if (nfq > kMaxFacePoints) return fail(c, HDG_ERR_UNSUPPORTED, "face rule of %d points", nfq);
+  if ((flags & HDG_DIRICHLET_REFERENCE) && (int)t.qw.size() < nfq)
+    return fail(c, HDG_ERR_ARG, "the cell rule has %d points, fewer than the face rule's %d",
+                (int)t.qw.size(), nfq);
+  DirichletRule R;
+  R.nfq = nfq;
+  for (int q = 0; q < nfq; ++q) {
+    R.x[q] = t.fx[q];
+    R.w[q] = t.fw[q];
+    R.wc[q] = q < (int)t.qw.size() ? t.qw[q] : 0.0;
+    double lv[5];
+    legendre01_eval(c->nt, t.fx[q], lv);
+    for (int i = 0; i < c->nt; ++i) R.leg[i * nfq + q] = lv[i];
+  }
+  const double p3[3] = {par ? par[0] : 0.0, par ? par[1] : 0.0, par ? par[2] : 0.0};
+  LAUNCHCHK(c, launch_dirichlet_project(c->mesh, c->nt, R, g_kind, p3, flags, c->d_dmask, c->d_uD, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->jacobi_dirty = true;
+  c->solved = c->recovered = false;
+  return HDG_OK;
+}
+
 int hdg_query_sizes(const hdg_ctx* c, hdg_sizes* s) {
   if (!c || !s) return HDG_ERR_ARG;
   std::memset(s, 0, sizeof(*s));
@@ -1170,10 +1201,13 @@ int hdg_run(hdg_ctx* c, hdg_solve_info* info) {
   return r;
 }
 
-int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs) {
-  if (!c) return HDG_ERR_ARG;
-  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
-  TRY(select_device(c));
+namespace {
+
+// Julia's CSC of K (the CSR of K^T: column (g, b) holds rows (f, a) for the
+// faces f of row g's pattern, which is structurally symmetric, ascending)
+// from a face-ELL value array and an rhs in device memory.
+int export_csc(hdg_ctx* c, const double* d_vals, const double* d_rhs, int64_t* colptr, int64_t* rowval,
+               double* nzval, double* rhs) {
   const int64_t nf = c->mesh.nf, N = c->T.N;
   const int nt = c->nt;
   std::vector<int> bcol(kSlots * nf);
@@ -1184,12 +1218,10 @@ int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, 
   std::vector<double> vals;
   if (nzval) {
     vals.resize((size_t)kSlots * nt * N);
-    HIPCHK(c, hipMemcpyAsync(vals.data(), c->T.vals, sizeof(double) * vals.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(vals.data(), d_vals, sizeof(double) * vals.size(), hipMemcpyDeviceToHost, c->stream));
   }
-  if (rhs) HIPCHK(c, hipMemcpyAsync(rhs, c->T.rhs, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+  if (rhs) HIPCHK(c, hipMemcpyAsync(rhs, d_rhs, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  // CSC of K: column (g, b) holds rows (f, a) for the faces f of row g's
-  // pattern (the pattern is structurally symmetric), in ascending order.
   int64_t w = 0;
   if (colptr) colptr[0] = 1;
   for (int64_t g = 0; g < nf; ++g)
@@ -1206,6 +1238,41 @@ int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, 
       if (colptr) colptr[g * nt + b + 1] = w + 1;
     }
   return HDG_OK;
+}
+
+}  // namespace
+
+int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  return export_csc(c, c->T.vals, c->T.rhs, colptr, rowval, nzval, rhs);
+}
+
+int hdg_get_applied_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs,
+                           double* meandiag_out) {
+  if (!c) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  const int64_t N = c->T.N;
+  double *vals = nullptr, *b = nullptr, *part = nullptr;
+  int r = dalloc(c, &vals, (size_t)kSlots * c->nt * N);
+  if (r == HDG_OK) r = dalloc(c, &b, N);
+  if (r == HDG_OK) r = dalloc(c, &part, 1024 + 1);
+  if (r == HDG_OK) {
+    const int e = launch_apply(c->pat, c->T, part, part + 1024, vals, b, c->stream);
+    if (e) r = fail(c, HDG_ERR_HIP, "apply! kernels: %s", hipGetErrorString((hipError_t)e));
+  }
+  if (r == HDG_OK && meandiag_out) {
+    const hipError_t e = hipMemcpyAsync(meandiag_out, part + 1024, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) r = fail(c, HDG_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+  }
+  if (r == HDG_OK) r = export_csc(c, vals, b, colptr, rowval, nzval, rhs);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(c, part);
+  dfree(c, b);
+  dfree(c, vals);
+  return r;
 }
 
 int hdg_get_local_ops(hdg_ctx* c, double* Ke, double* be) {

@@ -89,6 +89,17 @@ struct CondenseArgs {
          *bev = nullptr;
 };
 
+// The 1-D rule of the trace space for the device Dirichlet projection:
+// points x and weights w on [0, 1], the cell rule's first weights wc (the
+// weights the reference applies to face points, src/boundary.jl:33) and the
+// Legendre basis leg[i * nfq + q] at the points.
+constexpr int kMaxFacePoints = 16;
+struct DirichletRule {
+  int nfq = 0;
+  double x[kMaxFacePoints], w[kMaxFacePoints], wc[kMaxFacePoints];
+  double leg[5 * kMaxFacePoints];
+};
+
 // PCG coefficients kept for the Lanczos estimate of lambda_max(Dinv S)
 constexpr int kLanczos = 16;
 
@@ -322,6 +333,15 @@ int launch_errornorm(const DevMesh& m, const double* tables, const TabLayoutDev&
                      double* result, hipStream_t s);
 int cg_blocks(int64_t N, int nt);
 int errornorm_blocks(int64_t nc);
+// Dirichlet data on every face with one cell (src/boundary.jl:11-42):
+// g_kind 0 = par[0] + par[1] x + par[2] y, 1 = sin(pi x) sin(pi y); flags
+// bit 1 = the reference's weights and accumulator.  Other faces are freed.
+int launch_dirichlet_project(const DevMesh& m, int nt, const DirichletRule& R, int g_kind, const double* par,
+                             int flags, unsigned char* dmask, double* uD, hipStream_t s);
+// apply! (src/boundary.jl:121-175) into vals_out / rhs_out (shaped like
+// T.vals / T.rhs); part holds 1024 partial sums, md receives meandiag(K).
+int launch_apply(const DevPattern& P, const TraceSystem& T, double* part, double* md, double* vals_out,
+                 double* rhs_out, hipStream_t s);
 
 // Selector-generic forms of the CG and norm launchers (G = Whole or Grouped,
 // grid = blocks of the launch; the Whole forms above call these).

@@ -127,16 +127,23 @@ class Dirichlet:
     (src/boundary.jl:7-42): prescribed dofs (1-based, ascending by face, mode)
     and the projection of g onto the orthonormal trace basis.
 
-    The reference accumulates the projection across modes without resetting
-    it (boundary.jl:27-37), which only matters for g != 0; this class
-    projects each mode separately, which equals the reference whenever g = 0
-    (every configuration the reference ships)."""
+    The reference weights face point q with the cell rule's weight q
+    (fs.fs.qr_weights, boundary.jl:33) and accumulates across modes without
+    resetting (:27-37); both only matter for g != 0.  By default this class
+    computes the L2 projection, which equals the reference whenever g = 0
+    (every configuration the reference ships); reference=True reproduces the
+    reference's values for any g."""
 
-    def __init__(self, u_hat, mesh, faceset, g):
+    def __init__(self, u_hat, mesh, faceset, g, reference=False):
         fs = u_hat.fs
         nt = fs.nbase
         qdeg = fs.fs.quad_degree
         x, w = quadrature(1, qdeg)
+        if reference:
+            _, wc = quadrature(2, qdeg)
+            if len(wc) < len(w):
+                raise ValueError("the cell rule has fewer points than the face rule")
+            w = wc[: len(w)]
         Nt = legendre01(nt, x)  # (nt, nq)
         faces = sorted(mesh.facesets[faceset])
         dofs, vals = [], []
@@ -148,8 +155,11 @@ class Dirichlet:
             # trace parameter runs from the lower to the higher node id
             pts = pa[None, :] + x[:, None] * (pb - pa)[None, :]
             gv = np.array([float(g(p)) for p in pts]) if callable(g) else np.full(len(x), float(g))
+            acc = 0.0
             for i in range(nt):
                 dofs.append(f * nt - nt + i + 1)
-                vals.append(float(np.dot(w * gv, Nt[i])))
+                proj = float(np.dot(w * gv, Nt[i]))
+                acc = acc + proj if reference else proj
+                vals.append(acc)
         self.prescribed_dofs = np.array(dofs, dtype=np.int64)
         self.values = np.array(vals, dtype=np.float64)

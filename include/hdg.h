@@ -171,6 +171,21 @@ int hdg_set_dirichlet(hdg_ctx* ctx, int64_t n, const int64_t* dofs,
    cell), free elsewhere; the device-side form of hdg_set_dirichlet for a
    constant on the whole boundary. */
 int hdg_set_dirichlet_boundary(hdg_ctx* ctx, double value);
+/* Dirichlet(u_hat, mesh, "boundary", g) (src/boundary.jl:11-42) computed on
+   the device for g given in closed form: every trace dof of every face with
+   one cell is prescribed (ascending by face, mode), the others are freed.
+   g_kind HDG_G_LINEAR: g = par[0] + par[1] x + par[2] y (a constant with
+   par[1] = par[2] = 0); HDG_G_SINSIN: g = sin(pi x) sin(pi y) (par unused).
+   flags 0: the L2 projection onto the orthonormal trace basis;
+   HDG_DIRICHLET_REFERENCE: the reference's computation, which weights face
+   point q with the cell rule's weight q (fs.fs.qr_weights, boundary.jl:33)
+   and does not reset its accumulator between modes (:27-37); both give 0
+   for g = 0 and differ otherwise (HDG_ERR_ARG when the cell rule has fewer
+   points than the face rule, where the reference's loop leaves its
+   weights). */
+enum { HDG_G_LINEAR = 0, HDG_G_SINSIN = 1 };
+enum { HDG_DIRICHLET_REFERENCE = 1 };
+int hdg_set_dirichlet_function(hdg_ctx* ctx, int g_kind, const double* par, int flags);
 int hdg_query_sizes(const hdg_ctx* ctx, hdg_sizes* sizes);
 
 /* ---- the pipeline -------------------------------------------------------- */
@@ -182,6 +197,16 @@ int hdg_assemble_condense(hdg_ctx* ctx);
    NULL. */
 int hdg_get_system(hdg_ctx* ctx, int64_t* colptr, int64_t* rowval,
                    double* nzval, double* rhs);
+/* apply!(K, b, dbc) with APPLY_TRANSPOSE (src/boundary.jl:121-175) on the
+   device, exported as hdg_get_system exports K and b: the rhs lifted by the
+   prescribed values (b_i -= v_d K(i, d) over the prescribed dofs d in
+   ascending order, skipping v_d = 0), prescribed rows and columns zeroed
+   (stored entries stay stored), meandiag(K) = sum_i |K(i,i)| / n
+   (:169-175, summed in a fixed blocked order) on their diagonal and
+   b_d = v_d meandiag.  The device's own trace system is left unchanged;
+   any pointer may be NULL; meandiag_out receives meandiag(K) if not NULL. */
+int hdg_get_applied_system(hdg_ctx* ctx, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs,
+                           double* meandiag_out);
 /* K_element / b_element of doassemble: Ke[c] m x ntr column-major, stored
    cell after cell; be[c] of length m.  Either pointer may be NULL. */
 int hdg_get_local_ops(hdg_ctx* ctx, double* Ke, double* be);
@@ -214,6 +239,7 @@ int hdg_errornorm(hdg_ctx* ctx, int uex_kind, const double* uex_par,
 /* Fused assemble + condense + solve + recover with everything kept on the
    device; the timed unit of the benchmark. */
 int hdg_run(hdg_ctx* ctx, hdg_solve_info* info);
+
 
 /* ---- reference-element numerics (host) ----------------------------------- */
 /* value / gradient of the orthonormal Dubiner basis j = 1..nu at (r, s)

@@ -204,6 +204,12 @@ def test_dirichlet_projection_matches_oracle(hdg, O, k):
     np.testing.assert_allclose(dbc.values, v, atol=1e-14)
     z = hdg.Dirichlet(hdg.TrialFunction(Mh), m, "boundary", lambda x: 0.0)
     assert np.all(z.values == 0.0)
+    # the reference's own computation (cell-rule weights, running accumulator)
+    dr = hdg.Dirichlet(hdg.TrialFunction(Mh), m, "boundary", lambda x: g[0] + g[1] * x[0] + g[2] * x[1],
+                       reference=True)
+    _, vr = O.dirichlet_trace(O.problem(k=k, qdeg=qd), om, g=g, quirk=True)
+    np.testing.assert_allclose(dr.values, vr, atol=1e-14)
+    assert np.abs(dr.values - dbc.values).max() > 1e-3
 
 
 def test_apply_bc_matches_oracle(hdg, O):
