@@ -38,7 +38,7 @@ EXPORTS = (
     "hdg_server_stats", "hdg_server_destroy", "hdg_server_last_error", "hdg_server_set_batching",
     "hdg_batch_create", "hdg_batch_run", "hdg_batch_trace", "hdg_batch_info", "hdg_batch_destroy",
     "hdg_batch_last_error", "hdg_autotune", "hdg_set_schedule", "hdg_set_rectangle_mesh",
-    "hdg_set_dirichlet_boundary", "hdg_set_dirichlet_function",
+    "hdg_set_dirichlet_boundary", "hdg_errornorm_noalloc", "hdg_set_dirichlet_function",
     "hdg_get_applied_system",
 )
 
@@ -149,6 +149,8 @@ def _declare(L):
         "hdg_set_trace": (C.c_int, [_ctx, _d]),
         "hdg_recover": (C.c_int, [_ctx, _d, _d, _d]),
         "hdg_errornorm": (C.c_int, [_ctx, C.c_int, _d, _d]),
+        "hdg_errornorm_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_int, _d, C.c_void_p, C.c_void_p]),
         "hdg_run": (C.c_int, [_ctx, _P(SolveInfo)]),
         "hdg_dubiner": (C.c_int, [C.c_int, C.c_double, C.c_double, _d, _d]),
         "hdg_quadrature": (C.c_int, [C.c_int, C.c_int, C.c_int, _d, _d]),

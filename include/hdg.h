@@ -240,6 +240,26 @@ int hdg_errornorm(hdg_ctx* ctx, int uex_kind, const double* uex_par,
    device; the timed unit of the benchmark. */
 int hdg_run(hdg_ctx* ctx, hdg_solve_info* info);
 
+/* hdg_errornorm for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops); no counterpart in the reference.
+   Allocates nothing: the kernels' block partial sums (at most 2048 doubles)
+   and the reference tables of k = 1..4 at quad degrees 1..6 live in static
+   device arrays of the library, one copy per GPU, set up at the first call
+   on the current device (that call copies the tables and synchronises; later
+   calls only enqueue two kernels on `stream`).  Inputs are device arrays:
+   xy (2 doubles per node), cells (3 node ids per cell, int32, 0-based) and
+   qu (the recovered [q; u] of each cell, 3 nu doubles per cell, cell after
+   cell); the squared L2 error of u against HDG_UEX_SINSIN or HDG_UEX_LINEAR
+   (uex_par c0, c1, c2, read on the host) goes to sq_l2, one double in device
+   memory, asynchronously on stream (a hipStream_t; NULL = the default
+   stream).  HDG_ERR_ARG for n_cells outside 1..HDG_NOALLOC_MAX_CELLS or a
+   null pointer, HDG_ERR_UNSUPPORTED for k outside 1..4, HDG_ERR_QUAD for a
+   quad degree above 6 (0 = k+1).  Every call on one GPU shares that GPU's
+   partial sums, which each call writes in full before reading them: calls
+   that overlap on one GPU race. */
+#define HDG_NOALLOC_MAX_CELLS 2147483647LL
+int hdg_errornorm_noalloc(int k, int quad_degree, int64_t n_cells, const double* xy, const int32_t* cells,
+                          const double* qu, int uex_kind, const double* uex_par, double* sq_l2, void* stream);
 
 /* ---- reference-element numerics (host) ----------------------------------- */
 /* value / gradient of the orthonormal Dubiner basis j = 1..nu at (r, s)
