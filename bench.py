@@ -12,7 +12,10 @@ uneven size and order as one batch (hdg_batch: instances of one order
 packed into one mesh, one launch per kernel for all of them), `--workload
 batched --via-server` runs the same instances through the serving loop's
 slots instead, and `--workload serving` feeds the serving loop a Poisson
-stream of requests.
+stream of requests (then the same stream at 80% of the capacity it
+measured), and `--workload refine-probe` measures FP64 iterative refinement
+with the trace matrix rounded to float (hdg_refine_probe) at 256^2, 512^2 and
+1024^2 cells.
 One step = assemble + condense + trace solve (multigrid-preconditioned CG
 to a relative residual of 1e-12) + recovery, with mesh, tables and
 Dirichlet data already resident in HBM.
@@ -61,18 +64,27 @@ def spmv_bytes(ctx):
     return 8 * s.nnz + 4 * nblocks + 16 * N + 16 * N + N
 
 
-def condense_work(ctx):
-    """Algorithmic bytes and FP64 flops of one condensation launch.  Bytes:
-    per cell 3 node ids + 3 face ids + 3 node coordinates read (72 B) and
-    Ate (ntr^2), bte (ntr), K_e (m ntr), b_e (m) written.  Flops: the
-    Gauss-Jordan elimination of the bordered system, 2 per fma: at pivot
-    step j every other row updates its columns j+1 .. m+ntr (rows = m + ntr),
-    i.e. 2 (rows - 1) sum_j (m + ntr - j) over j < m; building the rows and
-    the quadrature sums are not counted."""
+def condense_work(ctx, form):
+    """Algorithmic bytes and FP64 flops (2 per fma) of one condensation
+    launch.  Bytes: per cell 3 node ids + 3 face ids + 3 node coordinates
+    read (72 B) and Ate (ntr^2), bte (ntr), K_e (m ntr), b_e (m) written.
+    Flops of the elimination form (form 0): the Gauss-Jordan elimination of
+    the bordered system, at pivot step j every other row updates its columns
+    j+1 .. m+ntr (rows = m + ntr), i.e. (rows - 1) sum_j (m + ntr - j) fma
+    over j < m; building the rows and the quadrature sums are not counted.
+    Flops of the Schur form (form 1), per cell with nu = m/3: S (6 nu^2 fma),
+    its Cholesky factor (nu^3/6), R (3 nu ntr), L^-1 [R be] and L^-T of the
+    same ((ntr+1) nu^2/2 each), Y'Y (ntr^2 nu), bte (ntr nu), Q = A^-1(BU-E)
+    (2 nu^2 (ntr+1)) and the load vector (nqp nu)."""
     s = ctx.sizes
     m, ntr, nc = s.m, s.ntr, s.n_cells
     rows = m + ntr
-    fma = (rows - 1) * sum(m + ntr - j for j in range(m))
+    if form == 1:
+        nu = m // 3
+        fma = (6 * nu * nu + nu ** 3 / 6 + 3 * nu * ntr + (ntr + 1) * nu * nu + ntr * ntr * nu + ntr * nu
+               + 2 * nu * nu * (ntr + 1) + s.nqp * nu)
+    else:
+        fma = (rows - 1) * sum(m + ntr - j for j in range(m))
     byts = nc * (72 + 8 * (ntr * ntr + ntr + m * ntr + m))
     return byts, 2 * fma * nc
 
@@ -251,29 +263,33 @@ def run_stream(args):
     with H.Server(slots=1) as w:
         for k in (1, 2, 3, 4):
             w.wait(w.submit(H.make_request(nx=8, ny=8, k=k)))
-    srv = H.Server(slots=args.slots, hbm_budget=budget, batch=args.batch, batch_cells=args.batch_cells)
-    ids = []
-    t0 = time.perf_counter()
-    if args.workload == "batched":
-        ids = [srv.submit(r) for r in reqs]
-    else:
-        gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
-        arrivals = np.cumsum(gaps)
-        n_req = int(np.searchsorted(arrivals, args.seconds))
+    def serve(rate):
+        srv = H.Server(slots=args.slots, hbm_budget=budget, batch=args.batch, batch_cells=args.batch_cells)
+        ids = []
+        t0 = time.perf_counter()
+        if args.workload == "batched":
+            ids = [srv.submit(r) for r in reqs]
+        else:
+            gaps = rng.exponential(1.0 / rate, size=len(reqs))
+            arrivals = np.cumsum(gaps)
+            n_req = int(np.searchsorted(arrivals, args.seconds))
 
-        def feed():
-            for t, r in zip(arrivals[:n_req], reqs[:n_req]):
-                dt = t - (time.perf_counter() - t0)
-                if dt > 0:
-                    time.sleep(dt)
-                ids.append(srv.submit(r))
-        th = threading.Thread(target=feed)
-        th.start()
-        th.join()
-    res = [srv.wait(i) for i in ids]
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
+            def feed():
+                for t, r in zip(arrivals[:n_req], reqs[:n_req]):
+                    dt = t - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                    ids.append(srv.submit(r))
+            th = threading.Thread(target=feed)
+            th.start()
+            th.join()
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        return res, wall, st
+
+    res, wall, st = serve(args.rate)
     ok = [r for r in res if r["status"] == 0]
     cells = sum(r["n_cells"] for r in ok)
     lat = np.array([r["t_done"] - r["t_submit"] for r in ok])
@@ -305,6 +321,103 @@ def run_stream(args):
     if args.workload == "batched":
         errs = [np.sqrt(r["errornorm"]) for r in ok]
         out["l2_error_max"] = float(max(errs)) if errs else None
+    elif args.sustainable > 0 and ok:
+        # the same stream at a rate the loop sustains: a fraction of the
+        # capacity just measured (cells per busy second) over the mean cells
+        # of a request
+        mean_cells = float(np.mean([r["n_cells"] for r in res]))
+        rate2 = args.sustainable * st["cells_per_second"] / mean_cells
+        res2, wall2, st2 = serve(rate2)
+        ok2 = [r for r in res2 if r["status"] == 0]
+        lat2 = np.array([r["t_done"] - r["t_submit"] for r in ok2])
+        out["sustainable"] = {
+            "fraction_of_capacity": args.sustainable, "rate_per_s": rate2,
+            "requests": len(res2), "failed": len(res2) - len(ok2),
+            "offered_cells_per_s": sum(r["n_cells"] for r in ok2) / args.seconds,
+            "cells_per_second": st2["cells_per_second"],
+            "p50_ms": 1e3 * float(np.percentile(lat2, 50)) if len(lat2) else None,
+            "p99_ms": 1e3 * float(np.percentile(lat2, 99)) if len(lat2) else None,
+            "mean_queue_wait_ms": 1e3 * st2["mean_queue_wait"], "wall_s": wall2,
+        }
+    print(json.dumps(out), flush=True)
+
+
+def run_refine_probe(args):
+    """What an FP32 operator costs: FP64 iterative refinement whose
+    corrections solve with the trace matrix rounded to float
+    (hdg_refine_probe), on the p3 workload at 256^2, 512^2 and 1024^2
+    cells.  Per size: the relative residuals of 4 refinement steps with
+    corrections solved to 1e-12 (the rounded operator's own contraction),
+    of up to 8 steps with corrections solved to 1e-3 (the inner accuracy an
+    FP32 solve reaches), the L2 error of the rounded system's solution
+    against the FP64 one, and eps32 cond(K~), K~ the point-Jacobi scaled
+    free block of K: cond measured by Lanczos (scipy eigsh, shift-invert for
+    the smallest eigenvalue) on the exported matrix at 16^2..64^2 cells and
+    extrapolated as c n^2 (c from 64^2)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    from __graft_entry__ import _import_package
+    H = _import_package()
+    k, qdeg = 3, 6
+
+    def make(n):
+        mesh = H.rectangle_mesh((n, n), jitter=0.2, seed=SEED)
+        ctx = H.Context(0)
+        ctx.set_mesh(mesh)
+        ctx.setup(k, quad_degree=qdeg, tau=1.0, rtol=1e-12)
+        ctx.set_source()
+        d = mesh.boundary_dofs(k + 1)
+        ctx.set_dirichlet(d, np.zeros(len(d)))
+        ctx.assemble_condense()
+        return ctx, d
+
+    conds = {}
+    for n in (16, 32, 64):
+        ctx, d = make(n)
+        colptr, rowval, nzval, _ = ctx.get_system()
+        ctx.close()
+        nd = len(colptr) - 1
+        A = sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(nd, nd)).tocsr()
+        free = np.ones(nd, bool)
+        free[d - 1] = False
+        A = -A[free][:, free]
+        Ds = sp.diags(1.0 / np.sqrt(A.diagonal()))
+        At = (Ds @ A @ Ds).tocsc()
+        lmax = sla.eigsh(At, k=1, which="LA", return_eigenvectors=False)[0]
+        lmin = sla.eigsh(At, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+        conds[n] = float(lmax / lmin)
+    c64 = conds[64] / 64 ** 2
+    eps32 = 2.0 ** -24
+    sizes = []
+    for n in (256, 512, 1024):
+        ctx, _ = make(n)
+        _, info = ctx.solve(return_uhat=False)
+        l2_64 = float(np.sqrt(ctx.errornorm()))
+        rel1, _ = ctx.refine_probe(1, inner_rtol=1e-12)
+        l2_32 = float(np.sqrt(ctx.errornorm()))
+        rel_x, it_x = ctx.refine_probe(4, inner_rtol=1e-12)
+        l2_x = float(np.sqrt(ctx.errornorm()))
+        rel_l, it_l = ctx.refine_probe(8, inner_rtol=1e-3)
+        ctx.close()
+        sizes.append({
+            "cells": 2 * n * n, "n": n,
+            "eps32_cond_extrapolated": eps32 * c64 * n * n,
+            "fp64_solve": {"pcg_iterations": int(info.iters), "true_relres": float(info.true_relres),
+                           "l2_error": l2_64},
+            "rounded_operator_solution_l2_error": l2_32,
+            "refine_exact_corrections": {"relres": [float(x) for x in rel_x], "inner_iterations": [int(i) for i in it_x],
+                                         "l2_error": l2_x},
+            "refine_inner_1e-3": {"relres": [float(x) for x in rel_l], "inner_iterations": [int(i) for i in it_l]},
+        })
+        print(f"refine probe n={n}: {sizes[-1]}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "FP64 iterative refinement with the trace matrix rounded to float: relative residual per step",
+        "unit": "relative residual", "n_gpus": 1, "dtype": "f64 residuals, f32-rounded operator",
+        "data": "synthetic",
+        "config": {"workload": "HDG Poisson P3, quad_degree 6, jitter 0.2h seed %d, tau=1, kappa=1" % SEED,
+                   "cond_scaled_measured": {str(n): c for n, c in conds.items()}, "cond_fit_c": c64},
+        "sizes": sizes,
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -315,11 +428,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2,
                     help="untimed steps (the first solve after a set-up runs eagerly, the second captures "
                          "the CG graphs that later steps replay)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + list(STREAM_WORKLOADS), default="p3-1024")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + list(STREAM_WORKLOADS) + ["refine-probe"],
+                    default="p3-1024")
     ap.add_argument("--slots", type=int, default=16, help="serving / --via-server: concurrent solves")
     ap.add_argument("--hbm-gib", type=float, default=64.0, help="batched/serving: device-memory budget")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: arrivals per second")
     ap.add_argument("--seconds", type=float, default=10.0, help="serving: length of the arrival stream")
+    ap.add_argument("--sustainable", type=float, default=0.8,
+                    help="serving: then replay the stream at this fraction of the measured capacity "
+                         "(0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--precond", choices=("auto", "bj", "mg"), default="auto",
@@ -343,6 +460,9 @@ def main():
               "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "refine-probe":
+        run_refine_probe(args)
+        return
     if args.workload == "batched" and not args.via_server:
         run_batched(args)
         return
@@ -379,7 +499,10 @@ def main():
     # dominant kernel: one CG trace-SpMV launch, HIP events on ctx's stream
     spmv_us = ctx.time_kernel(0, reps=200)
     cond_us = ctx.time_kernel(2, reps=5)
-    cond_bytes, cond_flops = condense_work(ctx)
+    form = int(sched.get("condense_form", 0))
+    cond_bytes, cond_flops = condense_work(ctx, form)
+    cond_kernel = (f"condense_spd_kernel<{k}, {sched.get('condense_waves') or 2}>" if form == 1
+                   else f"condense_kernel<{k}>")
     ab = spmv_bytes(ctx)
     achieved = ab / (spmv_us * 1e-6) / 1e9
     copy_us = ctx.time_kernel(3, reps=3) if args.calibrate else None
@@ -412,9 +535,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"cg_spmv_kernel<{nt}>", "avg_us": spmv_us, "bytes_per_launch": int(ab)},
-        # the condensation kernel against both ceilings: it is bound by neither (the pivot
-        # chain's cross-lane latency bounds it); FP64 dense MFMA peaks at the FP64 vector rate
-        "condense_roofline": {"kernel": f"condense_kernel<{k}>", "avg_us": cond_us,
+        # the condensation kernel against both ceilings (FP64 dense MFMA peaks at the FP64
+        # vector rate); it reaches neither
+        "condense_roofline": {"kernel": cond_kernel,
+                              "form": "Schur complement on u (Cholesky)" if form == 1 else "bordered elimination",
+                              "avg_us": cond_us,
                               "bytes_per_launch": int(cond_bytes), "flops_per_launch": int(cond_flops),
                               "achieved_GBps": cond_bytes / (cond_us * 1e-6) / 1e9,
                               "hbm_frac": cond_bytes / (cond_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
@@ -430,7 +555,7 @@ def main():
         ctx.assemble_condense()
         u64, _ = ctx.solve()
         configure(H.HDG_MIXED)
-        ctx.set_schedule(sched["condense_seg"], sched["condense_grid"])
+        ctx.set_schedule(**sched)
         m_elapsed, m_lat, m_infos = timed_steps(ctx, args.steps, args.warmup, torch.cuda.synchronize)
         m_l2 = float(np.sqrt(ctx.errornorm()))
         ctx.assemble_condense()

@@ -39,7 +39,7 @@ EXPORTS = (
     "hdg_batch_create", "hdg_batch_run", "hdg_batch_trace", "hdg_batch_info", "hdg_batch_destroy",
     "hdg_batch_last_error", "hdg_autotune", "hdg_set_schedule", "hdg_set_rectangle_mesh",
     "hdg_set_dirichlet_boundary", "hdg_errornorm_noalloc", "hdg_set_dirichlet_function",
-    "hdg_get_applied_system",
+    "hdg_get_applied_system", "hdg_refine_probe",
 )
 
 
@@ -97,7 +97,7 @@ class Result(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("condense_seg", C.c_int32), ("condense_grid", C.c_int32), ("condense_us", C.c_double),
-                ("default_us", C.c_double)]
+                ("default_us", C.c_double), ("condense_form", C.c_int32), ("condense_waves", C.c_int32)]
 
 
 class ServerSummary(C.Structure):
@@ -155,6 +155,7 @@ def _declare(L):
         "hdg_dubiner": (C.c_int, [C.c_int, C.c_double, C.c_double, _d, _d]),
         "hdg_quadrature": (C.c_int, [C.c_int, C.c_int, C.c_int, _d, _d]),
         "hdg_time_kernel": (C.c_int, [_ctx, C.c_int, C.c_int, _d]),
+        "hdg_refine_probe": (C.c_int, [_ctx, C.c_int, C.c_double, _d, C.c_void_p]),
         "hdg_get_trace": (C.c_int, [_ctx, _d]),
         "hdg_server_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_size_t, C.c_int]),
         "hdg_server_submit": (C.c_int, [C.c_void_p, _P(Request), _i64]),

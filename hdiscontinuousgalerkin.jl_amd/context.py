@@ -212,9 +212,23 @@ class Context:
         self._chk(lib().hdg_autotune(self._h, C.byref(sc)))
         return {f: getattr(sc, f) for f, _ in L.Schedule._fields_}
 
-    def set_schedule(self, condense_seg=0, condense_grid=0):
-        sc = L.Schedule(condense_seg=int(condense_seg), condense_grid=int(condense_grid))
+    def set_schedule(self, condense_seg=0, condense_grid=0, condense_waves=0, **_ignored):
+        """Sets the condensation schedule (hdg_set_schedule); accepts the
+        dict autotune returns."""
+        sc = L.Schedule(condense_seg=int(condense_seg), condense_grid=int(condense_grid),
+                        condense_waves=int(condense_waves))
         self._chk(lib().hdg_set_schedule(self._h, C.byref(sc)))
+
+    def refine_probe(self, outer, inner_rtol=0.0):
+        """FP64 iterative refinement whose corrections solve with the trace
+        matrix rounded to float (hdg_refine_probe).  Returns (relres, inner
+        iterations): ||r_j|| / ||r_0|| for j = 0..outer and the CG
+        iterations of each correction.  Leaves the refined iterate as the
+        trace."""
+        rel = np.zeros(int(outer) + 1)
+        its = np.zeros(max(int(outer), 1), dtype=np.int32)
+        self._chk(lib().hdg_refine_probe(self._h, int(outer), float(inner_rtol), ptr(rel), its.ctypes.data))
+        return rel, its[:int(outer)]
 
     def time_kernel(self, which, reps=50):
         out = np.zeros(1)

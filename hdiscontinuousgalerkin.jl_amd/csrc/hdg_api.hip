@@ -1375,8 +1375,11 @@ int hdg_autotune(hdg_ctx* c, hdg_schedule* out) {
   TRY(select_device(c));
   TRY(autotune_impl(c));
   if (out) {
-    out->condense_seg = c->csched.seg;
-    out->condense_grid = c->csched.grid;
+    const bool schur = condense_args(c).spd_tables != nullptr;
+    out->condense_form = schur ? HDG_CONDENSE_SCHUR : HDG_CONDENSE_ELIMINATION;
+    out->condense_seg = schur ? 16 : c->csched.seg;
+    out->condense_grid = schur ? c->csched.spd_grid : c->csched.grid;
+    out->condense_waves = schur ? c->csched.spd_wpe : 0;
     out->condense_us = c->tuned_us;
     out->default_us = c->default_us;
   }
@@ -1387,11 +1390,17 @@ int hdg_set_schedule(hdg_ctx* c, const hdg_schedule* sc) {
   if (!c || !sc) return HDG_ERR_ARG;
   if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
   const int seg = sc->condense_seg, nat = condense_native_seg(c->k);
-  if (seg != 0 && ((seg != 16 && seg != 32 && seg != 64) || seg < nat))
+  // the Schur form's 16 lanes (what hdg_autotune reports for it) are no
+  // width of the elimination form, which then keeps its narrowest
+  const bool schur = condense_args(c).spd_tables != nullptr;
+  if (seg != 0 && ((seg != 16 && seg != 32 && seg != 64) || (seg < nat && !schur)))
     return fail(c, HDG_ERR_ARG, "condense_seg=%d: 0 or 16/32/64 lanes, at least %d at k=%d", seg, nat, c->k);
+  const int eseg = seg < nat ? 0 : seg;
   if (sc->condense_grid < 0 || sc->condense_grid > (1 << 20))
     return fail(c, HDG_ERR_ARG, "condense_grid=%d out of range", sc->condense_grid);
-  c->csched = CondenseSched{seg, sc->condense_grid};
+  const int w = sc->condense_waves;
+  if (w != 0 && w != 2 && w != 3) return fail(c, HDG_ERR_ARG, "condense_waves=%d: 0, 2 or 3", w);
+  c->csched = CondenseSched{eseg, sc->condense_grid, w, sc->condense_grid};
   c->tuned = true;
   return HDG_OK;
 }
@@ -1426,6 +1435,76 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   *avg_us = 1e3 * ms / reps;
   c->solved = c->recovered = false;  // the CG state was overwritten
   return HDG_OK;
+}
+
+int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, int32_t* inner_iters) {
+  if (!c || !relres || outer < 0 || outer > 64 || !(inner_rtol >= 0.0)) return HDG_ERR_ARG;
+  if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
+  TRY(select_device(c));
+  for (int j = 0; j <= outer; ++j) relres[j] = std::nan("");
+  if (inner_iters)
+    for (int j = 0; j < outer; ++j) inner_iters[j] = 0;
+  const int64_t N = c->T.N, nv = (int64_t)kSlots * c->nt * N;
+  double *vr = nullptr, *u = nullptr, *res = nullptr, *zero = nullptr, *part = nullptr;
+  int rc = HDG_OK;
+  if ((rc = dalloc(c, &vr, nv)) == HDG_OK && (rc = dalloc(c, &u, N)) == HDG_OK && (rc = dalloc(c, &res, N)) == HDG_OK &&
+      (rc = dalloc(c, &zero, N)) == HDG_OK)
+    rc = dalloc(c, &part, refine_partials() + 1);
+  // graphs hold the operator's pointers: none is replayed or kept across the swap
+  drop_graphs(c);
+  const TraceSystem saved = c->T;
+  const hdg_options opt = c->opt;
+  c->opt.schedule |= HDG_SCHEDULE_EAGER;
+  if (inner_rtol > 0.0) c->opt.rtol = inner_rtol;
+  auto step = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(u, c->T.uD, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(zero, 0, sizeof(double) * N, c->stream));
+    LAUNCHCHK(c, launch_round_float(c->T.vals, vr, nv, c->stream));
+    double r0 = 0.0;
+    for (int j = 0;; ++j) {
+      c->T = saved;
+      LAUNCHCHK(c, launch_refine_residual(c->pat, c->T, u, res, part, part + refine_partials(), c->stream));
+      double r2 = 0.0;
+      HIPCHK(c, hipMemcpyAsync(&r2, part + refine_partials(), sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (j == 0) r0 = std::sqrt(r2);
+      relres[j] = r0 > 0.0 ? std::sqrt(r2) / r0 : 0.0;
+      if (j == outer) return HDG_OK;
+      // K32 d = r_j: rhs r_j, no Dirichlet lift, the rounded values
+      c->T.vals = vr;
+      c->T.rhs = res;
+      c->T.uD = zero;
+      LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
+      c->jacobi_dirty = false;
+      hdg_solve_info info{};
+      const int e = solve_impl(c, &info);
+      if (inner_iters) inner_iters[j] = info.iters;
+      if (e != HDG_OK) return e;
+      LAUNCHCHK(c, launch_add(u, c->d_uhat, N, c->stream));
+    }
+  };
+  if (rc == HDG_OK) rc = step();
+  // the assembled system as it was, u_outer as the trace
+  c->T = saved;
+  c->opt = opt;
+  drop_graphs(c);
+  int rc2 = launch_jacobi(c->pat, c->T, c->stream);
+  if (rc == HDG_OK && u) {
+    if (rc2 == 0) rc2 = (int)hipMemcpyAsync(c->d_uhat, u, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+    if (rc2 == 0) rc2 = (int)hipStreamSynchronize(c->stream);
+    if (rc2 != 0) rc = fail(c, HDG_ERR_HIP, "restoring the trace system failed: %s", hipGetErrorString((hipError_t)rc2));
+  } else {
+    (void)hipStreamSynchronize(c->stream);
+  }
+  dfree(c, vr);
+  dfree(c, u);
+  dfree(c, res);
+  dfree(c, zero);
+  dfree(c, part);
+  c->jacobi_dirty = false;
+  c->solved = rc == HDG_OK;
+  c->recovered = false;
+  return rc;
 }
 
 }  // extern "C"

@@ -63,8 +63,9 @@ __device__ __forceinline__ double ate_entry(const double* T, const double* ys, c
   return dot - cgs * om - h;
 }
 
-// WPE: waves per SIMD the register budget is cut for (3: 168 VGPRs, with
-// some spills at k = 3; 2: 256 VGPRs); the autotuner picks one.  GROUPED:
+// WPE: waves per SIMD the register budget is cut for (2: 256 VGPRs, no
+// spills at k <= 2 and 60 B per lane at k = 3; 3: 168 VGPRs, 12-40 B of
+// spills per lane); the autotuner picks one, 2 by default.  GROUPED:
 // several instances packed into one mesh, the source of cell c is
 // srcs[cell_inst[c]].
 template <int K, int WPE, bool GROUPED>
@@ -305,6 +306,7 @@ int spd_launch(const CondenseArgs& a, int wpe, int grid_cap, hipStream_t s) {
   if (grid < 1) grid = 1;
   if (grid > cap) grid = cap;
   const size_t lds = sizeof(double) * (a.spd.total + 4 * kEpw * (SpdDims<K>::YS + SpdDims<K>::NL));
+  if (wpe != 3) wpe = 2;
   if (a.srcs) {
     if (wpe == 2)
       hipLaunchKernelGGL((condense_spd_kernel<K, 2, true>), dim3((int)grid), dim3(256), lds, s, a);
@@ -364,7 +366,7 @@ int tune_condense_spd(int k, const CondenseArgs& a, hipStream_t s, CondenseSched
         bt = us;
         bs = sc;
       }
-      if (wpe == 3 && g == 16384) dt = us;
+      if (wpe == 2 && g == 16384) dt = us;
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -289,8 +289,9 @@ int launch_pattern(const DevMesh& m, DevPattern& P, hipStream_t s);
 struct CondenseSched {
   int seg = 0;
   int grid = 0;
-  // the Schur-complement kernel: waves per SIMD it is compiled for (2 or 3;
-  // 0 = 3) and its grid cap (0 = 16384)
+  // the Schur-complement kernel: waves per SIMD its register budget is cut
+  // for (2 or 3; 0 = 2, the budget without spills) and its grid cap
+  // (0 = 16384)
   int spd_wpe = 0;
   int spd_grid = 0;
 };
@@ -439,6 +440,15 @@ int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Kryl
                             const MgGroup& G, const MgVecs<W>& mv, Grouped g, int grid, hipStream_t s);
 // float copy of n doubles
 int launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
+
+// hdg_refine_probe: the trace matrix rounded to float (vr = (double)(float)
+// vals), res = rhs - K u on the free rows with ||res||^2 in *out2 (part:
+// refine_partials() doubles), u += d.
+int refine_partials();
+int launch_round_float(const double* src, double* dst, int64_t n, hipStream_t s);
+int launch_refine_residual(const DevPattern& P, const TraceSystem& T, const double* u, double* res, double* part,
+                           double* out2, hipStream_t s);
+int launch_add(double* u, const double* d, int64_t n, hipStream_t s);
 
 // rectangle_mesh generated on the device into (possibly packed) arrays:
 // nodes, cells and faces land at node0 / cell0 / face0; ids written are

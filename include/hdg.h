@@ -272,26 +272,39 @@ int hdg_dubiner(int k, double r, double s, double* val, double* grad);
 int hdg_quadrature(int dim, int order, int cap, double* pts, double* w);
 
 /* ---- autotuning ----------------------------------------------------------- */
-/* No counterpart in the reference.  The condensation kernel runs one element
-   per segment of 16, 32 or 64 lanes (any width that holds the element's
-   rows) over a grid of blocks that stride over the cells.  hdg_autotune
-   times every (width, grid cap) pair on the current problem with HIP events
-   and keeps the fastest for this context until the next hdg_setup
-   (HDG_SCHEDULE_AUTOTUNE in hdg_options.schedule does the same at the first
-   assembly after hdg_setup, and for every group of a batch at
-   hdg_batch_create).  Results do not depend on the schedule, bit for bit.
-   Needs hdg_setup and hdg_set_source; overwrites the assembled operators
-   (assemble again before solving). */
+/* No counterpart in the reference.  The condensation takes one of two forms:
+   HDG_CONDENSE_ELIMINATION, the elimination of the bordered local system
+   with one element per segment of 16, 32 or 64 lanes (any width that holds
+   the element's rows), or, where it applies (see HDG_SCHEDULE_LU),
+   HDG_CONDENSE_SCHUR, one element per 16 lanes with a register budget cut
+   for 2 or 3 waves per SIMD.  Both stride a grid of blocks over the cells.
+   hdg_autotune times every (width or register budget, grid cap) pair of the
+   form the current problem takes, with HIP events, and keeps the fastest for
+   this context until the next hdg_setup (HDG_SCHEDULE_AUTOTUNE in
+   hdg_options.schedule does the same at the first assembly after hdg_setup,
+   and for every group of a batch at hdg_batch_create).  Results do not
+   depend on the schedule, bit for bit.  Needs hdg_setup and hdg_set_source;
+   overwrites the assembled operators (assemble again before solving). */
+enum { HDG_CONDENSE_ELIMINATION = 0, HDG_CONDENSE_SCHUR = 1 };
 typedef struct {
-  int32_t condense_seg;   /* lanes per element */
+  int32_t condense_seg;   /* lanes per element (16 for the Schur form) */
   int32_t condense_grid;  /* grid cap, blocks of 256 threads */
   double condense_us;     /* measured duration of the chosen schedule */
-  double default_us;      /* of the default (narrowest width, 8192 blocks) */
+  double default_us;      /* of the default schedule: the narrowest width and
+                             8192 blocks, or (Schur form) 2 waves per SIMD
+                             and 16384 blocks */
+  int32_t condense_form;  /* out: HDG_CONDENSE_* of the problem tuned */
+  int32_t condense_waves; /* Schur form: 2 or 3 waves per SIMD (0 = 2) */
 } hdg_schedule;
 int hdg_autotune(hdg_ctx* ctx, hdg_schedule* out);
-/* Sets the condensation schedule (condense_seg 0 = narrowest width,
-   condense_grid 0 = 8192; HDG_ERR_ARG for a width below the element's rows
-   or not 16/32/64) until the next hdg_setup; e.g. a schedule tuned before. */
+/* Sets the condensation schedule until the next hdg_setup, e.g. one tuned
+   before: condense_seg 0 = narrowest width (the Schur form ignores it),
+   condense_grid 0 = the form's default cap, condense_waves 0, 2 or 3 (the
+   elimination form ignores it); condense_form is not read.  HDG_ERR_ARG for
+   a width not 16/32/64, a width below the element's rows where the problem
+   takes the elimination form (where it takes the Schur form such a width,
+   e.g. the 16 hdg_autotune reports, leaves the elimination form its
+   narrowest), or another waves value. */
 int hdg_set_schedule(hdg_ctx* ctx, const hdg_schedule* sched);
 
 /* ---- measurement --------------------------------------------------------- */
@@ -303,6 +316,23 @@ int hdg_set_schedule(hdg_ctx* ctx, const hdg_schedule* sched);
    point for hardware traffic counters.  Timing runs overwrite the solver
    state; solve again before reading results. */
 int hdg_time_kernel(hdg_ctx* ctx, int which, int reps, double* avg_us);
+
+/* No counterpart in the reference: a measurement of what an FP32 operator
+   costs in accuracy.  FP64 iterative refinement of the assembled system
+   K uhat = rhs (Dirichlet rows fixed) whose corrections solve with K rounded
+   entry by entry to float, the operator an FP32 condensation and assembly
+   would produce at best: from u_0 = the Dirichlet values, step j forms the
+   FP64 residual r_j = rhs - K u_j on the free rows, solves K32 d = r_j with
+   the context's preconditioned CG (FP64 arithmetic, stopping at
+   ||r|| <= inner_rtol ||r_j||; 0 keeps the context's rtol) and sets
+   u_{j+1} = u_j + d.  relres[j] = ||r_j|| / ||r_0|| for j = 0..outer
+   (outer + 1 values, outer <= 64); inner_iters[j] (may be null) the CG
+   iterations of step j.  Leaves u_outer as the trace (hdg_recover and
+   hdg_errornorm then measure it) and the assembled system as it was.
+   Allocates a second copy of the trace-matrix values while it runs.
+   HDG_ERR_NOT_CONVERGED if an inner solve fails (entries after it are
+   NaN). */
+int hdg_refine_probe(hdg_ctx* ctx, int outer, double inner_rtol, double* relres, int32_t* inner_iters);
 
 /* ---- serving: a request loop over one device ---------------------------- */
 /* No counterpart in the reference (it solves one problem per script run).

@@ -75,3 +75,42 @@ def test_spd_solution_and_kappa_fallback(hdg, O):
         b = _ops(hdg, ctx, m, k, 6, 1.0, kap, hdg.SCHEDULE_LU)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("k,qdeg", [(1, 2), (2, 4), (3, 6)])
+def test_spd_schedules_same_bits(hdg, k, qdeg):
+    """Both register budgets and every grid cap of the Schur-complement
+    kernel give the same operators bit for bit; hdg_autotune reports the
+    Schur form, and its choice set back through hdg_set_schedule gives the
+    same solution as the default."""
+    print(f"[spd schedules] k={k} qdeg={qdeg}", flush=True)
+    m = hdg.rectangle_mesh((37, 29), jitter=0.2, seed=31)
+    with hdg.Context(0) as ctx:
+        ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=qdeg, rtol=1e-12)
+        ctx.set_source()
+        d = m.boundary_dofs(k + 1)
+        ctx.set_dirichlet(d, np.zeros(len(d)))
+        ref = None
+        for waves in (2, 3):
+            for grid in (7, 512, 16384):
+                ctx.set_schedule(0, grid, waves)
+                ctx.assemble_condense()
+                got = ctx.get_condensed() + ctx.get_local_ops()
+                if ref is None:
+                    ref = got
+                else:
+                    for a, b in zip(ref, got):
+                        assert np.array_equal(a, b), (waves, grid)
+        with pytest.raises(hdg.HDGError):
+            ctx.set_schedule(0, 0, 1)
+        ctx.set_schedule(0, 0, 0)
+        ctx.run()
+        u_default = ctx.trace()
+        sc = ctx.autotune()
+        print(f"k={k}: tuned {sc}", flush=True)
+        assert sc["condense_form"] == 1 and sc["condense_seg"] == 16 and sc["condense_waves"] in (2, 3)
+        assert 0 < sc["condense_us"] <= sc["default_us"] * 1.0001
+        ctx.set_schedule(**sc)
+        ctx.run()
+        assert np.array_equal(ctx.trace(), u_default)

@@ -535,6 +535,7 @@ def test_condense_schedules_same_bits(hdg, ctx, k):
     u_default = ctx.trace()
     sc = ctx.autotune()
     print(f"k={k}: tuned {sc}", flush=True)
+    assert sc["condense_form"] == 0
     assert sc["condense_seg"] in (16, 32, 64) and sc["condense_seg"] >= nat
     assert 0 < sc["condense_us"] <= sc["default_us"] * 1.0001
     ctx.run()
