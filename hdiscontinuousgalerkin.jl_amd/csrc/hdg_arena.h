@@ -63,8 +63,6 @@ struct hdg_arena {
 };
 
 // One hipMalloc of `bytes` on `device`; HDG_OK or HDG_ERR_HIP.
-
-// One hipMalloc of `bytes` on `device`; HDG_OK or HDG_ERR_HIP.
 int hdg_arena_create(hdg_arena** out, int device, size_t bytes);
 // Frees the arena (every context that used it must be destroyed first).
 void hdg_arena_destroy(hdg_arena* a);

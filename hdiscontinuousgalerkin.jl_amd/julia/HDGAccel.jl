@@ -146,6 +146,33 @@ function set_dirichlet!(ctx::Context, dbc)
                      ctx.h, length(dbc.prescribed_dofs), dbc.prescribed_dofs, dbc.values))
 end
 
+"""Dirichlet(û_h, mesh, "boundary", g) (boundary.jl:11-42) computed on the
+GPU for g(x) = c0 + c1 x + c2 y (g_kind 0, par = (c0, c1, c2)) or
+sin(πx) sin(πy) (g_kind 1); reference = true reproduces the reference's own
+weights and accumulator instead of the projection."""
+function set_dirichlet_function!(ctx::Context, g_kind::Integer, par::NTuple{3,Float64}; reference::Bool = false)
+    p = collect(par)
+    check(ctx, ccall((:hdg_set_dirichlet_function, libhdg), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint),
+                     ctx.h, g_kind, p, reference ? 1 : 0))
+end
+
+"""apply!(K, b, dbc) (boundary.jl:121-175) done on the GPU to the assembled
+system: returns (K, b) with the Dirichlet rows and columns zeroed,
+meandiag(K) on their diagonal and b lifted, as a SparseMatrixCSC and a
+vector (the device's own system is left unchanged)."""
+function applied_system(ctx::Context)
+    s = sizes(ctx)
+    colptr = Vector{Int64}(undef, s.n_dofs + 1)
+    rowval = Vector{Int64}(undef, s.nnz)
+    nzval = Vector{Float64}(undef, s.nnz)
+    rhs = Vector{Float64}(undef, s.n_dofs)
+    md = Ref(0.0)
+    check(ctx, ccall((:hdg_get_applied_system, libhdg), Cint,
+                     (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.h, colptr, rowval, nzval, rhs, md))
+    SparseMatrixCSC(s.n_dofs, s.n_dofs, colptr, rowval, nzval), rhs
+end
+
 function recover!(ctx::Context, σ_h, u_h, û_h)
     check(ctx, ccall((:hdg_recover, libhdg), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
                      ctx.h, σ_h.m_values, u_h.m_values, û_h.m_values))

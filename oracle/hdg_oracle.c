@@ -18,7 +18,13 @@
  *     definiteness of the trace operator, nnz and face-count formulas,
  *     convergence rate k+1;
  *   - textbook reductions: Gauss-Legendre vs numpy.polynomial.legendre,
- *     dense LU vs numpy.linalg.solve, banded vs dense global solves.
+ *     dense LU vs numpy.linalg.solve, banded vs dense global solves;
+ *   - the sine-series source by its manufactured solution
+ *     u = sum c_pq sin(p pi x) sin(q pi y) / (pi^2 (p^2 + q^2)) at rate k+1;
+ *   - the Dirichlet values: the projection reproduces linear g exactly, and
+ *     the reference's own computation (cell-rule weights at face points, an
+ *     accumulator never reset, src/boundary.jl:27-37) against closed forms
+ *     of those weights and sums.
  * The Lagrange-P1 geometry basis via an inverted Dubiner Vandermonde
  * (src/basis.jl:264-299) and LAPACK's exact pivot order are restated up to
  * rounding; they agree with exact arithmetic to ~1e-16, which the pins above
