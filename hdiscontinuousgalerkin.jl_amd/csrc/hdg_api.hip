@@ -35,7 +35,10 @@ struct hdg_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // second stream for the multigrid set-up beside the Lanczos steps
   hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_fill = nullptr, ev_join = nullptr;
+  // hdg_run queued the multigrid set-up on aux beside the assembly; the
+  // solve joins it instead of queueing it
+  bool mg_forked = false;
   std::string err;
 
   // mesh
@@ -453,12 +456,46 @@ int autotune_impl(hdg_ctx* c) {
   return HDG_OK;
 }
 
-int assemble_impl(hdg_ctx* c) {
+bool use_mg(const hdg_ctx* c);
+bool mixed(const hdg_ctx* c);
+
+// The context's second stream and its events, created on first use.
+int ensure_aux(hdg_ctx* c) {
+  if (c->aux) return HDG_OK;
+  HIPCHK(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_fill, hipEventDisableTiming));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  return HDG_OK;
+}
+
+// Condensation, trace matrix and Jacobi blocks.  With mg_early (hdg_run on
+// a multigrid context with its own streams), the multigrid set-up goes to
+// the second stream as soon as its inputs exist: the per-cell coarse blocks
+// after the condensation (beside the fill and the Jacobi blocks), S P0 and
+// the vertex levels after the fill; the solve joins it.
+int assemble_impl(hdg_ctx* c, bool mg_early = false) {
   if (!c->has_setup) return fail(c, HDG_ERR_STATE, "hdg_setup has not been called");
   if ((c->opt.schedule & HDG_SCHEDULE_AUTOTUNE) && !c->tuned) TRY(autotune_impl(c));
+  c->mg_forked = false;
+  const bool early = mg_early && use_mg(c) && c->arena == nullptr;
   HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
   LAUNCHCHK(c, launch_condense_cells(c, condense_args(c)));
+  if (early) {
+    TRY(ensure_aux(c));
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    LAUNCHCHK(c, mg_setup_cells(c->mesh, c->mg, c->T, c->d_Ate, c->aux));
+  }
   LAUNCHCHK(c, launch_fill(c->pat, c->d_Ate, c->d_bte, c->T, c->stream));
+  if (early) {
+    HIPCHK(c, hipEventRecord(c->ev_fill, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fill, 0));
+    LAUNCHCHK(c, mg_setup_sp(c->mg, c->T, c->pat, mixed(c) ? c->f_sp : nullptr, c->aux));
+    LAUNCHCHK(c, mg_setup_levels(c->mg, c->d_mg, c->aux));
+    HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
+    c->mg_forked = true;
+  }
   LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
   c->jacobi_dirty = false;
   c->assembled = true;
@@ -569,18 +606,17 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
     // the other slots keep the device busy, and a second stream per slot
     // would outnumber the hardware queues.
     const bool fork = c->arena == nullptr;
-    if (fork && !c->aux) {
-      HIPCHK(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
-      HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-      HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    if (!c->mg_forked) {
+      if (fork) TRY(ensure_aux(c));
+      hipStream_t ss = fork ? c->aux : c->stream;
+      if (fork) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+      }
+      LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, mx ? c->f_sp : nullptr, ss));
+      if (fork) HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
     }
-    hipStream_t ss = fork ? c->aux : c->stream;
-    if (fork) {
-      HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-      HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-    }
-    LAUNCHCHK(c, mg_setup_numeric(c->mesh, c->mg, c->d_mg, c->T, c->pat, c->d_Ate, mx ? c->f_sp : nullptr, ss));
-    if (fork) HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
+    c->mg_forked = false;
     LAUNCHCHK(c, launch_lanczos_omega(c->pat, c->T, c->cg, Whole{c->T.N, c->cg.st}, c->cg.nblocks, nullptr, 1,
                                       c->mg.omega_factor, c->mg.lanczos_steps, c->stream));
     if (fork) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -722,6 +758,7 @@ void hdg_destroy(hdg_ctx* c) {
     (void)hipStreamDestroy(c->aux);
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_fill) (void)hipEventDestroy(c->ev_fill);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream) {
     (void)hipStreamSynchronize(c->stream);
@@ -1187,7 +1224,7 @@ int hdg_errornorm(hdg_ctx* c, int kind, const double* par, double* sq) {
 int hdg_run(hdg_ctx* c, hdg_solve_info* info) {
   if (!c) return HDG_ERR_ARG;
   TRY(select_device(c));
-  TRY(assemble_impl(c));
+  TRY(assemble_impl(c, true));
   int flag = 0;
   const int r = solve_impl(c, info, &flag);
   // a singular local matrix or a negative Jacobian is reported as such, not

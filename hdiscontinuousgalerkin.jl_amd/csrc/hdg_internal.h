@@ -373,6 +373,11 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, const MgData* dM, TraceSyst
 // its arrays at its own offsets; dM is its device copy)
 int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
                   const double* Ate, float* sp32, hipStream_t s);
+// mg_setup_mesh in two parts: what needs only the condensed blocks (vertex
+// masks, per-cell coarse blocks) and what reads the assembled trace matrix
+// (S P0), so that the first can run beside the assembly
+int mg_setup_cells(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const double* Ate, hipStream_t s);
+int mg_setup_sp(const MgData& M, const TraceSystem& T, const DevPattern& P, float* sp32, hipStream_t s);
 int mg_setup_levels(const MgData& M, const MgData* dM, hipStream_t s);
 // omega = factor / lambda_max(Dinv S), lambda_max the largest Ritz value of
 // `steps` block-Jacobi PCG iterations from a random right-hand side (the

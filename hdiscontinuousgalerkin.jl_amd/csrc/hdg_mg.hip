@@ -114,7 +114,8 @@ __device__ inline double sum_partials_mg(const double* p, int n, double* sh) {
 }
 
 // (S x)_row for S = -K on free rows, from the face-ELL matrix stored as V
-// (accumulated in FP64).
+// (accumulated in FP64; the values by nontemporal loads, as in the CG
+// product).
 template <int NT, class V>
 __device__ inline double s_row(const DevPattern& P, const TraceOp<V>& T, const V* x, int64_t r) {
   const int64_t f = r / NT;
@@ -124,7 +125,7 @@ __device__ inline double s_row(const DevPattern& P, const TraceOp<V>& T, const V
     const int64_t g = (int64_t)__ldg(P.bcol + f * kSlots + s) * NT;
 #pragma unroll
     for (int b = 0; b < NT; ++b)
-      acc = fma((double)__ldg(T.vals + (int64_t)(s * NT + b) * T.N + r), (double)x[g + b], acc);
+      acc = fma((double)__builtin_nontemporal_load(T.vals + (int64_t)(s * NT + b) * T.N + r), (double)x[g + b], acc);
   }
   return T.dmask[r] ? 0.0 : -acc;
 }
@@ -972,16 +973,26 @@ int vcycle_group(const MgGroup& G, hipStream_t s) {
 
 }  // namespace
 
-int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
-                  const double* Ate, float* sp32, hipStream_t s) {
+int mg_setup_cells(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const double* Ate, hipStream_t s) {
   hipLaunchKernelGGL(mg_vmask_kernel, dim3(grid1(M.nv)), dim3(256), 0, s, M, T);
 #define CELL(NT) hipLaunchKernelGGL(mg_cell_coarse_kernel<NT>, dim3(grid1(mesh.nc)), dim3(256), 0, s, mesh, M, T, Ate);
   MG_NT_DISPATCH(T.nt, CELL)
 #undef CELL
+  return (int)hipGetLastError();
+}
+
+int mg_setup_sp(const MgData& M, const TraceSystem& T, const DevPattern& P, float* sp32, hipStream_t s) {
 #define SPK(NT) hipLaunchKernelGGL(mg_sp_kernel<NT>, dim3(grid1(T.N)), dim3(256), 0, s, P, T, M, sp32);
   MG_NT_DISPATCH(T.nt, SPK)
 #undef SPK
   return (int)hipGetLastError();
+}
+
+int mg_setup_mesh(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const DevPattern& P,
+                  const double* Ate, float* sp32, hipStream_t s) {
+  const int e = mg_setup_cells(mesh, M, T, Ate, s);
+  if (e) return e;
+  return mg_setup_sp(M, T, P, sp32, s);
 }
 
 int mg_setup_levels(const MgData& M, const MgData* dM, hipStream_t s) {
