@@ -71,6 +71,10 @@ struct hdg_ctx {
 
   // per-cell data
   double *d_Ate = nullptr, *d_bte = nullptr, *d_Ke = nullptr, *d_be = nullptr, *d_qu = nullptr;
+  // the last condensation took the Schur form, which writes only the u rows
+  // of K_e and b_e (recovery forms q itself); ke_full: the q rows have been
+  // filled in since (hdg_get_local_ops)
+  bool ke_compact = false, ke_full = true;
   int* d_err = nullptr;
 
   // trace system and solver
@@ -440,9 +444,18 @@ CondenseArgs condense_args(const hdg_ctx* c) {
 }
 
 // condensation of every cell by the form condense_args selects
-int launch_condense_cells(const hdg_ctx* c, const CondenseArgs& a) {
+int launch_condense_cells(hdg_ctx* c, const CondenseArgs& a) {
+  c->ke_compact = a.spd_tables != nullptr;
+  c->ke_full = !c->ke_compact;
   if (a.spd_tables) return launch_condense_spd(c->k, a, c->csched, c->stream);
   return launch_condense(c->k, a, false, c->stream, c->csched);
+}
+
+// [q; u] = K_e uhat_c + b_e on every cell
+int launch_recover_cells(const hdg_ctx* c) {
+  if (c->ke_compact)
+    return launch_recover_spd(c->k, c->mesh, c->d_spd, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream);
+  return launch_recover(c->k, c->mesh, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream);
 }
 
 int autotune_impl(hdg_ctx* c) {
@@ -659,7 +672,7 @@ int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit, int* err_flag) {
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   LAUNCHCHK(c, launch_true_residual(c->pat, c->T, c->d_uhat, c->d_scal + 1, c->cg, c->stream));
   if (err_flag) {
-    LAUNCHCHK(c, launch_recover(c->k, c->mesh, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream));
+    LAUNCHCHK(c, launch_recover_cells(c));
     HIPCHK(c, hipMemcpyAsync(err_flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   }
   CgState st;
@@ -689,7 +702,7 @@ int finish_solve(hdg_ctx* c, hdg_solve_info* info, int maxit, int* err_flag) {
 
 int recover_impl(hdg_ctx* c) {
   if (!c->solved) return fail(c, HDG_ERR_STATE, "hdg_solve has not been called");
-  LAUNCHCHK(c, launch_recover(c->k, c->mesh, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream));
+  LAUNCHCHK(c, launch_recover_cells(c));
   c->recovered = true;
   return HDG_OK;
 }
@@ -1317,6 +1330,10 @@ int hdg_get_local_ops(hdg_ctx* c, double* Ke, double* be) {
   if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
   TRY(select_device(c));
   const int64_t nc = c->mesh.nc;
+  if ((Ke || be) && !c->ke_full) {
+    LAUNCHCHK(c, launch_expand_ke_spd(c->k, c->mesh, c->d_spd, c->d_Ke, c->d_be, c->stream));
+    c->ke_full = true;
+  }
   if (Ke) HIPCHK(c, hipMemcpyAsync(Ke, c->d_Ke, sizeof(double) * nc * c->m * c->ntr, hipMemcpyDeviceToHost, c->stream));
   if (be) HIPCHK(c, hipMemcpyAsync(be, c->d_be, sizeof(double) * nc * c->m, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -692,7 +692,11 @@ int enqueue_finish(hdg_batch* b, Group& g) {
   BLAUNCH(b, launch_cg_finish(g.T, g.cg, g.uhat, g.s));
   BLAUNCH(b, launch_true_residual_t(g.pat, g.T, g.uhat, g.cg.part, gr, g.nbt, g.s));
   BLAUNCH(b, launch_reduce_segs(g.cg.part, g.segs, n, g.res, g.s));
-  BLAUNCH(b, launch_recover(g.k, g.mesh, g.Ke, g.be, g.uhat, g.qu, g.s));
+  // a group on the Schur form holds only the u rows of K_e and b_e
+  if (g.d_spd)
+    BLAUNCH(b, launch_recover_spd(g.k, g.mesh, g.d_spd, g.Ke, g.be, g.uhat, g.qu, g.s));
+  else
+    BLAUNCH(b, launch_recover(g.k, g.mesh, g.Ke, g.be, g.uhat, g.qu, g.s));
   bool want = false;
   for (const Inst& I : g.in) want = want || b->reqs[I.req].uex_kind >= 0;
   if (want) {

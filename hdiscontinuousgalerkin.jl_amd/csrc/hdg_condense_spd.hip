@@ -267,27 +267,133 @@ void condense_spd_kernel(CondenseArgs a) {
     }
     __builtin_amdgcn_wave_barrier();
 
-    // Q = A^-1 (B U - E) = sum_d (J^-1)_dc P_d U - |e_f| n_fc / detJ V_fo
-    // (trace columns); Qb = sum_d (J^-1)_dc P_d Ub (rhs column)
+    // the u rows of K_e (U, trace columns) and of b_e (Ub, rhs column); the
+    // q rows Q = A^-1 (B U - E) follow from them and the cell's geometry,
+    // and are formed where they are needed (recover_spd_kernel for the
+    // recovered q, expand_ke_spd_kernel for hdg_get_local_ops): applied to
+    // one trace vector at recovery they cost an eighth of forming them here
+    // for every column, and their stores were a fifth of this kernel's time
     if (valid && ls <= NTR) {
       double* out = ls < NTR ? a.Ke + cell * (int64_t)(M * NTR) + ls * M : a.be + cell * (int64_t)M;
-      const double vs = ls < NTR ? el / g.detJ : 0.0;
-#pragma unroll 1
-      for (int i = 0; i < NU; ++i) {
-        double t0 = 0.0, t1 = 0.0;
 #pragma unroll
-        for (int j = 0; j < NU; ++j) {
-          t0 += T[D::P + i * NU + j] * rc[j];
-          t1 += T[D::P + (NU + i) * NU + j] * rc[j];
-        }
-        const double v = vs * T[D::V + (fo * NU + i) * NT + tl];
-        out[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - nx * v;
-        out[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - ny * v;
-        out[NQ + i] = rc[i];
-      }
+      for (int i = 0; i < NU; ++i) out[NQ + i] = rc[i];
     }
     if (valid && bad && ls == 0) atomicOr(a.err + (GROUPED ? a.cell_inst[cell] : 0), g.detJ > 0.0 ? 2 : 1);
   }
+}
+
+// The q rows of column t of K_e from its u rows (U_t, or Ub for the rhs
+// column) and the cell's geometry:
+//   Q_t = A^-1 (B U_t - E_t) = sum_d (J^-1)_dc P_d U_t - |e_f| n_fc / detJ V_fo(:, s)
+// for trace column t = (face f, mode s), without the E term for the rhs.
+// One thread per (cell, column); tables staged in LDS.
+template <int K>
+__global__ __launch_bounds__(256) void expand_ke_spd_kernel(DevMesh m, const double* __restrict__ tab,
+                                                            double* __restrict__ Ke, double* __restrict__ be) {
+  using D = SpdDims<K>;
+  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NT = D::NT, NTR = D::NTR;
+  __shared__ double T[D::Om - D::V];  // V then P
+  for (int i = threadIdx.x; i < D::Om - D::V; i += blockDim.x) T[i] = __ldg(tab + D::V + i);
+  __syncthreads();
+  const double* V = T;
+  const double* Pm = T + (D::P - D::V);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nc * (NTR + 1)) return;
+  const int64_t cell = t / (NTR + 1);
+  const int col = (int)(t - cell * (NTR + 1));
+  Geo g;
+  cell_geometry(m, cell, g);
+  double* out = col < NTR ? Ke + cell * (int64_t)(M * NTR) + col * M : be + cell * (int64_t)M;
+  double u[NU];
+#pragma unroll
+  for (int i = 0; i < NU; ++i) u[i] = out[NQ + i];
+  const int fl = col < NTR ? col / NT : 0, tl = col < NTR ? col - fl * NT : 0;
+  const double w0 = (double)(fl == 0), w1 = (double)(fl == 1), w2 = (double)(fl == 2);
+  const double el = w0 * g.len[0] + w1 * g.len[1] + w2 * g.len[2];
+  const double nx = w0 * g.nrm[0] + w1 * g.nrm[2] + w2 * g.nrm[4];
+  const double ny = w0 * g.nrm[1] + w1 * g.nrm[3] + w2 * g.nrm[5];
+  const int fo = fl * 2 + ((fl == 0) * g.ori[0] + (fl == 1) * g.ori[1] + (fl == 2) * g.ori[2]);
+  const double vs = col < NTR ? el / g.detJ : 0.0;
+#pragma unroll 1
+  for (int i = 0; i < NU; ++i) {
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NU; ++j) {
+      t0 += Pm[i * NU + j] * u[j];
+      t1 += Pm[(NU + i) * NU + j] * u[j];
+    }
+    const double v = vs * V[(fo * NU + i) * NT + tl];
+    out[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - nx * v;
+    out[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - ny * v;
+  }
+}
+
+// Recovery (examples/poisson2D_HDG.jl:197-212) from the u rows of K_e:
+// u = U uhat_c + Ub, q = sum_d (J^-1)_dc P_d u - sum_f |e_f| n_fc / detJ
+// V_fo uhat_f, i.e. K_e uhat_c + b_e with the q rows applied to the one
+// trace vector.  One thread per cell; tables staged in LDS.
+template <int K>
+__global__ __launch_bounds__(256) void recover_spd_kernel(DevMesh m, const double* __restrict__ tab,
+                                                          const double* __restrict__ Ke,
+                                                          const double* __restrict__ be,
+                                                          const double* __restrict__ uhat, double* __restrict__ qu) {
+  using D = SpdDims<K>;
+  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NT = D::NT, NTR = D::NTR;
+  __shared__ double T[D::Om - D::V];  // V then P
+  for (int i = threadIdx.x; i < D::Om - D::V; i += blockDim.x) T[i] = __ldg(tab + D::V + i);
+  __syncthreads();
+  const double* V = T;
+  const double* Pm = T + (D::P - D::V);
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= m.nc) return;
+  Geo g;
+  cell_geometry(m, cell, g);
+  double uh[NTR];
+#pragma unroll
+  for (int lf = 0; lf < 3; ++lf) {
+    const int64_t f = m.cf[3 * cell + lf];
+#pragma unroll
+    for (int b = 0; b < NT; ++b) uh[lf * NT + b] = uhat[f * NT + b];
+  }
+  const double* ke = Ke + cell * (int64_t)(M * NTR);
+  double u[NU];
+#pragma unroll
+  for (int i = 0; i < NU; ++i) u[i] = be[cell * M + NQ + i];
+#pragma unroll
+  for (int t = 0; t < NTR; ++t)
+#pragma unroll
+    for (int i = 0; i < NU; ++i) u[i] = fma(ke[t * M + NQ + i], uh[t], u[i]);
+  double wx[NU], wy[NU];
+#pragma unroll
+  for (int i = 0; i < NU; ++i) wx[i] = wy[i] = 0.0;
+#pragma unroll
+  for (int lf = 0; lf < 3; ++lf) {
+    const double sc = g.len[lf] / g.detJ;
+    const double sx = sc * g.nrm[2 * lf], sy = sc * g.nrm[2 * lf + 1];
+    const int fo = lf * 2 + g.ori[lf];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) v = fma(V[(fo * NU + i) * NT + b], uh[lf * NT + b], v);
+      wx[i] = fma(sx, v, wx[i]);
+      wy[i] = fma(sy, v, wy[i]);
+    }
+  }
+  double* o = qu + cell * (int64_t)M;
+#pragma unroll 1
+  for (int i = 0; i < NU; ++i) {
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NU; ++j) {
+      t0 = fma(Pm[i * NU + j], u[j], t0);
+      t1 = fma(Pm[(NU + i) * NU + j], u[j], t1);
+    }
+    o[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - wx[i];
+    o[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - wy[i];
+  }
+#pragma unroll
+  for (int i = 0; i < NU; ++i) o[NQ + i] = u[i];
 }
 
 template <int K>
@@ -331,6 +437,32 @@ int launch_condense_spd(int k, const CondenseArgs& a, const CondenseSched& sc, h
     case 3: return spd_launch<3>(a, sc.spd_wpe, sc.spd_grid, s);
     default: return (int)hipErrorInvalidValue;
   }
+}
+
+int launch_recover_spd(int k, const DevMesh& m, const double* tab, const double* Ke, const double* be,
+                       const double* uhat, double* qu, hipStream_t s) {
+  const int64_t g = (m.nc + 255) / 256;
+  const dim3 grid((unsigned)(g > 0 ? g : 1));
+  switch (k) {
+    case 1: hipLaunchKernelGGL(recover_spd_kernel<1>, grid, dim3(256), 0, s, m, tab, Ke, be, uhat, qu); break;
+    case 2: hipLaunchKernelGGL(recover_spd_kernel<2>, grid, dim3(256), 0, s, m, tab, Ke, be, uhat, qu); break;
+    case 3: hipLaunchKernelGGL(recover_spd_kernel<3>, grid, dim3(256), 0, s, m, tab, Ke, be, uhat, qu); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_expand_ke_spd(int k, const DevMesh& m, const double* tab, double* Ke, double* be, hipStream_t s) {
+  const int ncol = 3 * (k + 1) + 1;
+  const int64_t g = (m.nc * ncol + 255) / 256;
+  const dim3 grid((unsigned)(g > 0 ? g : 1));
+  switch (k) {
+    case 1: hipLaunchKernelGGL(expand_ke_spd_kernel<1>, grid, dim3(256), 0, s, m, tab, Ke, be); break;
+    case 2: hipLaunchKernelGGL(expand_ke_spd_kernel<2>, grid, dim3(256), 0, s, m, tab, Ke, be); break;
+    case 3: hipLaunchKernelGGL(expand_ke_spd_kernel<3>, grid, dim3(256), 0, s, m, tab, Ke, be); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
 }
 
 int tune_condense_spd(int k, const CondenseArgs& a, hipStream_t s, CondenseSched* best, double* best_us,
