@@ -74,18 +74,20 @@ def condense_work(ctx, form):
     over j < m; building the rows and the quadrature sums are not counted.
     Flops of the Schur form (form 1), per cell with nu = m/3: S (6 nu^2 fma),
     its Cholesky factor (nu^3/6), R (3 nu ntr), L^-1 [R be] and L^-T of the
-    same ((ntr+1) nu^2/2 each), Y'Y (ntr^2 nu), bte (ntr nu), Q = A^-1(BU-E)
-    (2 nu^2 (ntr+1)) and the load vector (nqp nu)."""
+    same ((ntr+1) nu^2/2 each), Y'Y (ntr^2 nu), bte (ntr nu) and the load
+    vector (nqp nu); that form writes only the u rows of K_e and b_e
+    (nu ntr + nu), the recovery forms q."""
     s = ctx.sizes
     m, ntr, nc = s.m, s.ntr, s.n_cells
     rows = m + ntr
     if form == 1:
         nu = m // 3
         fma = (6 * nu * nu + nu ** 3 / 6 + 3 * nu * ntr + (ntr + 1) * nu * nu + ntr * ntr * nu + ntr * nu
-               + 2 * nu * nu * (ntr + 1) + s.nqp * nu)
+               + s.nqp * nu)
+        byts = nc * (72 + 8 * (ntr * ntr + ntr + nu * ntr + nu))
     else:
         fma = (rows - 1) * sum(m + ntr - j for j in range(m))
-    byts = nc * (72 + 8 * (ntr * ntr + ntr + m * ntr + m))
+        byts = nc * (72 + 8 * (ntr * ntr + ntr + m * ntr + m))
     return byts, 2 * fma * nc
 
 
