@@ -331,7 +331,10 @@ __global__ __launch_bounds__(256) void expand_ke_spd_kernel(DevMesh m, const dou
 // Recovery (examples/poisson2D_HDG.jl:197-212) from the u rows of K_e:
 // u = U uhat_c + Ub, q = sum_d (J^-1)_dc P_d u - sum_f |e_f| n_fc / detJ
 // V_fo uhat_f, i.e. K_e uhat_c + b_e with the q rows applied to the one
-// trace vector.  One thread per cell; tables staged in LDS.
+// trace vector.  16 lanes per cell (4 per wavefront): lane i < nu forms
+// u_i (reading row i of U: the lanes of a cell read runs of nu doubles),
+// then q_i from u, which the lanes of the cell exchange by shuffles.
+// Tables staged in LDS.
 template <int K>
 __global__ __launch_bounds__(256) void recover_spd_kernel(DevMesh m, const double* __restrict__ tab,
                                                           const double* __restrict__ Ke,
@@ -339,12 +342,16 @@ __global__ __launch_bounds__(256) void recover_spd_kernel(DevMesh m, const doubl
                                                           const double* __restrict__ uhat, double* __restrict__ qu) {
   using D = SpdDims<K>;
   constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NT = D::NT, NTR = D::NTR;
+  static_assert(NU <= kSeg, "segment too small");
   __shared__ double T[D::Om - D::V];  // V then P
   for (int i = threadIdx.x; i < D::Om - D::V; i += blockDim.x) T[i] = __ldg(tab + D::V + i);
   __syncthreads();
   const double* V = T;
   const double* Pm = T + (D::P - D::V);
-  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int sbase = (lane / kSeg) * kSeg, ls = lane - sbase;
+  // every lane of a segment has the same cell: segments leave together
+  const int64_t cell = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kSeg;
   if (cell >= m.nc) return;
   Geo g;
   cell_geometry(m, cell, g);
@@ -355,45 +362,36 @@ __global__ __launch_bounds__(256) void recover_spd_kernel(DevMesh m, const doubl
 #pragma unroll
     for (int b = 0; b < NT; ++b) uh[lf * NT + b] = uhat[f * NT + b];
   }
-  const double* ke = Ke + cell * (int64_t)(M * NTR);
-  double u[NU];
+  const int i = ls < NU ? ls : NU - 1;  // lanes past nu repeat the last row and store nothing
+  const double* ke = Ke + cell * (int64_t)(M * NTR) + NQ + i;
+  double u = be[cell * M + NQ + i];
 #pragma unroll
-  for (int i = 0; i < NU; ++i) u[i] = be[cell * M + NQ + i];
-#pragma unroll
-  for (int t = 0; t < NTR; ++t)
-#pragma unroll
-    for (int i = 0; i < NU; ++i) u[i] = fma(ke[t * M + NQ + i], uh[t], u[i]);
-  double wx[NU], wy[NU];
-#pragma unroll
-  for (int i = 0; i < NU; ++i) wx[i] = wy[i] = 0.0;
+  for (int t = 0; t < NTR; ++t) u = fma(ke[t * M], uh[t], u);
+  double wx = 0.0, wy = 0.0;
 #pragma unroll
   for (int lf = 0; lf < 3; ++lf) {
     const double sc = g.len[lf] / g.detJ;
     const double sx = sc * g.nrm[2 * lf], sy = sc * g.nrm[2 * lf + 1];
     const int fo = lf * 2 + g.ori[lf];
+    double v = 0.0;
 #pragma unroll
-    for (int i = 0; i < NU; ++i) {
-      double v = 0.0;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) v = fma(V[(fo * NU + i) * NT + b], uh[lf * NT + b], v);
-      wx[i] = fma(sx, v, wx[i]);
-      wy[i] = fma(sy, v, wy[i]);
-    }
+    for (int b = 0; b < NT; ++b) v = fma(V[(fo * NU + i) * NT + b], uh[lf * NT + b], v);
+    wx = fma(sx, v, wx);
+    wy = fma(sy, v, wy);
   }
-  double* o = qu + cell * (int64_t)M;
-#pragma unroll 1
-  for (int i = 0; i < NU; ++i) {
-    double t0 = 0.0, t1 = 0.0;
+  double t0 = 0.0, t1 = 0.0;
 #pragma unroll
-    for (int j = 0; j < NU; ++j) {
-      t0 = fma(Pm[i * NU + j], u[j], t0);
-      t1 = fma(Pm[(NU + i) * NU + j], u[j], t1);
-    }
-    o[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - wx[i];
-    o[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - wy[i];
+  for (int j = 0; j < NU; ++j) {
+    const double uj = __shfl(u, sbase + j, 64);
+    t0 = fma(Pm[i * NU + j], uj, t0);
+    t1 = fma(Pm[(NU + i) * NU + j], uj, t1);
   }
-#pragma unroll
-  for (int i = 0; i < NU; ++i) o[NQ + i] = u[i];
+  if (ls < NU) {
+    double* o = qu + cell * (int64_t)M;
+    o[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - wx;
+    o[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - wy;
+    o[NQ + i] = u;
+  }
 }
 
 template <int K>
@@ -441,7 +439,7 @@ int launch_condense_spd(int k, const CondenseArgs& a, const CondenseSched& sc, h
 
 int launch_recover_spd(int k, const DevMesh& m, const double* tab, const double* Ke, const double* be,
                        const double* uhat, double* qu, hipStream_t s) {
-  const int64_t g = (m.nc + 255) / 256;
+  const int64_t g = (m.nc * kSeg + 255) / 256;
   const dim3 grid((unsigned)(g > 0 ? g : 1));
   switch (k) {
     case 1: hipLaunchKernelGGL(recover_spd_kernel<1>, grid, dim3(256), 0, s, m, tab, Ke, be, uhat, qu); break;

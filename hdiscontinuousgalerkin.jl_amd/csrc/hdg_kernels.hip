@@ -529,15 +529,32 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
   bool dm[NT];
 #pragma unroll
   for (int i = 0; i < NT; ++i) dm[i] = T.dmask[f * NT + i] != 0;
+  // column j of the face's diagonal block: the NT consecutive rows of
+  // slot column (s0, j), read in 16-byte pieces where NT is even (the block
+  // starts at a multiple of 2 doubles)
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+  for (int j = 0; j < NT; ++j) {
+    const double* col = T.vals + (int64_t)(s0 * NT + j) * T.N + f * NT;
+    double cv[NT];
+    if constexpr (NT % 2 == 0) {
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      double v = -T.vals[(int64_t)(s0 * NT + j) * T.N + f * NT + i];
+      for (int i = 0; i < NT; i += 2) {
+        const double2 w = *reinterpret_cast<const double2*>(col + i);
+        cv[i] = w.x;
+        cv[i + 1] = w.y;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NT; ++i) cv[i] = col[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      double v = -cv[i];
       if (dm[i] || dm[j]) v = (i == j) ? 1.0 : 0.0;
       a[i][j] = v;
       a[i][NT + j] = (i == j) ? 1.0 : 0.0;
     }
+  }
 #pragma unroll
   for (int c = 0; c < NT; ++c) {
     int p = c;
@@ -566,10 +583,18 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
       }
   }
 #pragma unroll
-  for (int i = 0; i < NT; ++i)
+  for (int j = 0; j < NT; ++j) {
+    double* out = T.Dinv + (int64_t)j * T.N + f * NT;
+    if constexpr (NT % 2 == 0) {
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
-      T.Dinv[(int64_t)j * T.N + f * NT + i] = (dm[i] || dm[j]) ? 0.0 : a[i][NT + j];
+      for (int i = 0; i < NT; i += 2)
+        *reinterpret_cast<double2*>(out + i) = make_double2((dm[i] || dm[j]) ? 0.0 : a[i][NT + j],
+                                                             (dm[i + 1] || dm[j]) ? 0.0 : a[i + 1][NT + j]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NT; ++i) out[i] = (dm[i] || dm[j]) ? 0.0 : a[i][NT + j];
+    }
+  }
 }
 
 // ------------------------------------------------------------------- CG
