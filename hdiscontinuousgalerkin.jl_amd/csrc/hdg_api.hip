@@ -71,9 +71,9 @@ struct hdg_ctx {
 
   // per-cell data
   double *d_Ate = nullptr, *d_bte = nullptr, *d_Ke = nullptr, *d_be = nullptr, *d_qu = nullptr;
-  // the last condensation took the Schur form, which writes only the u rows
-  // of K_e and b_e (recovery forms q itself); ke_full: the q rows have been
-  // filled in since (hdg_get_local_ops)
+  // K_e and b_e hold the compact u rows of the Schur form (recovery forms q
+  // itself), or (ke_full) their full layout: from the elimination form, or
+  // rewritten by hdg_get_local_ops
   bool ke_compact = false, ke_full = true;
   int* d_err = nullptr;
 
@@ -453,7 +453,7 @@ int launch_condense_cells(hdg_ctx* c, const CondenseArgs& a) {
 
 // [q; u] = K_e uhat_c + b_e on every cell
 int launch_recover_cells(const hdg_ctx* c) {
-  if (c->ke_compact)
+  if (c->ke_compact && !c->ke_full)
     return launch_recover_spd(c->k, c->mesh, c->d_spd, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream);
   return launch_recover(c->k, c->mesh, c->d_Ke, c->d_be, c->d_uhat, c->d_qu, c->stream);
 }

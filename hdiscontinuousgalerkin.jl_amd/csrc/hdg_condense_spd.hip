@@ -267,26 +267,32 @@ void condense_spd_kernel(CondenseArgs a) {
     }
     __builtin_amdgcn_wave_barrier();
 
-    // the u rows of K_e (U, trace columns) and of b_e (Ub, rhs column); the
-    // q rows Q = A^-1 (B U - E) follow from them and the cell's geometry,
-    // and are formed where they are needed (recover_spd_kernel for the
-    // recovered q, expand_ke_spd_kernel for hdg_get_local_ops): applied to
+    // the u rows of K_e (U, trace columns) and of b_e (Ub, rhs column),
+    // compact at the start of the cell's K_e and b_e blocks (U column-major
+    // nu x ntr); the q rows Q = A^-1 (B U - E) follow from them and the
+    // cell's geometry, and are formed where they are needed
+    // (recover_spd_kernel for the recovered q, expand_ke_spd_kernel for
+    // hdg_get_local_ops, which also restores the layout of K_e): applied to
     // one trace vector at recovery they cost an eighth of forming them here
     // for every column, and their stores were a fifth of this kernel's time
     if (valid && ls <= NTR) {
-      double* out = ls < NTR ? a.Ke + cell * (int64_t)(M * NTR) + ls * M : a.be + cell * (int64_t)M;
+      double* out = ls < NTR ? a.Ke + cell * (int64_t)(M * NTR) + ls * NU : a.be + cell * (int64_t)M;
 #pragma unroll
-      for (int i = 0; i < NU; ++i) out[NQ + i] = rc[i];
+      for (int i = 0; i < NU; ++i) out[i] = rc[i];
     }
     if (valid && bad && ls == 0) atomicOr(a.err + (GROUPED ? a.cell_inst[cell] : 0), g.detJ > 0.0 ? 2 : 1);
   }
 }
 
-// The q rows of column t of K_e from its u rows (U_t, or Ub for the rhs
-// column) and the cell's geometry:
+// K_e and b_e in their exported layout (column t of K_e: q rows then u rows)
+// from the compact u rows the Schur kernel stores (U column-major at the
+// start of the cell's block, Ub at the start of b_e's), with the q rows
 //   Q_t = A^-1 (B U_t - E_t) = sum_d (J^-1)_dc P_d U_t - |e_f| n_fc / detJ V_fo(:, s)
 // for trace column t = (face f, mode s), without the E term for the rhs.
-// One thread per (cell, column); tables staged in LDS.
+// One thread per cell, in place: column t (written at t m) reads compact
+// column t (at t nu) before writing, and the columns run from the last, so
+// every compact column a write covers has been read already.  Tables
+// staged in LDS.
 template <int K>
 __global__ __launch_bounds__(256) void expand_ke_spd_kernel(DevMesh m, const double* __restrict__ tab,
                                                             double* __restrict__ Ke, double* __restrict__ be) {
@@ -297,34 +303,38 @@ __global__ __launch_bounds__(256) void expand_ke_spd_kernel(DevMesh m, const dou
   __syncthreads();
   const double* V = T;
   const double* Pm = T + (D::P - D::V);
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m.nc * (NTR + 1)) return;
-  const int64_t cell = t / (NTR + 1);
-  const int col = (int)(t - cell * (NTR + 1));
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= m.nc) return;
   Geo g;
   cell_geometry(m, cell, g);
-  double* out = col < NTR ? Ke + cell * (int64_t)(M * NTR) + col * M : be + cell * (int64_t)M;
-  double u[NU];
-#pragma unroll
-  for (int i = 0; i < NU; ++i) u[i] = out[NQ + i];
-  const int fl = col < NTR ? col / NT : 0, tl = col < NTR ? col - fl * NT : 0;
-  const double w0 = (double)(fl == 0), w1 = (double)(fl == 1), w2 = (double)(fl == 2);
-  const double el = w0 * g.len[0] + w1 * g.len[1] + w2 * g.len[2];
-  const double nx = w0 * g.nrm[0] + w1 * g.nrm[2] + w2 * g.nrm[4];
-  const double ny = w0 * g.nrm[1] + w1 * g.nrm[3] + w2 * g.nrm[5];
-  const int fo = fl * 2 + ((fl == 0) * g.ori[0] + (fl == 1) * g.ori[1] + (fl == 2) * g.ori[2]);
-  const double vs = col < NTR ? el / g.detJ : 0.0;
+  double* ke = Ke + cell * (int64_t)(M * NTR);
 #pragma unroll 1
-  for (int i = 0; i < NU; ++i) {
-    double t0 = 0.0, t1 = 0.0;
+  for (int col = NTR; col >= 0; --col) {
+    // col == NTR: the rhs column (b_e)
+    const double* in = col < NTR ? ke + col * NU : be + cell * (int64_t)M;
+    double* out = col < NTR ? ke + col * M : be + cell * (int64_t)M;
+    double u[NU];
 #pragma unroll
-    for (int j = 0; j < NU; ++j) {
-      t0 += Pm[i * NU + j] * u[j];
-      t1 += Pm[(NU + i) * NU + j] * u[j];
+    for (int i = 0; i < NU; ++i) u[i] = in[i];
+    const int fl = col < NTR ? col / NT : 0, tl = col < NTR ? col - fl * NT : 0;
+    const double el = g.len[fl];
+    const double nx = g.nrm[2 * fl], ny = g.nrm[2 * fl + 1];
+    const int fo = fl * 2 + g.ori[fl];
+    const double vs = col < NTR ? el / g.detJ : 0.0;
+#pragma unroll 1
+    for (int i = 0; i < NU; ++i) {
+      double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < NU; ++j) {
+        t0 += Pm[i * NU + j] * u[j];
+        t1 += Pm[(NU + i) * NU + j] * u[j];
+      }
+      const double v = vs * V[(fo * NU + i) * NT + tl];
+      out[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - nx * v;
+      out[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - ny * v;
     }
-    const double v = vs * V[(fo * NU + i) * NT + tl];
-    out[i] = g.Ji[0] * t0 + g.Ji[1] * t1 - nx * v;
-    out[NU + i] = g.Ji[2] * t0 + g.Ji[3] * t1 - ny * v;
+#pragma unroll
+    for (int i = 0; i < NU; ++i) out[NQ + i] = u[i];
   }
 }
 
@@ -363,10 +373,12 @@ __global__ __launch_bounds__(256) void recover_spd_kernel(DevMesh m, const doubl
     for (int b = 0; b < NT; ++b) uh[lf * NT + b] = uhat[f * NT + b];
   }
   const int i = ls < NU ? ls : NU - 1;  // lanes past nu repeat the last row and store nothing
-  const double* ke = Ke + cell * (int64_t)(M * NTR) + NQ + i;
-  double u = be[cell * M + NQ + i];
+  // the compact u rows: U column-major (nu x ntr) at the start of the cell's
+  // K_e block, Ub at the start of its b_e block
+  const double* ke = Ke + cell * (int64_t)(M * NTR) + i;
+  double u = be[cell * M + i];
 #pragma unroll
-  for (int t = 0; t < NTR; ++t) u = fma(ke[t * M], uh[t], u);
+  for (int t = 0; t < NTR; ++t) u = fma(ke[t * NU], uh[t], u);
   double wx = 0.0, wy = 0.0;
 #pragma unroll
   for (int lf = 0; lf < 3; ++lf) {
@@ -451,8 +463,7 @@ int launch_recover_spd(int k, const DevMesh& m, const double* tab, const double*
 }
 
 int launch_expand_ke_spd(int k, const DevMesh& m, const double* tab, double* Ke, double* be, hipStream_t s) {
-  const int ncol = 3 * (k + 1) + 1;
-  const int64_t g = (m.nc * ncol + 255) / 256;
+  const int64_t g = (m.nc + 255) / 256;
   const dim3 grid((unsigned)(g > 0 ? g : 1));
   switch (k) {
     case 1: hipLaunchKernelGGL(expand_ke_spd_kernel<1>, grid, dim3(256), 0, s, m, tab, Ke, be); break;

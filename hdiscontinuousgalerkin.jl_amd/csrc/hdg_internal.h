@@ -303,9 +303,11 @@ int condense_native_seg(int k);
 // outputs as launch_condense up to rounding.  Its autotuner times the
 // (spd_wpe, spd_grid) schedules and keeps the fastest in *best.
 int launch_condense_spd(int k, const CondenseArgs& a, const CondenseSched& sc, hipStream_t s);
-// The Schur-complement kernel writes only the u rows of K_e and b_e.
+// The Schur-complement kernel writes only the u rows of K_e and b_e, compact
+// at the start of each cell's blocks (U column-major nu x ntr, then Ub).
 // launch_recover_spd recovers [q; u] from them (and the SPD tables `tab`);
-// launch_expand_ke_spd fills the q rows in, for the export of K_e.
+// launch_expand_ke_spd rewrites K_e and b_e in place in their full layout
+// (q rows formed), for the export of K_e.
 int launch_recover_spd(int k, const DevMesh& m, const double* tab, const double* Ke, const double* be,
                        const double* uhat, double* qu, hipStream_t s);
 int launch_expand_ke_spd(int k, const DevMesh& m, const double* tab, double* Ke, double* be, hipStream_t s);

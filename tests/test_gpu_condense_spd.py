@@ -114,3 +114,43 @@ def test_spd_schedules_same_bits(hdg, k, qdeg):
         ctx.set_schedule(**sc)
         ctx.run()
         assert np.array_equal(ctx.trace(), u_default)
+
+
+@pytest.mark.parametrize("k,qdeg", [(1, 2), (1, 4), (2, 4), (3, 6)])
+def test_spd_recovery_vs_oracle(hdg, O, k, qdeg):
+    """Recovery on the Schur form (the kernel keeps only the u rows of K_e
+    and forms q from u and the geometry) for a trace given from outside
+    (hdg_set_trace), against the oracle's recovery with its own K_e and b_e
+    (poisson2D_HDG.jl:197-212) to 1e-12, and against the exported K_e
+    applied on the host."""
+    print(f"[spd recovery] k={k} qdeg={qdeg}", flush=True)
+    m = hdg.rectangle_mesh((11, 9), jitter=0.2, seed=21)
+    om = O.rectangle_mesh(11, 9, jitter=0.2, seed=21)
+    P = O.problem(k=k, qdeg=qdeg)
+    _, _, oK, obe = O.condense(P, om)
+    rng = np.random.default_rng(k * 10 + qdeg)
+    nt = k + 1
+    uhat = rng.standard_normal(om.nfaces * nt)
+    oq, ou, _ = O.recover(om, k, oK, obe, uhat)
+    with hdg.Context(0) as ctx:
+        ctx.set_mesh(m)
+        ctx.setup(k, quad_degree=qdeg)
+        ctx.set_source()
+        d = m.boundary_dofs(nt)
+        ctx.set_dirichlet(d, np.zeros(len(d)))
+        ctx.assemble_condense()
+        ctx.set_trace(uhat)
+        q, u, _ = ctx.recover()
+        Ke, be = ctx.get_local_ops()
+        # the export rewrote K_e in its full layout: recovery reads that now
+        ctx.set_trace(uhat)
+        q2, u2, _ = ctx.recover()
+    rel = lambda a, b: float(np.abs(a - b).max() / np.abs(b).max())
+    assert rel(u, ou) < 1e-12 and rel(q, oq) < 1e-12
+    assert rel(u2, u) < 1e-13 and rel(q2, q) < 1e-13
+    # the exported K_e (its q rows filled in on demand) applied on the host
+    faces = np.asarray(m.cells)[:, 3:] - 1          # local faces 0..2 of every cell
+    uc = uhat.reshape(-1, nt)[faces].reshape(len(faces), -1)
+    qu = np.einsum("cmt,ct->cm", Ke, uc) + be
+    nq = q.shape[1]
+    assert rel(qu[:, :nq], q) < 1e-12 and rel(qu[:, nq:], u) < 1e-12
