@@ -791,7 +791,8 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
         for (int sl = 0; sl < kSlots; ++sl) {
           const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-          for (int c = 0; c < NT; ++c) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
+          for (int c = 0; c < NT; ++c)  // uD is zero off the Dirichlet dofs
+            if (T.dmask[g + c]) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
         }
         b = -s;
       }

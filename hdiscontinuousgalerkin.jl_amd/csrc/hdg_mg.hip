@@ -472,7 +472,8 @@ __global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov
 }
 
 // init (update(0)) for the multigrid path: r0 = b~ = -(rhs - K uD), x = p = 0,
-// e0 = omega Dinv r0.
+// e0 = omega Dinv r0.  uD is zero off the Dirichlet dofs, so the lift reads
+// only the matrix columns of Dirichlet dofs (the mask decides per column).
 template <int NT, class W, class G>
 __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSystem T64, TraceOp<W> T,
                                                           Krylov<double> v, MgVecs<W> mv, G gsel) {
@@ -494,7 +495,8 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
         for (int sl = 0; sl < kSlots; ++sl) {
           const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-          for (int c = 0; c < NT; ++c) s -= T64.vals[(int64_t)(sl * NT + c) * T64.N + r] * T64.uD[g + c];
+          for (int c = 0; c < NT; ++c)
+            if (T64.dmask[g + c]) s -= T64.vals[(int64_t)(sl * NT + c) * T64.N + r] * T64.uD[g + c];
         }
         b = -s;
       }
