@@ -214,6 +214,7 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->mg.cellA);
   dfree(c, c->mg.e0);
   dfree(c, c->mg.res0);
+  dfree(c, c->mg.fc);
   dfree(c, c->mg.sp);
   dfree(c, c->d_mg);
   dfree(c, c->f_vals);
@@ -240,6 +241,7 @@ void free_mesh(hdg_ctx* c) {
   dfree(c, c->mg.fv);
   dfree(c, c->mg.vf_ptr);
   dfree(c, c->mg.vf_idx);
+  dfree(c, c->mg.vf2);
   dfree(c, c->mg.fq4);
   dfree(c, c->mg.tailpack);
   dfree(c, c->mg.cinv);
@@ -368,6 +370,7 @@ int build_vertex_structures(hdg_ctx* c, int64_t nn, int64_t nf, const int64_t* f
   TRY(dalloc(c, &M.fv, 2 * nf));
   TRY(dalloc(c, &M.vf_ptr, nn + 1));
   TRY(dalloc(c, &M.vf_idx, idx.size()));
+  TRY(dalloc(c, &M.vf2, idx.size()));
   HIPCHK(c, hipMemcpyAsync(M.fv, fv.data(), sizeof(int) * fv.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(M.vf_ptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(M.vf_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->stream));
@@ -876,6 +879,7 @@ int hdg_set_rectangle_mesh(hdg_ctx* c, int64_t nx, int64_t ny, double x0, double
   TRY(dalloc(c, &M.fv, 2 * nf));
   TRY(dalloc(c, &M.vf_ptr, nn + 1));
   TRY(dalloc(c, &M.vf_idx, 2 * nf));
+  TRY(dalloc(c, &M.vf2, 2 * nf));
   TRY(dalloc(c, &M.fq4, 4 * nf));
   HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
   RectMeshArgs a;
@@ -1023,6 +1027,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     TRY(dalloc(c, &c->mg.cellA, 9 * nc));
     TRY(dalloc(c, &c->mg.e0, N));
     TRY(dalloc(c, &c->mg.res0, N));
+    TRY(dalloc(c, &c->mg.fc, 2 * c->mesh.nf));
     TRY(dalloc(c, &c->mg.sp, 4 * N));
     TRY(dalloc(c, &c->d_mg, 1));
     HIPCHK(c, hipMemcpyAsync(c->d_mg, &c->mg, sizeof(MgData), hipMemcpyHostToDevice, c->stream));

@@ -73,7 +73,7 @@ struct Group {
   int* err = nullptr;  // [ninst] condensation error flags
   SrcParams* srcs = nullptr;
   int* cell_inst = nullptr;
-  int *fv = nullptr, *vf_ptr = nullptr, *vf_idx = nullptr, *fq4 = nullptr;  // face end points, vertex -> faces
+  int *fv = nullptr, *vf_ptr = nullptr, *vf_idx = nullptr, *vf2 = nullptr, *fq4 = nullptr;  // face end points, vertex -> faces
   // launch tables
   CgState* st = nullptr;
   Seg *segs = nullptr, *csegs = nullptr;
@@ -303,6 +303,7 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &g.fv, 2 * g.nf));
   BTRY(galloc(b, g, &g.vf_ptr, g.nn + 1));
   BTRY(galloc(b, g, &g.vf_idx, 2 * g.nf));
+  BTRY(galloc(b, g, &g.vf2, 2 * g.nf));
   if (g.mg) BTRY(galloc(b, g, &g.fq4, 4 * g.nf));
   BTRY(galloc(b, g, &merr, 1));
   BHIP(b, hipMemsetAsync(merr, 0, sizeof(int), g.s));
@@ -459,6 +460,7 @@ int build_group(hdg_batch* b, Group& g) {
   U.fv = g.fv;
   U.vf_ptr = g.vf_ptr;
   U.vf_idx = g.vf_idx;
+  U.vf2 = g.vf2;
   U.fq4 = g.fq4;
   U.cellA = cellA;
   {
@@ -474,6 +476,7 @@ int build_group(hdg_batch* b, Group& g) {
   }
   BTRY(galloc(b, g, &U.e0, N));
   BTRY(galloc(b, g, &U.res0, N));
+  BTRY(galloc(b, g, &U.fc, 2 * g.nf));
   BTRY(galloc(b, g, &U.sp, 4 * N));
   if (g.mixed) {
     BTRY(galloc(b, g, &g.f_vals, (size_t)kSlots * g.nt * N));

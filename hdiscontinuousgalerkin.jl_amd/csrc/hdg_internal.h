@@ -225,6 +225,9 @@ struct MgData {
   int* fv = nullptr;             // [2*nf] lower, higher node id of each face
   int* vf_ptr = nullptr;         // vertex -> incident faces (CSR)
   int* vf_idx = nullptr;
+  int* vf2 = nullptr;            // [2*nf] the same entries as 2 f + (1 where the vertex is f's higher end)
+  double* fc = nullptr;          // [2*nf] per face: P0^T res0 restricted to the face, at its lower and
+                                 // higher end (even nt: written by the level-0 residual)
   double* cellA = nullptr;       // [9*nc] per-cell P^T (-Ate) P
   double* e0 = nullptr;          // trace-level correction
   double* res0 = nullptr;        // trace-level residual

@@ -528,13 +528,38 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
 }
 
 // res0 = r - S e0 over all rows (of every instance when packed; v.st null:
-// no convergence check).
+// no convergence check).  For even nt the lane of mode 0 of a face also
+// writes the face's restriction weights fc (P0^T res0 at its lower and
+// higher end: 0.5 res0_0 -+ res0_1 / (2 sqrt 3)), with mode 1 taken from the
+// next lane (a face's rows are consecutive lanes of one wavefront).
 template <int NT, class W>
 __global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceOp<W> T, Krylov<double> v,
-                                                        MgVecs<W> mv) {
+                                                        MgVecs<W> mv, double* __restrict__ fc) {
   if (v.st && v.st->done) return;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x)
-    mv.res0[r] = T.dmask[r] ? (W)0 : (W)(v.r[r] - s_row<NT, W>(P, T, mv.e0, r));
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x) {
+    const W res = T.dmask[r] ? (W)0 : (W)(v.r[r] - s_row<NT, W>(P, T, mv.e0, r));
+    mv.res0[r] = res;
+    if constexpr (NT % 2 == 0) {
+      // the grid is a multiple of 64 threads and N of NT: lane r + 1 of a
+      // mode-0 lane r is in the same wavefront and active
+      const double r1 = (double)__shfl_down((double)res, 1, 64);
+      if (r % NT == 0) {
+        const double r0 = (double)res;
+        fc[2 * (r / NT)] = 0.5 * r0 + (-kS3) * r1;
+        fc[2 * (r / NT) + 1] = 0.5 * r0 + kS3 * r1;
+      }
+    }
+  }
+}
+
+// vf2: the vertex -> face entries with the side of the vertex on the face.
+__global__ void mg_vf2_kernel(MgData M) {
+  const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vv >= M.nv) return;
+  for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
+    const int f = M.vf_idx[k];
+    M.vf2[k] = 2 * f + (M.fv[2 * f] == vv ? 0 : 1);
+  }
 }
 
 // r1 = P0^T res0, vertex by vertex over the incident faces.
@@ -545,12 +570,18 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgD
   const int64_t vv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (vv >= L.n) return;
   double s = 0.0;
-  if (!L.mask[vv])
-    for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
-      const int64_t f = M.vf_idx[k];
-      const bool lo = M.fv[2 * f] == vv;
-      s += 0.5 * (double)mv.res0[f * NT] + (lo ? -kS3 : kS3) * (double)mv.res0[f * NT + 1];
+  if (!L.mask[vv]) {
+    if constexpr (NT % 2 == 0) {
+      // the weights the residual kernel wrote per face end
+      for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) s += M.fc[M.vf2[k]];
+    } else {
+      for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
+        const int64_t f = M.vf_idx[k];
+        const bool lo = M.fv[2 * f] == vv;
+        s += 0.5 * (double)mv.res0[f * NT] + (lo ? -kS3 : kS3) * (double)mv.res0[f * NT + 1];
+      }
     }
+  }
   L.r[vv] = s;
 }
 
@@ -977,6 +1008,7 @@ int vcycle_group(const MgGroup& G, hipStream_t s) {
 
 int mg_setup_cells(const DevMesh& mesh, const MgData& M, const TraceSystem& T, const double* Ate, hipStream_t s) {
   hipLaunchKernelGGL(mg_vmask_kernel, dim3(grid1(M.nv)), dim3(256), 0, s, M, T);
+  if (T.nt % 2 == 0) hipLaunchKernelGGL(mg_vf2_kernel, dim3(grid1(M.nv)), dim3(256), 0, s, M);
 #define CELL(NT) hipLaunchKernelGGL(mg_cell_coarse_kernel<NT>, dim3(grid1(mesh.nc)), dim3(256), 0, s, mesh, M, T, Ate);
   MG_NT_DISPATCH(T.nt, CELL)
 #undef CELL
@@ -1075,7 +1107,7 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
                       const MgData* dM, const MgVecs<W>& mv, hipStream_t s) {
   const int gr = grid1(T.N, 256);
   const int grows = gr > 4096 ? 4096 : gr;
-#define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv);
+#define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv, M.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES
 #define RST(NT) \
@@ -1102,7 +1134,7 @@ int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Kryl
   vu.st = nullptr;
   const int gr = grid1(T.N, 256);
   const int grows = gr > 8192 ? 8192 : gr;
-#define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, vu, mv);
+#define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, vu, mv, Mu.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES
 #define RST(NT) \
