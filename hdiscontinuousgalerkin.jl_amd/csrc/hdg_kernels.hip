@@ -699,7 +699,9 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
   if (threadIdx.x == 0) v.part[blockIdx.x] = bs;
 }
 
-template <int NT, class G>
+// X false: the iterate x is not carried (the Lanczos run behind the
+// smoothing weight needs only the CG coefficients).
+template <int NT, class G, bool X = true>
 __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVectors v, G gsel, int i_fixed,
                                                             int force) {
   constexpr int NTH = 64 * NT;
@@ -729,10 +731,9 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
     const int64_t r = r0 + threadIdx.x;
     double rn = 0.0;
     if (r < S.hi) {
-      const double xn = fma(alpha, p[r], v.x[r]);
       rn = fma(-alpha, v.q[r], v.r[r]);
       if (!force) {
-        v.x[r] = xn;
+        if constexpr (X) v.x[r] = fma(alpha, p[r], v.x[r]);
         v.r[r] = rn;
       }
     }
@@ -1230,7 +1231,12 @@ int launch_lanczos_omega(const DevPattern& P, const TraceSystem& T, CgVectors& v
   const Krylov<double> kv{v.r, v.z, v.p0, v.p1, v.q, v.x, v.part, v.st, v.nblocks};
   for (int i = 0; i < steps; ++i) {
     int e = launch_cg_spmv_t(P, op, kv, g, grid, 0, 0.0, steps, 0, s);
-    if (!e) e = launch_cg_update_t(T, v, g, grid, 0, 0, s);
+    if (e) return e;
+#define LUPD(NT) \
+  hipLaunchKernelGGL((cg_update_kernel<NT, G, false>), dim3(grid), dim3(64 * NT), 0, s, T, v, g, 0, 0);
+    HDG_NT_DISPATCH(T.nt, LUPD)
+#undef LUPD
+    e = (int)hipGetLastError();
     if (e) return e;
   }
   if (seg)

@@ -1106,7 +1106,7 @@ template <class W>
 int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<double>& v, const MgData& M,
                       const MgData* dM, const MgVecs<W>& mv, hipStream_t s) {
   const int gr = grid1(T.N, 256);
-  const int grows = gr > 4096 ? 4096 : gr;
+  const int grows = gr > 16384 ? 16384 : gr;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv, M.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES
