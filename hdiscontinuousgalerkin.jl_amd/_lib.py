@@ -75,7 +75,7 @@ class Sizes(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("n_cells", C.c_int64), ("n_faces", C.c_int64),
                 ("n_dofs", C.c_int64), ("nnz", C.c_int64), ("nu", C.c_int32), ("nq", C.c_int32),
                 ("nt", C.c_int32), ("m", C.c_int32), ("ntr", C.c_int32), ("nqp", C.c_int32),
-                ("nfq", C.c_int32)]
+                ("nfq", C.c_int32), ("sym_blocks", C.c_int32)]
 
 
 class SolveInfo(C.Structure):

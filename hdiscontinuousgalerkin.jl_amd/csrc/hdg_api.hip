@@ -87,7 +87,7 @@ struct hdg_ctx {
 
   // mixed precision (HDG_MIXED): the multigrid preconditioner's trace-level
   // data in float: trace matrix, Jacobi blocks, S P0, e0, res0
-  float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr;
+  float *f_vals = nullptr, *f_blk = nullptr, *f_Dinv = nullptr, *f_sp = nullptr;
   float *f_e0 = nullptr, *f_res0 = nullptr;
 
   // captured chunks of CG iterations, one per read-back slot (valid until
@@ -200,6 +200,7 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->d_be);
   dfree(c, c->d_qu);
   dfree(c, c->T.vals);
+  dfree(c, c->T.blk);
   dfree(c, c->T.rhs);
   dfree(c, c->T.Dinv);
   dfree(c, c->cg.x);
@@ -218,6 +219,7 @@ void free_problem(hdg_ctx* c) {
   dfree(c, c->mg.sp);
   dfree(c, c->d_mg);
   dfree(c, c->f_vals);
+  dfree(c, c->f_blk);
   dfree(c, c->f_Dinv);
   dfree(c, c->f_sp);
   dfree(c, c->f_e0);
@@ -238,6 +240,7 @@ void free_mesh(hdg_ctx* c) {
   dfree(c, c->pat.nslot);
   dfree(c, c->pat.self);
   dfree(c, c->pat.tslot);
+  dfree(c, c->pat.nbr);
   dfree(c, c->mg.fv);
   dfree(c, c->mg.vf_ptr);
   dfree(c, c->mg.vf_idx);
@@ -526,10 +529,10 @@ bool mixed(const hdg_ctx* c) { return use_mg(c) && c->opt.precision == HDG_MIXED
 
 // views of the trace system and Krylov vectors for the CG kernels
 TraceOp<double> op64(const hdg_ctx* c) {
-  return TraceOp<double>{c->T.nt, c->T.nf, c->T.N, c->T.vals, c->T.Dinv, c->T.dmask};
+  return TraceOp<double>{c->T.nt, c->T.nf, c->T.N, c->T.vals, c->T.Dinv, c->T.dmask, c->T.blk};
 }
 TraceOp<float> op32(const hdg_ctx* c) {
-  return TraceOp<float>{c->T.nt, c->T.nf, c->T.N, c->f_vals, c->f_Dinv, c->T.dmask};
+  return TraceOp<float>{c->T.nt, c->T.nf, c->T.N, c->f_vals, c->f_Dinv, c->T.dmask, c->f_blk};
 }
 Krylov<double> kv64(const hdg_ctx* c) {
   const CgVectors& v = c->cg;
@@ -610,7 +613,10 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
     const bool mx = mixed(c);
     if (mx) {
       const int64_t N = c->T.N;
-      LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
+      if (c->T.blk)
+        LAUNCHCHK(c, launch_to_float(c->T.blk, c->f_blk, (int64_t)3 * c->nt * N, c->stream));
+      else
+        LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
       LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
     }
     // The multigrid set-up (small, latency-bound kernels writing the level
@@ -845,6 +851,7 @@ int hdg_set_mesh(hdg_ctx* c, int64_t nn, const double* xy, int64_t nc, const int
   TRY(dalloc(c, &c->pat.nslot, nf));
   TRY(dalloc(c, &c->pat.self, nf));
   TRY(dalloc(c, &c->pat.tslot, kSlots * nf));
+  TRY(dalloc(c, &c->pat.nbr, 4 * nf));
   LAUNCHCHK(c, launch_pattern(c->mesh, c->pat, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->h_cf.swap(cf);
@@ -914,6 +921,7 @@ int hdg_set_rectangle_mesh(hdg_ctx* c, int64_t nx, int64_t ny, double x0, double
   TRY(dalloc(c, &c->pat.nslot, nf));
   TRY(dalloc(c, &c->pat.self, nf));
   TRY(dalloc(c, &c->pat.tslot, kSlots * nf));
+  TRY(dalloc(c, &c->pat.nbr, 4 * nf));
   LAUNCHCHK(c, launch_pattern(c->mesh, c->pat, c->stream));
   int flag = 0;
   HIPCHK(c, hipMemcpyAsync(&flag, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1005,6 +1013,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   c->T.nf = nf;
   c->T.N = N;
   TRY(dalloc(c, &c->T.vals, (size_t)kSlots * c->nt * N));
+  if (sym_blocks(c->nt)) TRY(dalloc(c, &c->T.blk, (size_t)3 * c->nt * N));
   TRY(dalloc(c, &c->T.rhs, N));
   TRY(dalloc(c, &c->T.Dinv, (size_t)c->nt * N));
   TRY(dalloc(c, &c->d_dmask, N));
@@ -1033,7 +1042,10 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
     HIPCHK(c, hipMemcpyAsync(c->d_mg, &c->mg, sizeof(MgData), hipMemcpyHostToDevice, c->stream));
   }
   if (c->structured && o.precision == HDG_MIXED) {
-    TRY(dalloc(c, &c->f_vals, (size_t)kSlots * c->nt * N));
+    if (c->T.blk)
+      TRY(dalloc(c, &c->f_blk, (size_t)3 * c->nt * N));
+    else
+      TRY(dalloc(c, &c->f_vals, (size_t)kSlots * c->nt * N));
     TRY(dalloc(c, &c->f_Dinv, (size_t)c->nt * N));
     TRY(dalloc(c, &c->f_sp, 4 * N));
     TRY(dalloc(c, &c->f_e0, N));
@@ -1151,6 +1163,7 @@ int hdg_query_sizes(const hdg_ctx* c, hdg_sizes* s) {
   s->nfq = c->nfq;
   // rows of a face with two cells hold 5 block columns, with one cell 3
   s->nnz = c->nblock_cols * (int64_t)c->nt * c->nt;
+  s->sym_blocks = c->T.blk ? 1 : 0;
   return HDG_OK;
 }
 
@@ -1504,10 +1517,11 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
   if (inner_iters)
     for (int j = 0; j < outer; ++j) inner_iters[j] = 0;
   const int64_t N = c->T.N, nv = (int64_t)kSlots * c->nt * N;
-  double *vr = nullptr, *u = nullptr, *res = nullptr, *zero = nullptr, *part = nullptr;
+  const int64_t nb = c->T.blk ? (int64_t)3 * c->nt * N : 0;
+  double *vr = nullptr, *vb = nullptr, *u = nullptr, *res = nullptr, *zero = nullptr, *part = nullptr;
   int rc = HDG_OK;
   if ((rc = dalloc(c, &vr, nv)) == HDG_OK && (rc = dalloc(c, &u, N)) == HDG_OK && (rc = dalloc(c, &res, N)) == HDG_OK &&
-      (rc = dalloc(c, &zero, N)) == HDG_OK)
+      (rc = dalloc(c, &zero, N)) == HDG_OK && (!nb || (rc = dalloc(c, &vb, nb)) == HDG_OK))
     rc = dalloc(c, &part, refine_partials() + 1);
   // graphs hold the operator's pointers: none is replayed or kept across the swap
   drop_graphs(c);
@@ -1519,6 +1533,7 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
     HIPCHK(c, hipMemcpyAsync(u, c->T.uD, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(zero, 0, sizeof(double) * N, c->stream));
     LAUNCHCHK(c, launch_round_float(c->T.vals, vr, nv, c->stream));
+    if (nb) LAUNCHCHK(c, launch_round_float(c->T.blk, vb, nb, c->stream));
     double r0 = 0.0;
     for (int j = 0;; ++j) {
       c->T = saved;
@@ -1531,6 +1546,7 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
       if (j == outer) return HDG_OK;
       // K32 d = r_j: rhs r_j, no Dirichlet lift, the rounded values
       c->T.vals = vr;
+      if (nb) c->T.blk = vb;
       c->T.rhs = res;
       c->T.uD = zero;
       LAUNCHCHK(c, launch_jacobi(c->pat, c->T, c->stream));
@@ -1556,6 +1572,7 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
     (void)hipStreamSynchronize(c->stream);
   }
   dfree(c, vr);
+  dfree(c, vb);
   dfree(c, u);
   dfree(c, res);
   dfree(c, zero);

@@ -89,7 +89,7 @@ struct Group {
   MgGroup G;
   MgSetupGroup S;
   CondenseSched csched;  // condensation schedule (autotuned at create with HDG_SCHEDULE_AUTOTUNE)
-  float *f_vals = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr;
+  float *f_vals = nullptr, *f_blk = nullptr, *f_Dinv = nullptr, *f_sp = nullptr, *f_e0 = nullptr, *f_res0 = nullptr;
   // read-back
   CgState* h_st = nullptr;  // pinned, 2 x ninst (chunk slots) + ninst (final)
   double* h_res = nullptr;  // pinned, 2 x ninst, then ninst error flags (ints)
@@ -347,6 +347,7 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &g.pat.nslot, g.nf));
   BTRY(galloc(b, g, &g.pat.self, g.nf));
   BTRY(galloc(b, g, &g.pat.tslot, kSlots * g.nf));
+  BTRY(galloc(b, g, &g.pat.nbr, 4 * g.nf));
   BLAUNCH(b, launch_pattern(g.mesh, g.pat, g.s));
   BTRY(galloc(b, g, &g.d_tab, L.total));
   BHIP(b, hipMemcpyAsync(g.d_tab, tab.data.data(), sizeof(double) * L.total, hipMemcpyHostToDevice, g.s));
@@ -379,6 +380,7 @@ int build_group(hdg_batch* b, Group& g) {
   g.T.nf = g.nf;
   g.T.N = N;
   BTRY(galloc(b, g, &g.T.vals, (size_t)kSlots * g.nt * N));
+  if (sym_blocks(g.nt)) BTRY(galloc(b, g, &g.T.blk, (size_t)3 * g.nt * N));
   BTRY(galloc(b, g, &g.T.rhs, N));
   BTRY(galloc(b, g, &g.T.Dinv, (size_t)g.nt * N));
   g.T.dmask = g.dmask;
@@ -479,7 +481,10 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &U.fc, 2 * g.nf));
   BTRY(galloc(b, g, &U.sp, 4 * N));
   if (g.mixed) {
-    BTRY(galloc(b, g, &g.f_vals, (size_t)kSlots * g.nt * N));
+    if (g.T.blk)
+      BTRY(galloc(b, g, &g.f_blk, (size_t)3 * g.nt * N));
+    else
+      BTRY(galloc(b, g, &g.f_vals, (size_t)kSlots * g.nt * N));
     BTRY(galloc(b, g, &g.f_Dinv, (size_t)g.nt * N));
     BTRY(galloc(b, g, &g.f_sp, 4 * N));
     BTRY(galloc(b, g, &g.f_e0, N));
@@ -582,8 +587,12 @@ int build_group(hdg_batch* b, Group& g) {
 }
 
 Grouped rows(const Group& g) { return Grouped{g.segs, g.binst}; }
-TraceOp<double> op64(const Group& g) { return TraceOp<double>{g.nt, g.nf, g.N, g.T.vals, g.T.Dinv, g.dmask}; }
-TraceOp<float> op32(const Group& g) { return TraceOp<float>{g.nt, g.nf, g.N, g.f_vals, g.f_Dinv, g.dmask}; }
+TraceOp<double> op64(const Group& g) {
+  return TraceOp<double>{g.nt, g.nf, g.N, g.T.vals, g.T.Dinv, g.dmask, g.T.blk};
+}
+TraceOp<float> op32(const Group& g) {
+  return TraceOp<float>{g.nt, g.nf, g.N, g.f_vals, g.f_Dinv, g.dmask, g.f_blk};
+}
 Krylov<double> kv(const Group& g) {
   const CgVectors& v = g.cg;
   return Krylov<double>{v.r, v.z, v.p0, v.p1, v.q, v.x, v.part, v.st, v.nblocks};
@@ -611,7 +620,10 @@ int enqueue_setup(hdg_batch* b, Group& g) {
   }
   const double factor = g.Mi[0].omega_factor;
   if (g.mixed) {
-    BLAUNCH(b, launch_to_float(g.T.vals, g.f_vals, (int64_t)kSlots * g.nt * g.N, g.s));
+    if (g.T.blk)
+      BLAUNCH(b, launch_to_float(g.T.blk, g.f_blk, (int64_t)3 * g.nt * g.N, g.s));
+    else
+      BLAUNCH(b, launch_to_float(g.T.vals, g.f_vals, (int64_t)kSlots * g.nt * g.N, g.s));
     BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)g.nt * g.N, g.s));
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, g.f_sp, g.s));
     BLAUNCH(b, mg_setup_levels_group(g.S, g.s));

@@ -35,13 +35,32 @@ struct DevMesh {
 // the t-th contributing element block, encoded cell*9 + la*3 + lb (local
 // face of f, local face of the column face), or -1.  tslot[f*5+s] is the
 // slot of f in the row of bcol[f*5+s] (the transpose map).
+//
+// nbr[4f .. 4f+3] indexes the symmetric block storage (TraceSystem::blk).
+// In every cell the face of local index l owns the block coupling it to
+// local face (l + 1) % 3, so a face owns one off-diagonal block per cell
+// (k = 0 for fcell[2f], 1 for fcell[2f+1]) and meets the other face of each
+// of its cells through that face's block, transposed.  nbr[4f+k] is the
+// face its block k couples to; nbr[4f+2+k] = 2 g + k' names the block k' of
+// face g that holds K_gf.  A boundary face's missing block is zero:
+// nbr[4f+1] = f and nbr[4f+3] = 2f + 1.
 struct DevPattern {
   int* bcol = nullptr;
   int* src = nullptr;
   unsigned char* nslot = nullptr;
   unsigned char* self = nullptr;
   unsigned char* tslot = nullptr;
+  int* nbr = nullptr;
 };
+
+// Symmetric block storage for 4 trace modes per face (k = 3): a face's
+// block row is two 16-byte pieces per row of the block, and the product
+// reads 3 blocks per face from HBM instead of 5 (the transposed reads of a
+// neighbour's block come from the caches).  With 2, 3 or 5 modes the face-ELL
+// product is faster and is kept.
+template <int NT>
+constexpr bool kSymBlocks = NT == 4;
+inline bool sym_blocks(int nt) { return nt == 4; }
 
 struct SrcParams {
   int kind = 0, kappa_on = 0;
@@ -132,6 +151,10 @@ struct TraceSystem {
   int nt = 0;
   int64_t nf = 0, N = 0;
   double* vals = nullptr;  // ELL, column-major: vals[(s*nt + b)*N + row]
+  // symmetric blocks (sym_blocks(nt), else null): blk[((3f + j) nt + a) nt + b],
+  // j = 0 the diagonal block K_ff, j = 1 + k the block face f owns in its
+  // cell k (row a, column b; DevPattern::nbr)
+  double* blk = nullptr;
   double* rhs = nullptr;   // assembled rhs (N)
   double* Dinv = nullptr;  // block-Jacobi inverse: Dinv[b*N + row] (0 on Dirichlet dofs)
   const unsigned char* dmask = nullptr;  // 1 = Dirichlet dof
@@ -179,7 +202,49 @@ struct TraceOp {
   const W* vals;   // face-ELL, vals[(s*nt + b)*N + row]
   const W* Dinv;   // Dinv[b*N + row]
   const unsigned char* dmask;
+  const W* blk;    // symmetric blocks (TraceSystem::blk) where kSymBlocks<nt>
+  // every field named at every construction: an operator without its
+  // blocks would send the product to a null pointer
+  __host__ __device__ TraceOp(int nt_, int64_t nf_, int64_t N_, const W* vals_, const W* Dinv_,
+                              const unsigned char* dmask_, const W* blk_)
+      : nt(nt_), nf(nf_), N(N_), vals(vals_), Dinv(Dinv_), dmask(dmask_), blk(blk_) {}
 };
+// Row a of face f of K x over the symmetric blocks (kSymBlocks<NT>), x given
+// as xv(g, b) = entry b of face g.  The diagonal block and the face's own
+// blocks are read by rows (the diagonal block once per product:
+// nontemporal); the two blocks of the neighbours that own the coupling are
+// read by columns, and their lines are shared with the owners' own reads
+// through the caches.  Entry-wide loads: 16-byte loads of the row pairs
+// made the 1024^2 P3 product slower (441 -> 473 us).
+template <int NT, class V, class X>
+__device__ __forceinline__ double sym_row(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t f, int a,
+                                          const X& xv) {
+  constexpr int B = NT * NT;
+  const int4 nb = *reinterpret_cast<const int4*>(nbr + 4 * f);
+  double acc = 0.0;
+  {
+    const V* d = blk + 3 * f * B + a * NT;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) acc = fma((double)__builtin_nontemporal_load(d + b), xv(f, b), acc);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const V* d = blk + (3 * f + 1 + k) * B + a * NT;
+    const int64_t g = k ? nb.y : nb.x;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) acc = fma((double)d[b], xv(g, b), acc);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int code = k ? nb.w : nb.z;
+    const int64_t g = code >> 1;
+    const V* d = blk + (3 * g + 1 + (code & 1)) * B + a;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) acc = fma((double)d[b * NT], xv(g, b), acc);
+  }
+  return acc;
+}
+
 template <class V>
 struct Krylov {
   V *r, *z, *p0, *p1, *q;

@@ -459,6 +459,19 @@ __global__ void pattern_kernel(DevMesh m, DevPattern P) {
   }
   P.nslot[f] = (unsigned char)n;
   P.self[f] = (unsigned char)self;
+  if (P.nbr) {
+    // the block face f owns in each cell couples it to local face la + 1;
+    // it meets local face la + 2 through that face's block
+    int o[4] = {(int)f, (int)f, (int)(2 * f + 1), (int)(2 * f + 1)};
+    const int cs[2] = {c1, c2}, ls[2] = {la1, la2};
+    for (int k = 0; k < 2; ++k) {
+      if (cs[k] < 0) continue;
+      o[k] = m.cf[3 * cs[k] + (ls[k] + 1) % 3];
+      const int g = m.cf[3 * cs[k] + (ls[k] + 2) % 3];
+      o[2 + k] = 2 * g + (m.fcell[2 * g] == cs[k] ? 0 : 1);
+    }
+    *reinterpret_cast<int4*>(P.nbr + 4 * f) = make_int4(o[0], o[1], o[2], o[3]);
+  }
 }
 
 __global__ void tslot_kernel(int64_t nf, DevPattern P) {
@@ -514,6 +527,29 @@ __global__ __launch_bounds__(256) void fill_kernel(DevPattern P, const double* _
   }
   const int sf = P.self[f];
   const int c1 = sp[2 * sf], c2 = sp[2 * sf + 1];
+  if constexpr (kSymBlocks<NT>) {
+    // symmetric blocks: K_ff (the self slot's sum, in the same order) and
+    // the block the face owns in each cell (local face la + 1); zero for
+    // the missing cell of a boundary face
+    constexpr int B = NT * NT;
+    const int cl[2] = {c1 / 9, c2 >= 0 ? c2 / 9 : -1};
+    const int ll[2] = {(c1 / 3) % 3, c2 >= 0 ? (c2 / 3) % 3 : 0};
+    double* out = T.blk + 3 * f * B + a * NT;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      double v = Ate[(int64_t)cl[0] * NTR * NTR + (ll[0] * NT + a) * NTR + ll[0] * NT + b];
+      if (cl[1] >= 0) v = v + Ate[(int64_t)cl[1] * NTR * NTR + (ll[1] * NT + a) * NTR + ll[1] * NT + b];
+      out[b] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int lb = (ll[k] + 1) % 3;
+#pragma unroll
+      for (int b = 0; b < NT; ++b)
+        out[(1 + k) * B + b] =
+            cl[k] >= 0 ? Ate[(int64_t)cl[k] * NTR * NTR + (ll[k] * NT + a) * NTR + lb * NT + b] : 0.0;
+    }
+  }
   double rh = bte[(int64_t)(c1 / 9) * NTR + ((c1 / 3) % 3) * NT + a];
   if (c2 >= 0) rh = rh + bte[(int64_t)(c2 / 9) * NTR + ((c2 / 3) % 3) * NT + a];
   T.rhs[r] = rh;
@@ -605,7 +641,8 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
 // the same bits.  Iteration i (i >= 1), host-numbered:
 //   spmv(i):   rz_i = (r, z) and rr_i = (r, r) from update(i-1)'s partials;
 //              stop if rr_i <= rtol^2 bb or i-1 >= maxit; beta = rz_i/rz_{i-1}
-//              (0 for i = 1); p_i = z + beta p_{i-1}; q = S p_i; (p_i, q)
+//              (0 for i = 1); p_i = z + beta p_{i-1}; q = S z + beta q (= S p_i);
+//              (p_i, q)
 //   update(i): alpha = rz_i/(p_i, q); x += alpha p_i; r -= alpha q;
 //              z = Dinv r; partials of (r, z) and (r, r)
 // The init kernel plays update(0): r = b~, z = Dinv r, x = p = 0.
@@ -632,10 +669,15 @@ __device__ inline double sum_partials(const double* p, int n, double* sh) {
 
 // CG iteration i, first half: convergence test on the (r, r) partials of
 // update(i-1), beta, p = z + beta p_old, q = S p, partial (p, q).  G selects
-// the instance of the block (one, or one of a group).  The matrix values are
-// read once per launch, by nontemporal loads, so that they do not displace
-// the gathered z and p_old from the caches (1024^2 P3: 562 -> 555 us;
-// 512^2 P2: 91 -> 86 us).
+// the instance of the block (one, or one of a group).  S p = S z + beta S p_old
+// and S p_old is the previous q, so the product gathers z alone and q is
+// carried by that recurrence (its rounding, like that of a direct product, is
+// of order eps ||S|| ||p||, and the beta factors damp what earlier steps left:
+// the true residual of the result is checked after every solve).  Gathering
+// z and p_old for p at every neighbour cost 1024^2 P3: 474 -> 410 us,
+// 512^2 P2: 88 -> 76 us.  The matrix values are read once per launch, by
+// nontemporal loads, so that they do not displace the gathered z from the
+// caches (1024^2 P3 face-ELL: 562 -> 555 us; 512^2 P2: 91 -> 86 us).
 template <int NT, class G>
 __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, TraceOp<double> T, Krylov<double> v,
                                                              G gsel, int i_fixed, double rtol2, int maxit,
@@ -680,16 +722,21 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
   double part = 0.0;
   for (int64_t r = S.lo + (int64_t)lb * blockDim.x + threadIdx.x; r < S.hi; r += (int64_t)S.nb * blockDim.x) {
     const int64_t f = r / NT;
-    const int* bc = P.bcol + f * kSlots;
     double acc = 0.0;
+    if constexpr (kSymBlocks<NT>) {
+      acc = sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int b) { return z[g * NT + b]; });
+    } else {
+      const int* bc = P.bcol + f * kSlots;
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-      const int64_t g = (int64_t)__ldg(bc + s) * NT;
+      for (int s = 0; s < kSlots; ++s) {
+        const int64_t g = (int64_t)__ldg(bc + s) * NT;
 #pragma unroll
-      for (int b = 0; b < NT; ++b)
-        acc = fma(__builtin_nontemporal_load(T.vals + (int64_t)(s * NT + b) * T.N + r), pnew_of(z[g + b], pold[g + b], beta), acc);
+        for (int b = 0; b < NT; ++b)
+          acc = fma(__builtin_nontemporal_load(T.vals + (int64_t)(s * NT + b) * T.N + r), z[g + b], acc);
+      }
     }
-    const double q = T.dmask[r] ? 0.0 : -acc;
+    double q = T.dmask[r] ? 0.0 : -acc;
+    if (i > 1) q = fma(beta, qo[r], q);  // S p = S z + beta S p_old
     const double pn = pnew_of(z[r], pold[r], beta);
     pnw[r] = pn;
     qo[r] = q;
@@ -1135,6 +1182,7 @@ int condense_native_seg(int k) {
 
 int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s) {
+  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
   const int gr = grid_for(T.N, 256, 1 << 30);
 #define FILL(NT) hipLaunchKernelGGL(fill_kernel<NT>, dim3(gr), dim3(256), 0, s, P, Ate, bte, T);
   HDG_NT_DISPATCH(T.nt, FILL)
@@ -1161,6 +1209,7 @@ int launch_cg_init_t(const DevPattern& P, const TraceSystem& T, CgVectors& v, G 
 template <class G>
 int launch_cg_spmv_t(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, G g, int grid,
                      int i_fixed, double rtol2, int maxit, int force, hipStream_t s) {
+  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
 #define SPMV(NT)                                                                                          \
   hipLaunchKernelGGL((cg_spmv_kernel<NT, G>), dim3(grid), dim3(kCgThreads), 0, s, P, T, v, g, i_fixed, rtol2, \
                      maxit, force);
@@ -1227,7 +1276,7 @@ int launch_lanczos_omega(const DevPattern& P, const TraceSystem& T, CgVectors& v
   hipLaunchKernelGGL((cg_init_kernel<NT, G, true>), dim3(grid), dim3(64 * NT), 0, s, P, T, v, g);
   HDG_NT_DISPATCH(T.nt, LINIT)
 #undef LINIT
-  const TraceOp<double> op{T.nt, T.nf, T.N, T.vals, T.Dinv, T.dmask};
+  const TraceOp<double> op{T.nt, T.nf, T.N, T.vals, T.Dinv, T.dmask, T.blk};
   const Krylov<double> kv{v.r, v.z, v.p0, v.p1, v.q, v.x, v.part, v.st, v.nblocks};
   for (int i = 0; i < steps; ++i) {
     int e = launch_cg_spmv_t(P, op, kv, g, grid, 0, 0.0, steps, 0, s);

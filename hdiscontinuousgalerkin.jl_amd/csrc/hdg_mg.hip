@@ -113,19 +113,25 @@ __device__ inline double sum_partials_mg(const double* p, int n, double* sh) {
   return t;
 }
 
-// (S x)_row for S = -K on free rows, from the face-ELL matrix stored as V
-// (accumulated in FP64; the values by nontemporal loads, as in the CG
-// product).
+// (S x)_row for S = -K on free rows, from the matrix stored as V: the
+// symmetric blocks at four modes per face, else the face-ELL values
+// (accumulated in FP64; values read once per product by nontemporal loads,
+// as in the CG product).
 template <int NT, class V>
 __device__ inline double s_row(const DevPattern& P, const TraceOp<V>& T, const V* x, int64_t r) {
   const int64_t f = r / NT;
   double acc = 0.0;
+  if constexpr (kSymBlocks<NT>) {
+    acc = sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int b) { return (double)x[g * NT + b]; });
+  } else {
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    const int64_t g = (int64_t)__ldg(P.bcol + f * kSlots + s) * NT;
+    for (int s = 0; s < kSlots; ++s) {
+      const int64_t g = (int64_t)__ldg(P.bcol + f * kSlots + s) * NT;
 #pragma unroll
-    for (int b = 0; b < NT; ++b)
-      acc = fma((double)__builtin_nontemporal_load(T.vals + (int64_t)(s * NT + b) * T.N + r), (double)x[g + b], acc);
+      for (int b = 0; b < NT; ++b)
+        acc = fma((double)__builtin_nontemporal_load(T.vals + (int64_t)(s * NT + b) * T.N + r), (double)x[g + b],
+                  acc);
+    }
   }
   return T.dmask[r] ? 0.0 : -acc;
 }
@@ -1107,6 +1113,7 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
                       const MgData* dM, const MgVecs<W>& mv, hipStream_t s) {
   const int gr = grid1(T.N, 256);
   const int grows = gr > 16384 ? 16384 : gr;
+  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv, M.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES
@@ -1134,6 +1141,7 @@ int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Kryl
   vu.st = nullptr;
   const int gr = grid1(T.N, 256);
   const int grows = gr > 8192 ? 8192 : gr;
+  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, vu, mv, Mu.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES

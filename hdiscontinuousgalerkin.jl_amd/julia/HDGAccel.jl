@@ -50,6 +50,7 @@ end
 struct Sizes
     n_nodes::Int64; n_cells::Int64; n_faces::Int64; n_dofs::Int64; nnz::Int64
     nu::Int32; nq::Int32; nt::Int32; m::Int32; ntr::Int32; nqp::Int32; nfq::Int32
+    sym_blocks::Int32
 end
 
 mutable struct SolveInfo

@@ -114,6 +114,9 @@ typedef struct {
   int64_t n_dofs;      /* trace dofs = n_faces * nt */
   int64_t nnz;         /* stored entries of K (sparse(I,J,V) semantics) */
   int32_t nu, nq, nt, m, ntr, nqp, nfq;
+  int32_t sym_blocks;  /* 1: the trace product reads K as symmetric face blocks
+                          (nt = 4: K_ff and one owned block per cell and face,
+                          3 nt^2 values per face); 0: the face-ELL (nnz values) */
 } hdg_sizes;
 
 typedef struct {
