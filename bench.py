@@ -54,8 +54,8 @@ STREAM_WORKLOADS = ("batched", "serving")
 
 def spmv_bytes(ctx):
     """Algorithmic bytes of one trace-SpMV launch (FP64): the matrix values
-    once, the gathered input vectors z and p_old read once (16 N), p_new
-    and q written (16 N), the Dirichlet mask (N bytes).  Values: with
+    once, z (gathered), p_old and q_old read once (24 N), p_new and q
+    written (16 N), the Dirichlet mask (N bytes).  Values: with
     symmetric face blocks (k = 3) 3 nt^2 per face and 4 int32 block
     references per face (the transposed reads of a neighbour's block are
     the owner's bytes again); with the face-ELL 8 nnz and 4 bytes of block
@@ -63,7 +63,7 @@ def spmv_bytes(ctx):
     s = ctx.sizes
     nt = s.nt
     N = s.n_dofs
-    vec = 16 * N + 16 * N + N
+    vec = 24 * N + 16 * N + N
     if s.sym_blocks:
         return 8 * 3 * nt * nt * s.n_faces + 16 * s.n_faces + vec
     nblocks = s.nnz // (nt * nt)

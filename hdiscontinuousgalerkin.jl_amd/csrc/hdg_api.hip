@@ -1012,8 +1012,12 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   c->T.nt = c->nt;
   c->T.nf = nf;
   c->T.N = N;
-  TRY(dalloc(c, &c->T.vals, (size_t)kSlots * c->nt * N));
-  if (sym_blocks(c->nt)) TRY(dalloc(c, &c->T.blk, (size_t)3 * c->nt * N));
+  // the trace operator: symmetric blocks at four modes per face, else the
+  // face-ELL (the exports expand the blocks into a face-ELL on demand)
+  if (sym_blocks(c->nt))
+    TRY(dalloc(c, &c->T.blk, (size_t)3 * c->nt * N));
+  else
+    TRY(dalloc(c, &c->T.vals, (size_t)kSlots * c->nt * N));
   TRY(dalloc(c, &c->T.rhs, N));
   TRY(dalloc(c, &c->T.Dinv, (size_t)c->nt * N));
   TRY(dalloc(c, &c->d_dmask, N));
@@ -1314,7 +1318,16 @@ int hdg_get_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, 
   if (!c) return HDG_ERR_ARG;
   if (!c->assembled) return fail(c, HDG_ERR_STATE, "hdg_assemble_condense has not been called");
   TRY(select_device(c));
-  return export_csc(c, c->T.vals, c->T.rhs, colptr, rowval, nzval, rhs);
+  if (!c->T.blk || !nzval) return export_csc(c, c->T.vals, c->T.rhs, colptr, rowval, nzval, rhs);
+  double* vals = nullptr;
+  TRY(dalloc(c, &vals, (size_t)kSlots * c->nt * c->T.N));
+  int r = HDG_OK;
+  const int e = launch_expand_ell(c->pat, c->T, vals, c->stream);
+  if (e) r = fail(c, HDG_ERR_HIP, "expanding the trace blocks: %s", hipGetErrorString((hipError_t)e));
+  if (r == HDG_OK) r = export_csc(c, vals, c->T.rhs, colptr, rowval, nzval, rhs);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(c, vals);
+  return r;
 }
 
 int hdg_get_applied_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double* nzval, double* rhs,
@@ -1328,7 +1341,14 @@ int hdg_get_applied_system(hdg_ctx* c, int64_t* colptr, int64_t* rowval, double*
   if (r == HDG_OK) r = dalloc(c, &b, N);
   if (r == HDG_OK) r = dalloc(c, &part, 1024 + 1);
   if (r == HDG_OK) {
-    const int e = launch_apply(c->pat, c->T, part, part + 1024, vals, b, c->stream);
+    // with symmetric blocks apply! runs in place on their face-ELL expansion
+    TraceSystem T = c->T;
+    int e = 0;
+    if (T.blk) {
+      e = launch_expand_ell(c->pat, T, vals, c->stream);
+      T.vals = vals;
+    }
+    if (!e) e = launch_apply(c->pat, T, part, part + 1024, vals, b, c->stream);
     if (e) r = fail(c, HDG_ERR_HIP, "apply! kernels: %s", hipGetErrorString((hipError_t)e));
   }
   if (r == HDG_OK && meandiag_out) {
@@ -1486,12 +1506,20 @@ int hdg_time_kernel(hdg_ctx* c, int which, int reps, double* avg_us) {
   if (!c->solved) LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   const CondenseArgs a = condense_args(c);
   double* scratch = nullptr;
-  const int64_t ncopy = (int64_t)kSlots * c->nt * c->T.N;  // the size of the trace matrix values
-  if (which == 3) TRY(dalloc(c, &scratch, ncopy));
+  const int64_t ncopy = (int64_t)kSlots * c->nt * c->T.N;  // the size of the face-ELL values
+  // source: the face-ELL values, or (symmetric blocks) a second scratch half
+  const double* csrc = c->T.vals;
+  if (which == 3) {
+    TRY(dalloc(c, &scratch, c->T.vals ? ncopy : 2 * ncopy));
+    if (!csrc) {
+      csrc = scratch + ncopy;
+      HIPCHK(c, hipMemsetAsync(scratch + ncopy, 0, sizeof(double) * ncopy, c->stream));
+    }
+  }
   auto once = [&]() -> int {
     if (which == 0) return launch_cg_spmv(c->pat, op64(c), kv64(c), 2, 0.0, 0, 1, c->stream);
     if (which == 1) return launch_cg_update(c->T, c->cg, 2, 1, c->stream);
-    if (which == 3) return launch_copy(c->T.vals, scratch, ncopy, c->stream);
+    if (which == 3) return launch_copy(csrc, scratch, ncopy, c->stream);
     return launch_condense_cells(c, a);
   };
   int rc = once();  // warm-up
@@ -1516,11 +1544,11 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
   for (int j = 0; j <= outer; ++j) relres[j] = std::nan("");
   if (inner_iters)
     for (int j = 0; j < outer; ++j) inner_iters[j] = 0;
-  const int64_t N = c->T.N, nv = (int64_t)kSlots * c->nt * N;
+  const int64_t N = c->T.N, nv = c->T.vals ? (int64_t)kSlots * c->nt * N : 0;
   const int64_t nb = c->T.blk ? (int64_t)3 * c->nt * N : 0;
   double *vr = nullptr, *vb = nullptr, *u = nullptr, *res = nullptr, *zero = nullptr, *part = nullptr;
   int rc = HDG_OK;
-  if ((rc = dalloc(c, &vr, nv)) == HDG_OK && (rc = dalloc(c, &u, N)) == HDG_OK && (rc = dalloc(c, &res, N)) == HDG_OK &&
+  if ((!nv || (rc = dalloc(c, &vr, nv)) == HDG_OK) && (rc = dalloc(c, &u, N)) == HDG_OK && (rc = dalloc(c, &res, N)) == HDG_OK &&
       (rc = dalloc(c, &zero, N)) == HDG_OK && (!nb || (rc = dalloc(c, &vb, nb)) == HDG_OK))
     rc = dalloc(c, &part, refine_partials() + 1);
   // graphs hold the operator's pointers: none is replayed or kept across the swap
@@ -1532,7 +1560,7 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
   auto step = [&]() -> int {
     HIPCHK(c, hipMemcpyAsync(u, c->T.uD, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(zero, 0, sizeof(double) * N, c->stream));
-    LAUNCHCHK(c, launch_round_float(c->T.vals, vr, nv, c->stream));
+    if (nv) LAUNCHCHK(c, launch_round_float(c->T.vals, vr, nv, c->stream));
     if (nb) LAUNCHCHK(c, launch_round_float(c->T.blk, vb, nb, c->stream));
     double r0 = 0.0;
     for (int j = 0;; ++j) {
@@ -1545,7 +1573,7 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
       relres[j] = r0 > 0.0 ? std::sqrt(r2) / r0 : 0.0;
       if (j == outer) return HDG_OK;
       // K32 d = r_j: rhs r_j, no Dirichlet lift, the rounded values
-      c->T.vals = vr;
+      if (nv) c->T.vals = vr;
       if (nb) c->T.blk = vb;
       c->T.rhs = res;
       c->T.uD = zero;

@@ -379,8 +379,10 @@ int build_group(hdg_batch* b, Group& g) {
   g.T.nt = g.nt;
   g.T.nf = g.nf;
   g.T.N = N;
-  BTRY(galloc(b, g, &g.T.vals, (size_t)kSlots * g.nt * N));
-  if (sym_blocks(g.nt)) BTRY(galloc(b, g, &g.T.blk, (size_t)3 * g.nt * N));
+  if (sym_blocks(g.nt))
+    BTRY(galloc(b, g, &g.T.blk, (size_t)3 * g.nt * N));
+  else
+    BTRY(galloc(b, g, &g.T.vals, (size_t)kSlots * g.nt * N));
   BTRY(galloc(b, g, &g.T.rhs, N));
   BTRY(galloc(b, g, &g.T.Dinv, (size_t)g.nt * N));
   g.T.dmask = g.dmask;

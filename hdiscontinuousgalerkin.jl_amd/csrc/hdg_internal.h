@@ -161,6 +161,14 @@ struct TraceSystem {
   const double* uD = nullptr;            // Dirichlet values (0 elsewhere)
 };
 
+// The trace operator's storage is there for the kernels that read it: the
+// symmetric blocks and their references at four modes, the face-ELL values
+// otherwise (launchers return hipErrorInvalidValue instead of launching).
+template <class TS>
+inline bool trace_op_ready(const DevPattern& P, const TS& T) {
+  return sym_blocks(T.nt) ? (T.blk != nullptr && P.nbr != nullptr) : T.vals != nullptr;
+}
+
 struct CgVectors {
   double *x, *r, *z, *p0, *p1, *q;
   double* part;  // 4 * nblocks partial sums
@@ -209,30 +217,29 @@ struct TraceOp {
                               const unsigned char* dmask_, const W* blk_)
       : nt(nt_), nf(nf_), N(N_), vals(vals_), Dinv(Dinv_), dmask(dmask_), blk(blk_) {}
 };
-// Row a of face f of K x over the symmetric blocks (kSymBlocks<NT>), x given
-// as xv(g, b) = entry b of face g.  The diagonal block and the face's own
-// blocks are read by rows (the diagonal block once per product:
-// nontemporal); the two blocks of the neighbours that own the coupling are
-// read by columns, and their lines are shared with the owners' own reads
-// through the caches.  Entry-wide loads: 16-byte loads of the row pairs
-// made the 1024^2 P3 product slower (441 -> 473 us).
-template <int NT, class V, class X>
-__device__ __forceinline__ double sym_row(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t f, int a,
-                                          const X& xv) {
+// The entries of row a of face f of K over the symmetric blocks
+// (kSymBlocks<NT>): fn(g, b, K[(f, a), (g, b)]) block by block, K_ff, the
+// face's own blocks, then the two blocks of the neighbours that own the
+// coupling.  The diagonal block and the own blocks are read by rows (the
+// diagonal block once per product: nontemporal); the neighbours' blocks
+// by columns, their lines shared with the owners' own reads through the
+// caches.
+template <int NT, class V, class F>
+__device__ __forceinline__ void sym_row_entries(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t f,
+                                                int a, const F& fn) {
   constexpr int B = NT * NT;
   const int4 nb = *reinterpret_cast<const int4*>(nbr + 4 * f);
-  double acc = 0.0;
   {
     const V* d = blk + 3 * f * B + a * NT;
 #pragma unroll
-    for (int b = 0; b < NT; ++b) acc = fma((double)__builtin_nontemporal_load(d + b), xv(f, b), acc);
+    for (int b = 0; b < NT; ++b) fn(f, b, (double)__builtin_nontemporal_load(d + b));
   }
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const V* d = blk + (3 * f + 1 + k) * B + a * NT;
     const int64_t g = k ? nb.y : nb.x;
 #pragma unroll
-    for (int b = 0; b < NT; ++b) acc = fma((double)d[b], xv(g, b), acc);
+    for (int b = 0; b < NT; ++b) fn(g, b, (double)d[b]);
   }
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -240,8 +247,17 @@ __device__ __forceinline__ double sym_row(const int* __restrict__ nbr, const V* 
     const int64_t g = code >> 1;
     const V* d = blk + (3 * g + 1 + (code & 1)) * B + a;
 #pragma unroll
-    for (int b = 0; b < NT; ++b) acc = fma((double)d[b * NT], xv(g, b), acc);
+    for (int b = 0; b < NT; ++b) fn(g, b, (double)d[b * NT]);
   }
+}
+
+// Row a of face f of K x over the symmetric blocks, x given as xv(g, b) =
+// entry b of face g.
+template <int NT, class V, class X>
+__device__ __forceinline__ double sym_row(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t f, int a,
+                                          const X& xv) {
+  double acc = 0.0;
+  sym_row_entries<NT>(nbr, blk, f, a, [&](int64_t g, int b, double v) { acc = fma(v, xv(g, b), acc); });
   return acc;
 }
 
@@ -392,6 +408,9 @@ int tune_condense(int k, const CondenseArgs& a, hipStream_t s, CondenseSched* be
 int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s);
 int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);
+// the face-ELL values (kSlots nt N doubles, TraceSystem::vals layout) of the
+// symmetric blocks, for the exports
+int launch_expand_ell(const DevPattern& P, const TraceSystem& T, double* vals, hipStream_t s);
 int launch_cg_init(const DevPattern& P, const TraceSystem& T, CgVectors& v, hipStream_t s);
 // i_fixed = 0 takes the iteration number from the device state; force = 1
 // (timing only, with i_fixed > 0) runs the kernel regardless of the

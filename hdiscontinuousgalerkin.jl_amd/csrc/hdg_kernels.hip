@@ -527,32 +527,82 @@ __global__ __launch_bounds__(256) void fill_kernel(DevPattern P, const double* _
   }
   const int sf = P.self[f];
   const int c1 = sp[2 * sf], c2 = sp[2 * sf + 1];
-  if constexpr (kSymBlocks<NT>) {
-    // symmetric blocks: K_ff (the self slot's sum, in the same order) and
-    // the block the face owns in each cell (local face la + 1); zero for
-    // the missing cell of a boundary face
-    constexpr int B = NT * NT;
-    const int cl[2] = {c1 / 9, c2 >= 0 ? c2 / 9 : -1};
-    const int ll[2] = {(c1 / 3) % 3, c2 >= 0 ? (c2 / 3) % 3 : 0};
-    double* out = T.blk + 3 * f * B + a * NT;
-#pragma unroll
-    for (int b = 0; b < NT; ++b) {
-      double v = Ate[(int64_t)cl[0] * NTR * NTR + (ll[0] * NT + a) * NTR + ll[0] * NT + b];
-      if (cl[1] >= 0) v = v + Ate[(int64_t)cl[1] * NTR * NTR + (ll[1] * NT + a) * NTR + ll[1] * NT + b];
-      out[b] = v;
-    }
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int lb = (ll[k] + 1) % 3;
-#pragma unroll
-      for (int b = 0; b < NT; ++b)
-        out[(1 + k) * B + b] =
-            cl[k] >= 0 ? Ate[(int64_t)cl[k] * NTR * NTR + (ll[k] * NT + a) * NTR + lb * NT + b] : 0.0;
-    }
-  }
   double rh = bte[(int64_t)(c1 / 9) * NTR + ((c1 / 3) % 3) * NT + a];
   if (c2 >= 0) rh = rh + bte[(int64_t)(c2 / 9) * NTR + ((c2 / 3) % 3) * NT + a];
   T.rhs[r] = rh;
+}
+
+// The symmetric blocks (kSymBlocks<NT>) and the rhs: thread per block row
+// (f, j, a), j = 0 the diagonal block K_ff (the two cells' blocks summed in
+// cell order, as the face-ELL sums them), j = 1 + k the block face f owns in
+// its cell k (local face la + 1; zero for the missing cell of a boundary
+// face).  Consecutive threads write consecutive rows.
+template <int NT>
+__global__ __launch_bounds__(256) void fill_sym_kernel(DevPattern P, const double* __restrict__ Ate,
+                                                       const double* __restrict__ bte, TraceSystem T) {
+  constexpr int NTR = 3 * NT;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * T.N) return;
+  const int64_t f = t / (3 * NT);
+  const int j = (int)(t / NT - 3 * f), a = (int)(t % NT);
+  const int* sp = P.src + (f * kSlots + P.self[f]) * 2;  // cell * 9 + la * 3 + la of the face's cells
+  const int c1 = sp[0], c2 = sp[1];
+  const int64_t cl[2] = {c1 / 9, c2 >= 0 ? c2 / 9 : -1};
+  const int ll[2] = {(c1 / 3) % 3, c2 >= 0 ? (c2 / 3) % 3 : 0};
+  double v[NT];
+  if (j == 0) {
+    const double* b1 = Ate + cl[0] * NTR * NTR + (ll[0] * NT + a) * NTR + ll[0] * NT;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) v[b] = b1[b];
+    if (cl[1] >= 0) {
+      const double* b2 = Ate + cl[1] * NTR * NTR + (ll[1] * NT + a) * NTR + ll[1] * NT;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) v[b] = v[b] + b2[b];
+    }
+    double rh = bte[cl[0] * NTR + ll[0] * NT + a];
+    if (cl[1] >= 0) rh = rh + bte[cl[1] * NTR + ll[1] * NT + a];
+    T.rhs[f * NT + a] = rh;
+  } else {
+    const int k = j - 1;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) v[b] = 0.0;
+    if (cl[k] >= 0) {
+      const double* bk = Ate + cl[k] * NTR * NTR + (ll[k] * NT + a) * NTR + ((ll[k] + 1) % 3) * NT;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) v[b] = bk[b];
+    }
+  }
+  double* out = T.blk + t * NT;
+#pragma unroll
+  for (int b = 0; b < NT; ++b) out[b] = v[b];
+}
+
+// The face-ELL values of the symmetric blocks (kSymBlocks<NT>), for the
+// exports: row (f, a), slot s, entry b = K[(f, a), (bcol, b)]; padding slots
+// zero.
+template <int NT>
+__global__ __launch_bounds__(256) void expand_ell_kernel(DevPattern P, TraceSystem T, double* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= T.N) return;
+  const int64_t f = r / NT;
+  const int a = (int)(r - f * NT);
+  double row[5][NT];
+  int64_t col[5];
+  int e = 0;
+  sym_row_entries<NT>(P.nbr, T.blk, f, a, [&](int64_t g, int b, double v) {
+    row[e / NT][b] = v;
+    col[e / NT] = g;
+    ++e;
+  });
+  const int n = P.nslot[f];
+  for (int s = 0; s < kSlots; ++s) {
+    const int64_t g = P.bcol[f * kSlots + s];
+    int m = -1;  // the block of column face g: K_ff first (a boundary face's missing blocks name f)
+    for (int q = 4; q >= 0; --q)
+      if (col[q] == g) m = q;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) vals[(int64_t)(s * NT + b) * T.N + r] = (s < n && m >= 0) ? row[m][b] : 0.0;
+  }
 }
 
 // Block-Jacobi: inverse of -K_ff restricted to the non-Dirichlet dofs.
@@ -567,12 +617,16 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
   for (int i = 0; i < NT; ++i) dm[i] = T.dmask[f * NT + i] != 0;
   // column j of the face's diagonal block: the NT consecutive rows of
   // slot column (s0, j), read in 16-byte pieces where NT is even (the block
-  // starts at a multiple of 2 doubles)
+  // starts at a multiple of 2 doubles); with symmetric blocks the column of
+  // the row-major K_ff
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
-    const double* col = T.vals + (int64_t)(s0 * NT + j) * T.N + f * NT;
     double cv[NT];
-    if constexpr (NT % 2 == 0) {
+    if constexpr (kSymBlocks<NT>) {
+#pragma unroll
+      for (int i = 0; i < NT; ++i) cv[i] = T.blk[3 * f * NT * NT + i * NT + j];
+    } else if constexpr (NT % 2 == 0) {
+      const double* col = T.vals + (int64_t)(s0 * NT + j) * T.N + f * NT;
 #pragma unroll
       for (int i = 0; i < NT; i += 2) {
         const double2 w = *reinterpret_cast<const double2*>(col + i);
@@ -580,6 +634,7 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
         cv[i + 1] = w.y;
       }
     } else {
+      const double* col = T.vals + (int64_t)(s0 * NT + j) * T.N + f * NT;
 #pragma unroll
       for (int i = 0; i < NT; ++i) cv[i] = col[i];
     }
@@ -835,12 +890,18 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
       } else if (!T.dmask[r]) {
         const int64_t f = r / NT;
         double s = T.rhs[r];
+        if constexpr (kSymBlocks<NT>) {
+          sym_row_entries<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int c, double v) {
+            if (T.dmask[g * NT + c]) s -= v * T.uD[g * NT + c];
+          });
+        } else {
 #pragma unroll
-        for (int sl = 0; sl < kSlots; ++sl) {
-          const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+          for (int sl = 0; sl < kSlots; ++sl) {
+            const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-          for (int c = 0; c < NT; ++c)  // uD is zero off the Dirichlet dofs
-            if (T.dmask[g + c]) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
+            for (int c = 0; c < NT; ++c)  // uD is zero off the Dirichlet dofs
+              if (T.dmask[g + c]) s -= T.vals[(int64_t)(sl * NT + c) * T.N + r] * T.uD[g + c];
+          }
         }
         b = -s;
       }
@@ -925,11 +986,15 @@ __global__ __launch_bounds__(kCgThreads) void true_residual_kernel(DevPattern P,
     if (T.dmask[r]) continue;
     const int64_t f = r / NT;
     double s = -T.rhs[r];
+    if constexpr (kSymBlocks<NT>) {
+      s += sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int c) { return uhat[g * NT + c]; });
+    } else {
 #pragma unroll
-    for (int sl = 0; sl < kSlots; ++sl) {
-      const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+      for (int sl = 0; sl < kSlots; ++sl) {
+        const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-      for (int c = 0; c < NT; ++c) s = fma(T.vals[(int64_t)(sl * NT + c) * T.N + r], uhat[g + c], s);
+        for (int c = 0; c < NT; ++c) s = fma(T.vals[(int64_t)(sl * NT + c) * T.N + r], uhat[g + c], s);
+      }
     }
     acc2 = fma(s, s, acc2);
   }
@@ -1182,7 +1247,11 @@ int condense_native_seg(int k) {
 
 int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s) {
-  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
+  if (sym_blocks(T.nt)) {
+    hipLaunchKernelGGL(fill_sym_kernel<4>, dim3(grid_for(3 * T.N, 256, 1 << 30)), dim3(256), 0, s, P, Ate, bte, T);
+    return (int)hipGetLastError();
+  }
   const int gr = grid_for(T.N, 256, 1 << 30);
 #define FILL(NT) hipLaunchKernelGGL(fill_kernel<NT>, dim3(gr), dim3(256), 0, s, P, Ate, bte, T);
   HDG_NT_DISPATCH(T.nt, FILL)
@@ -1190,7 +1259,14 @@ int launch_fill(const DevPattern& P, const double* Ate, const double* bte, Trace
   return (int)hipGetLastError();
 }
 
+int launch_expand_ell(const DevPattern& P, const TraceSystem& T, double* vals, hipStream_t s) {
+  if (!sym_blocks(T.nt) || !trace_op_ready(P, T) || !vals) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(expand_ell_kernel<4>, dim3(grid_for(T.N, 256, 1 << 30)), dim3(256), 0, s, P, T, vals);
+  return (int)hipGetLastError();
+}
+
 int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s) {
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
   const int gf = grid_for(T.nf, 256, 1 << 30);
 #define JAC(NT) hipLaunchKernelGGL(jacobi_kernel<NT>, dim3(gf), dim3(256), 0, s, P, T);
   HDG_NT_DISPATCH(T.nt, JAC)
@@ -1200,6 +1276,7 @@ int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s) {
 
 template <class G>
 int launch_cg_init_t(const DevPattern& P, const TraceSystem& T, CgVectors& v, G g, int grid, hipStream_t s) {
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
 #define INIT(NT) hipLaunchKernelGGL((cg_init_kernel<NT, G>), dim3(grid), dim3(64 * NT), 0, s, P, T, v, g);
   HDG_NT_DISPATCH(T.nt, INIT)
 #undef INIT
@@ -1209,7 +1286,7 @@ int launch_cg_init_t(const DevPattern& P, const TraceSystem& T, CgVectors& v, G 
 template <class G>
 int launch_cg_spmv_t(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, G g, int grid,
                      int i_fixed, double rtol2, int maxit, int force, hipStream_t s) {
-  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
 #define SPMV(NT)                                                                                          \
   hipLaunchKernelGGL((cg_spmv_kernel<NT, G>), dim3(grid), dim3(kCgThreads), 0, s, P, T, v, g, i_fixed, rtol2, \
                      maxit, force);
@@ -1231,6 +1308,7 @@ int launch_cg_update_t(const TraceSystem& T, CgVectors& v, G g, int grid, int i_
 template <class G>
 int launch_true_residual_t(const DevPattern& P, const TraceSystem& T, const double* uhat, double* part, G g,
                            int grid, hipStream_t s) {
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
 #define TR(NT) \
   hipLaunchKernelGGL((true_residual_kernel<NT, G>), dim3(grid), dim3(kCgThreads), 0, s, P, T, uhat, part, g);
   HDG_NT_DISPATCH(T.nt, TR)

@@ -397,24 +397,30 @@ __global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T,
 #pragma unroll
   for (int j = 0; j < 4; ++j) q[j] = M.fq4[4 * f + j];
   double w[4] = {0.0, 0.0, 0.0, 0.0};
+  // entry S[r, (g, b)] = -kv of a mode b < 2 column
+  auto add = [&](int64_t g, int b, double kv) {
+    if (b >= 2 || T.dmask[g * NT + b]) return;
+    const int lo = M.fv[2 * g], hi = M.fv[2 * g + 1];
+    const double sv = -kv;
+    const double cl = b == 0 ? 0.5 : -kS3, ch = b == 0 ? 0.5 : kS3;
+    bool dl = false, dh = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool ml = !dl && q[j] == lo, mh = !dh && q[j] == hi;
+      w[j] = ml ? fma(sv, cl, w[j]) : w[j];
+      w[j] = mh ? fma(sv, ch, w[j]) : w[j];
+      dl = dl || ml;
+      dh = dh || mh;
+    }
+  };
   if (!T.dmask[r]) {
-    for (int sl = 0; sl < kSlots; ++sl) {
-      const int64_t g = P.bcol[f * kSlots + sl];
-      const int lo = M.fv[2 * g], hi = M.fv[2 * g + 1];
+    if constexpr (kSymBlocks<NT>) {
+      sym_row_entries<NT>(P.nbr, T.blk, f, (int)(r - f * NT), add);
+    } else {
+      for (int sl = 0; sl < kSlots; ++sl) {
+        const int64_t g = P.bcol[f * kSlots + sl];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        if (T.dmask[g * NT + b]) continue;
-        const double sv = -T.vals[(int64_t)(sl * NT + b) * T.N + r];
-        const double cl = b == 0 ? 0.5 : -kS3, ch = b == 0 ? 0.5 : kS3;
-        bool dl = false, dh = false;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const bool ml = !dl && q[j] == lo, mh = !dh && q[j] == hi;
-          w[j] = ml ? fma(sv, cl, w[j]) : w[j];
-          w[j] = mh ? fma(sv, ch, w[j]) : w[j];
-          dl = dl || ml;
-          dh = dh || mh;
-        }
+        for (int b = 0; b < 2; ++b) add(g, b, T.vals[(int64_t)(sl * NT + b) * T.N + r]);
       }
     }
   }
@@ -497,12 +503,18 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
       if (!T64.dmask[r]) {
         const int64_t f = r / NT;
         double s = T64.rhs[r];
+        if constexpr (kSymBlocks<NT>) {
+          sym_row_entries<NT>(P.nbr, T64.blk, f, (int)(r - f * NT), [&](int64_t g, int c, double v) {
+            if (T64.dmask[g * NT + c]) s -= v * T64.uD[g * NT + c];
+          });
+        } else {
 #pragma unroll
-        for (int sl = 0; sl < kSlots; ++sl) {
-          const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+          for (int sl = 0; sl < kSlots; ++sl) {
+            const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-          for (int c = 0; c < NT; ++c)
-            if (T64.dmask[g + c]) s -= T64.vals[(int64_t)(sl * NT + c) * T64.N + r] * T64.uD[g + c];
+            for (int c = 0; c < NT; ++c)
+              if (T64.dmask[g + c]) s -= T64.vals[(int64_t)(sl * NT + c) * T64.N + r] * T64.uD[g + c];
+          }
         }
         b = -s;
       }
@@ -1022,6 +1034,7 @@ int mg_setup_cells(const DevMesh& mesh, const MgData& M, const TraceSystem& T, c
 }
 
 int mg_setup_sp(const MgData& M, const TraceSystem& T, const DevPattern& P, float* sp32, hipStream_t s) {
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
 #define SPK(NT) hipLaunchKernelGGL(mg_sp_kernel<NT>, dim3(grid1(T.N)), dim3(256), 0, s, P, T, M, sp32);
   MG_NT_DISPATCH(T.nt, SPK)
 #undef SPK
@@ -1068,6 +1081,7 @@ int mg_setup_numeric(const DevMesh& mesh, MgData& M, const MgData* dM, TraceSyst
 template <class W, class G>
 int mg_init_t(const DevPattern& P, const TraceSystem& T64, const TraceOp<W>& T, const Krylov<double>& v,
               const MgVecs<W>& mv, G g, int grid, hipStream_t s) {
+  if (!trace_op_ready(P, T64)) return (int)hipErrorInvalidValue;
 #define INIT(NT) \
   hipLaunchKernelGGL((mg_init_kernel<NT, W, G>), dim3(grid), dim3(64 * NT), 0, s, P, T64, T, v, mv, g);
   MG_NT_DISPATCH(T.nt, INIT)
@@ -1113,7 +1127,7 @@ int launch_mg_precond(const DevPattern& P, const TraceOp<W>& T, const Krylov<dou
                       const MgData* dM, const MgVecs<W>& mv, hipStream_t s) {
   const int gr = grid1(T.N, 256);
   const int grows = gr > 16384 ? 16384 : gr;
-  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, v, mv, M.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES
@@ -1141,7 +1155,7 @@ int launch_mg_precond_group(const DevPattern& P, const TraceOp<W>& T, const Kryl
   vu.st = nullptr;
   const int gr = grid1(T.N, 256);
   const int grows = gr > 8192 ? 8192 : gr;
-  if (sym_blocks(T.nt) && (!T.blk || !P.nbr)) return (int)hipErrorInvalidValue;
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
 #define RES(NT) hipLaunchKernelGGL((mg_resid0_kernel<NT, W>), dim3(grows), dim3(256), 0, s, P, T, vu, mv, Mu.fc);
   MG_NT_DISPATCH(T.nt, RES)
 #undef RES

@@ -42,11 +42,15 @@ __global__ __launch_bounds__(256) void refine_resid_kernel(DevPattern P, TraceSy
     if (!T.dmask[r]) {
       const int64_t f = r / NT;
       s = T.rhs[r];
+      if constexpr (kSymBlocks<NT>) {
+        s -= sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int c) { return u[g * NT + c]; });
+      } else {
 #pragma unroll
-      for (int sl = 0; sl < kSlots; ++sl) {
-        const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
+        for (int sl = 0; sl < kSlots; ++sl) {
+          const int64_t g = (int64_t)P.bcol[f * kSlots + sl] * NT;
 #pragma unroll
-        for (int c = 0; c < NT; ++c) s = fma(-T.vals[(int64_t)(sl * NT + c) * T.N + r], u[g + c], s);
+          for (int c = 0; c < NT; ++c) s = fma(-T.vals[(int64_t)(sl * NT + c) * T.N + r], u[g + c], s);
+        }
       }
     }
     res[r] = s;
@@ -88,6 +92,7 @@ int launch_round_float(const double* src, double* dst, int64_t n, hipStream_t s)
 
 int launch_refine_residual(const DevPattern& P, const TraceSystem& T, const double* u, double* res, double* part,
                            double* out2, hipStream_t s) {
+  if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
   switch (T.nt) {
 #define RES(NT)                                                                                                  \
   case NT:                                                                                                       \
