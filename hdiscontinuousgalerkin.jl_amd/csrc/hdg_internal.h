@@ -335,9 +335,12 @@ struct MgData {
   int lanczos_steps = 6;
 };
 // per-order choices of the V-cycle, shared by single problems and batches:
-// V(1,1) on the vertex levels for k <= 2, V(2,2) above (fewest CG
-// iterations x cost per iteration measured on 256..1024 meshes)
-inline void mg_order_choices(int k, MgData& M) { M.vsweeps = k <= 2 ? 1 : 2; }
+// V(1,1) on the vertex levels for k <= 3, V(2,2) at k = 4 (fewest CG
+// iterations x cost per iteration measured on 256..1536 meshes: at k = 3
+// V(1,1) takes one iteration more and two launches fewer per level, 256^2
+// 6.40 -> 5.90 ms, 512^2 16.8 -> 16.0, 1024^2 58.6 -> 57.9, 1536^2 even;
+// at k = 4 1024^2 loses 147 -> 150 ms)
+inline void mg_order_choices(int k, MgData& M) { M.vsweeps = k <= 3 ? 1 : 2; }
 
 // A fine level, the next coarser one and the CG state of one instance: the
 // unit of a vertex-level kernel.  VOne launches one instance over the whole
