@@ -506,7 +506,7 @@ int assemble_impl(hdg_ctx* c, bool mg_early = false) {
     HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
     LAUNCHCHK(c, mg_setup_cells(c->mesh, c->mg, c->T, c->d_Ate, c->aux));
   }
-  LAUNCHCHK(c, launch_fill(c->pat, c->d_Ate, c->d_bte, c->T, c->stream));
+  LAUNCHCHK(c, launch_fill(c->mesh, c->pat, c->d_Ate, c->d_bte, c->T, c->stream));
   if (early) {
     HIPCHK(c, hipEventRecord(c->ev_fill, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fill, 0));

@@ -612,7 +612,7 @@ int maxit_of(const Group& g) { return g.opt.maxit > 0 ? g.opt.maxit : 200000; }
 int enqueue_setup(hdg_batch* b, Group& g) {
   BHIP(b, hipMemsetAsync(g.err, 0, sizeof(int) * g.in.size(), g.s));
   BLAUNCH(b, launch_group_condense(g, group_condense_args(g)));
-  BLAUNCH(b, launch_fill(g.pat, g.Ate, g.bte, g.T, g.s));
+  BLAUNCH(b, launch_fill(g.mesh, g.pat, g.Ate, g.bte, g.T, g.s));
   BLAUNCH(b, launch_jacobi(g.pat, g.T, g.s));
   const Grouped gr = rows(g);
   const int n = (int)g.in.size();

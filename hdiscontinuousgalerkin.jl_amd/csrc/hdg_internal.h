@@ -408,7 +408,7 @@ int tune_condense_spd(int k, const CondenseArgs& a, hipStream_t s, CondenseSched
 // outputs of a.
 int tune_condense(int k, const CondenseArgs& a, hipStream_t s, CondenseSched* best, double* best_us,
                   double* default_us);
-int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
+int launch_fill(const DevMesh& m, const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s);
 int launch_jacobi(const DevPattern& P, TraceSystem& T, hipStream_t s);
 // the face-ELL values (kSlots nt N doubles, TraceSystem::vals layout) of the

@@ -532,49 +532,55 @@ __global__ __launch_bounds__(256) void fill_kernel(DevPattern P, const double* _
   T.rhs[r] = rh;
 }
 
-// The symmetric blocks (kSymBlocks<NT>) and the rhs: thread per block row
-// (f, j, a), j = 0 the diagonal block K_ff (the two cells' blocks summed in
-// cell order, as the face-ELL sums them), j = 1 + k the block face f owns in
-// its cell k (local face la + 1; zero for the missing cell of a boundary
-// face).  Consecutive threads write consecutive rows.
+// The symmetric blocks (kSymBlocks<NT>) and the rhs, cell by cell: thread
+// per (cell c, local face l, j, row a), so that a cell's threads read its
+// element block Ate together.  j = 1: the block face f = cf[3c + l] owns in
+// c (coupling to local face l + 1), into block 1 + k of f, k the side of c
+// in fcell.  j = 0, by the cell on side 0 only: K_ff as the two cells'
+// diagonal blocks summed in cell order (as the face-ELL sums them), the rhs
+// entry, and a zero block 2 where f has one cell.
 template <int NT>
-__global__ __launch_bounds__(256) void fill_sym_kernel(DevPattern P, const double* __restrict__ Ate,
+__global__ __launch_bounds__(256) void fill_sym_kernel(DevMesh m, const double* __restrict__ Ate,
                                                        const double* __restrict__ bte, TraceSystem T) {
-  constexpr int NTR = 3 * NT;
+  constexpr int NTR = 3 * NT, B = NT * NT;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 3 * T.N) return;
-  const int64_t f = t / (3 * NT);
-  const int j = (int)(t / NT - 3 * f), a = (int)(t % NT);
-  const int* sp = P.src + (f * kSlots + P.self[f]) * 2;  // cell * 9 + la * 3 + la of the face's cells
-  const int c1 = sp[0], c2 = sp[1];
-  const int64_t cl[2] = {c1 / 9, c2 >= 0 ? c2 / 9 : -1};
-  const int ll[2] = {(c1 / 3) % 3, c2 >= 0 ? (c2 / 3) % 3 : 0};
+  if (t >= m.nc * 3 * 2 * NT) return;
+  const int64_t c = t / (6 * NT);
+  const int rem = (int)(t - c * 6 * NT);
+  const int l = rem / (2 * NT), j = (rem / NT) & 1, a = rem % NT;
+  const int64_t f = m.cf[3 * c + l];
+  const int k = m.fcell[2 * f] == c ? 0 : 1;
+  const double* row = Ate + c * NTR * NTR + (l * NT + a) * NTR;
   double v[NT];
-  if (j == 0) {
-    const double* b1 = Ate + cl[0] * NTR * NTR + (ll[0] * NT + a) * NTR + ll[0] * NT;
+  if (j == 1) {
 #pragma unroll
-    for (int b = 0; b < NT; ++b) v[b] = b1[b];
-    if (cl[1] >= 0) {
-      const double* b2 = Ate + cl[1] * NTR * NTR + (ll[1] * NT + a) * NTR + ll[1] * NT;
+    for (int b = 0; b < NT; ++b) v[b] = row[((l + 1) % 3) * NT + b];
+    double* out = T.blk + (3 * f + 1 + k) * B + a * NT;
 #pragma unroll
-      for (int b = 0; b < NT; ++b) v[b] = v[b] + b2[b];
-    }
-    double rh = bte[cl[0] * NTR + ll[0] * NT + a];
-    if (cl[1] >= 0) rh = rh + bte[cl[1] * NTR + ll[1] * NT + a];
-    T.rhs[f * NT + a] = rh;
-  } else {
-    const int k = j - 1;
-#pragma unroll
-    for (int b = 0; b < NT; ++b) v[b] = 0.0;
-    if (cl[k] >= 0) {
-      const double* bk = Ate + cl[k] * NTR * NTR + (ll[k] * NT + a) * NTR + ((ll[k] + 1) % 3) * NT;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) v[b] = bk[b];
-    }
+    for (int b = 0; b < NT; ++b) out[b] = v[b];
+    return;
   }
-  double* out = T.blk + t * NT;
+  if (k != 0) return;
+#pragma unroll
+  for (int b = 0; b < NT; ++b) v[b] = row[l * NT + b];
+  double rh = bte[c * NTR + l * NT + a];
+  const int64_t c2 = m.fcell[2 * f + 1];
+  double* out = T.blk + 3 * f * B + a * NT;
+  if (c2 >= 0) {
+    int l2 = 0;
+    for (int q = 1; q < 3; ++q)
+      if (m.cf[3 * c2 + q] == f) l2 = q;
+    const double* r2 = Ate + c2 * NTR * NTR + (l2 * NT + a) * NTR + l2 * NT;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) v[b] = v[b] + r2[b];
+    rh = rh + bte[c2 * NTR + l2 * NT + a];
+  } else {
+#pragma unroll
+    for (int b = 0; b < NT; ++b) out[2 * B + b] = 0.0;
+  }
 #pragma unroll
   for (int b = 0; b < NT; ++b) out[b] = v[b];
+  T.rhs[f * NT + a] = rh;
 }
 
 // The face-ELL values of the symmetric blocks (kSymBlocks<NT>), for the
@@ -1245,11 +1251,12 @@ int condense_native_seg(int k) {
     default: return (int)hipErrorInvalidValue; \
   }
 
-int launch_fill(const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
+int launch_fill(const DevMesh& m, const DevPattern& P, const double* Ate, const double* bte, TraceSystem& T,
                 hipStream_t s) {
   if (!trace_op_ready(P, T)) return (int)hipErrorInvalidValue;
   if (sym_blocks(T.nt)) {
-    hipLaunchKernelGGL(fill_sym_kernel<4>, dim3(grid_for(3 * T.N, 256, 1 << 30)), dim3(256), 0, s, P, Ate, bte, T);
+    hipLaunchKernelGGL(fill_sym_kernel<4>, dim3(grid_for(m.nc * 6 * 4, 256, 1 << 30)), dim3(256), 0, s, m, Ate, bte,
+                       T);
     return (int)hipGetLastError();
   }
   const int gr = grid_for(T.N, 256, 1 << 30);
