@@ -251,6 +251,33 @@ __device__ __forceinline__ void sym_row_entries(const int* __restrict__ nbr, con
   }
 }
 
+// The Dirichlet lift of row a of face f over the symmetric blocks:
+// sum over the prescribed columns (g, c) of K[(f, a), (g, c)] uD[(g, c)], in
+// sym_row_entries' order; a block's values are read only for prescribed
+// columns, so rows away from the boundary read none.
+template <int NT>
+__device__ __forceinline__ double sym_row_lift(const int* __restrict__ nbr, const double* __restrict__ blk,
+                                               const unsigned char* __restrict__ dmask,
+                                               const double* __restrict__ uD, int64_t f, int a) {
+  constexpr int B = NT * NT;
+  const int4 nb = *reinterpret_cast<const int4*>(nbr + 4 * f);
+  const int64_t gs[5] = {f, nb.x, nb.y, nb.z >> 1, nb.w >> 1};
+  const int64_t base[5] = {3 * f * B + a * NT, (3 * f + 1) * B + a * NT, (3 * f + 2) * B + a * NT,
+                           (3 * (int64_t)(nb.z >> 1) + 1 + (nb.z & 1)) * B + a,
+                           (3 * (int64_t)(nb.w >> 1) + 1 + (nb.w & 1)) * B + a};
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int stride = j < 3 ? 1 : NT;
+#pragma unroll
+    for (int c = 0; c < NT; ++c) {
+      const int64_t col = gs[j] * NT + c;
+      if (dmask[col]) s = fma(blk[base[j] + c * stride], uD[col], s);
+    }
+  }
+  return s;
+}
+
 // Row a of face f of K x over the symmetric blocks, x given as xv(g, b) =
 // entry b of face g.
 template <int NT, class V, class X>

@@ -897,9 +897,7 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
         const int64_t f = r / NT;
         double s = T.rhs[r];
         if constexpr (kSymBlocks<NT>) {
-          sym_row_entries<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int c, double v) {
-            if (T.dmask[g * NT + c]) s -= v * T.uD[g * NT + c];
-          });
+          s -= sym_row_lift<NT>(P.nbr, T.blk, T.dmask, T.uD, f, (int)(r - f * NT));
         } else {
 #pragma unroll
           for (int sl = 0; sl < kSlots; ++sl) {

@@ -504,9 +504,7 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
         const int64_t f = r / NT;
         double s = T64.rhs[r];
         if constexpr (kSymBlocks<NT>) {
-          sym_row_entries<NT>(P.nbr, T64.blk, f, (int)(r - f * NT), [&](int64_t g, int c, double v) {
-            if (T64.dmask[g * NT + c]) s -= v * T64.uD[g * NT + c];
-          });
+          s -= sym_row_lift<NT>(P.nbr, T64.blk, T64.dmask, T64.uD, f, (int)(r - f * NT));
         } else {
 #pragma unroll
           for (int sl = 0; sl < kSlots; ++sl) {
