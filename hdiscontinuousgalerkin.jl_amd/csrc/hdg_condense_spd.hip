@@ -72,7 +72,7 @@ template <int K, int WPE, bool GROUPED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 void condense_spd_kernel(CondenseArgs a) {
   using D = SpdDims<K>;
-  constexpr int NU = D::NU, NQ = D::NQ, M = D::M, NT = D::NT, NTR = D::NTR, NL = D::NL;
+  constexpr int NU = D::NU, M = D::M, NT = D::NT, NTR = D::NTR, NL = D::NL;
   static_assert(NTR + 1 <= kSeg, "segment too small");
   extern __shared__ double sh[];
   const SpdLayoutDev L = a.spd;

@@ -97,7 +97,6 @@ __global__ void apply_kernel(DevPattern P, TraceSystem T, const double* md, doub
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= N) return;
   const int64_t f = r / nt;
-  const int a = (int)(r - f * nt);
   const double mdiag = *md;
   const bool dr = T.dmask[r] != 0;
   double rhs = T.rhs[r];
