@@ -603,7 +603,9 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgD
 
 // z = e0 + P0 e1 + omega Dinv (res0 - (S P0) e1): the post-smoothing step.
 // S (e0 + P0 e1) = S e0 + (S P0) e1 and res0 = r - S e0, so the trace matrix
-// is not read again; partial (r, z) for the next iteration.
+// is not read again; and e0 = omega Dinv r, so z = P0 e1 + omega Dinv (r +
+// res0 - (S P0) e1) and e0 is not read either (r is, for the partial (r, z)
+// of the next iteration).
 template <int NT, class W, class G>
 __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylov<double> v, MgData M,
                                                              MgVecs<W> mv, G gsel) {
@@ -621,10 +623,11 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
   double prz = 0.0;
   for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
-    double rr = 0.0, pe = 0.0;
+    double rv = 0.0, rr = 0.0, pe = 0.0;
+    if (r < S.hi) rv = v.r[r];
     if (r < S.hi && !T.dmask[r]) {
       const int64_t f = r / NT;
-      double s = (double)mv.res0[r];
+      double s = rv + (double)mv.res0[r];
 #pragma unroll
       for (int j = 0; j < 4; ++j) s = fma(-(double)mv.sp[(int64_t)j * T.N + r], e1[M.fq4[4 * f + j]], s);
       rr = s;
@@ -641,9 +644,9 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
       double z = 0.0;
 #pragma unroll
       for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
-      const double zv = fma(om, z, (double)mv.e0[r] + pe);
+      const double zv = fma(om, z, pe);
       v.z[r] = zv;
-      prz = fma(v.r[r], zv, prz);
+      prz = fma(rv, zv, prz);
     }
     __syncthreads();
   }
