@@ -56,3 +56,28 @@ def test_symmetric_block_product(hdg, case):
     # sees the same solution to the FP64 floor of ||K|| ||uhat|| eps
     assert rr < 1e-11
     assert info_rel < 1e-11
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (2, 1), (1, 3), (3, 2)])
+def test_symmetric_blocks_small_meshes(hdg, O, nx, ny):
+    """Meshes where most faces lie on the boundary (their second block and
+    neighbour are the zero placeholders): the operators, the exported K and
+    the solution (block-Jacobi CG: the product is the same one) against the
+    oracle's direct solve to 1e-12."""
+    print(f"[symmetric blocks] {nx}x{ny}", flush=True)
+    m = hdg.rectangle_mesh((nx, ny), jitter=0.1, seed=3)
+    om = O.rectangle_mesh(nx, ny, jitter=0.1, seed=3)
+    P = O.problem(k=3, qdeg=6)
+    r = O.solve_pipeline(P, om)
+    with hdg.Context(0) as ctx:
+        ctx.set_mesh(m)
+        ctx.setup(3, quad_degree=6, rtol=1e-15, maxit=20000, precond=hdg.PC_BLOCK_JACOBI)
+        ctx.set_source()
+        ctx.set_dirichlet(r["dofs"], r["vals"])
+        ctx.run()
+        uhat = ctx.trace()
+        colptr, rowval, nzval, rhs = ctx.get_system()
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    assert np.array_equal(colptr, r["K"].colptr) and np.array_equal(rowval, r["K"].rowval)
+    assert rel(nzval, r["K"].nzval) < 1e-12 and rel(rhs, r["rhs"]) < 1e-12
+    assert rel(uhat, r["uhat"]) < 1e-12
