@@ -1,8 +1,9 @@
 """GPU parity at the benchmark's sizes and at the direct-solve tolerance.
 
-- 512 x 512 cells (524,288 triangles), P2, jittered from seed 20261202 (the
-  FP64 drop-in configuration): mesh, face numbering and the CSC pattern of
-  K bit-exact against the oracle; Ate, bte, K_e, b_e of every cell, the
+- 512 x 512 cells (524,288 triangles), P2, and 1024 x 1024 cells
+  (2,097,152 triangles), P3, jittered from seed 20261202 (the two benchmark
+  workloads): mesh, face numbering and the CSC pattern of K bit-exact
+  against the oracle; Ate, bte, K_e, b_e of every cell, the
   assembled values and the rhs to 1e-12 relative (per cell: the largest
   deviation over the largest entry of the oracle's block).
 - Trace solutions against the oracle's direct (banded LU) solve to 1e-12
@@ -44,9 +45,16 @@ def ctx(hdg):
     c.close()
 
 
-def test_full_c2_mesh_pattern_and_operators(hdg, O, ctx):
-    n, k = 512, 2
-    print(f"[full C2] {n}x{n} P{k} jitter 0.2 seed {SEED}", flush=True)
+@pytest.mark.parametrize("n,k,qdeg,nnz", [
+    (512, 2, 3, 35_398_656),
+    # the oracle's condensation and assembly at this size take 20-45 s of CPU time
+    pytest.param(1024, 3, 6, 251_691_008, marks=pytest.mark.timeout(600)),
+])
+def test_full_mesh_pattern_and_operators(hdg, O, ctx, n, k, qdeg, nnz):
+    """At k = 3 the device keeps symmetric face blocks (each face's K_ff and
+    the blocks it owns): the exported K is expanded from them, so its values
+    check the block fill as well as the condensation."""
+    print(f"[full] {n}x{n} P{k} quad_degree {qdeg} jitter 0.2 seed {SEED}", flush=True)
     m = hdg.rectangle_mesh((n, n), jitter=0.2, seed=SEED)
     om = O.rectangle_mesh(n, n, jitter=0.2, seed=SEED)
     assert np.array_equal(m.cells, om.cells)
@@ -54,12 +62,13 @@ def test_full_c2_mesh_pattern_and_operators(hdg, O, ctx):
     assert np.array_equal(m.nodes.view(np.uint64), om.xy.view(np.uint64))
     assert m.facesets == om.facesets
     ctx.set_mesh(m)
-    ctx.setup(k, quad_degree=3, rtol=1e-12)
+    ctx.setup(k, quad_degree=qdeg, rtol=1e-12)
+    assert ctx.sizes.sym_blocks == (k == 3)
     ctx.set_source()
     d = m.boundary_dofs(k + 1)
     ctx.set_dirichlet(d, np.zeros(len(d)))
     ctx.assemble_condense()
-    P = O.problem(k=k, qdeg=3)
+    P = O.problem(k=k, qdeg=qdeg)
     oA, ob, oK, obe = O.condense(P, om, 0)
     Ate, bte = ctx.get_condensed()
     assert per_cell_rel(Ate, oA) < 1e-12
@@ -76,10 +85,11 @@ def test_full_c2_mesh_pattern_and_operators(hdg, O, ctx):
     assert np.array_equal(rowval, K.rowval)
     assert rel(nzval, K.nzval) < 1e-12
     assert rel(rhs, orhs) < 1e-12
-    assert ctx.sizes.nnz == 35_398_656 == len(K.nzval)
+    assert ctx.sizes.nnz == nnz == len(K.nzval)
+    del K, orhs
     # the device generator gives the same pattern (and solution) bit for bit
     ctx.set_rectangle_mesh(n, n, jitter=0.2, seed=SEED)
-    ctx.setup(k, quad_degree=3, rtol=1e-12)
+    ctx.setup(k, quad_degree=qdeg, rtol=1e-12)
     ctx.set_source()
     ctx.set_dirichlet_boundary(0.0)
     ctx.assemble_condense()
