@@ -617,7 +617,7 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
         LAUNCHCHK(c, launch_to_float(c->T.blk, c->f_blk, (int64_t)3 * c->nt * N, c->stream));
       else
         LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
-      LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)c->nt * N, c->stream));
+      LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)dinv_count(c->nt, c->mesh.nf), c->stream));
     }
     // The multigrid set-up (small, latency-bound kernels writing the level
     // arrays and S P0) and the Lanczos estimate of the smoothing weight
@@ -1019,7 +1019,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   else
     TRY(dalloc(c, &c->T.vals, (size_t)kSlots * c->nt * N));
   TRY(dalloc(c, &c->T.rhs, N));
-  TRY(dalloc(c, &c->T.Dinv, (size_t)c->nt * N));
+  TRY(dalloc(c, &c->T.Dinv, dinv_count(c->nt, c->mesh.nf)));
   TRY(dalloc(c, &c->d_dmask, N));
   TRY(dalloc(c, &c->d_uD, N));
   HIPCHK(c, hipMemsetAsync(c->d_dmask, 0, N, c->stream));
@@ -1050,7 +1050,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
       TRY(dalloc(c, &c->f_blk, (size_t)3 * c->nt * N));
     else
       TRY(dalloc(c, &c->f_vals, (size_t)kSlots * c->nt * N));
-    TRY(dalloc(c, &c->f_Dinv, (size_t)c->nt * N));
+    TRY(dalloc(c, &c->f_Dinv, dinv_count(c->nt, c->mesh.nf)));
     TRY(dalloc(c, &c->f_sp, 4 * N));
     TRY(dalloc(c, &c->f_e0, N));
     TRY(dalloc(c, &c->f_res0, N));

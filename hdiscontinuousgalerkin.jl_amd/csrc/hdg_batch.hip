@@ -384,7 +384,7 @@ int build_group(hdg_batch* b, Group& g) {
   else
     BTRY(galloc(b, g, &g.T.vals, (size_t)kSlots * g.nt * N));
   BTRY(galloc(b, g, &g.T.rhs, N));
-  BTRY(galloc(b, g, &g.T.Dinv, (size_t)g.nt * N));
+  BTRY(galloc(b, g, &g.T.Dinv, dinv_count(g.nt, g.nf)));
   g.T.dmask = g.dmask;
   g.T.uD = g.uD;
 
@@ -487,7 +487,7 @@ int build_group(hdg_batch* b, Group& g) {
       BTRY(galloc(b, g, &g.f_blk, (size_t)3 * g.nt * N));
     else
       BTRY(galloc(b, g, &g.f_vals, (size_t)kSlots * g.nt * N));
-    BTRY(galloc(b, g, &g.f_Dinv, (size_t)g.nt * N));
+    BTRY(galloc(b, g, &g.f_Dinv, dinv_count(g.nt, g.nf)));
     BTRY(galloc(b, g, &g.f_sp, 4 * N));
     BTRY(galloc(b, g, &g.f_e0, N));
     BTRY(galloc(b, g, &g.f_res0, N));
@@ -626,7 +626,7 @@ int enqueue_setup(hdg_batch* b, Group& g) {
       BLAUNCH(b, launch_to_float(g.T.blk, g.f_blk, (int64_t)3 * g.nt * g.N, g.s));
     else
       BLAUNCH(b, launch_to_float(g.T.vals, g.f_vals, (int64_t)kSlots * g.nt * g.N, g.s));
-    BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)g.nt * g.N, g.s));
+    BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)dinv_count(g.nt, g.nf), g.s));
     BLAUNCH(b, mg_setup_mesh(g.mesh, g.Mu, g.T, g.pat, g.Ate, g.f_sp, g.s));
     BLAUNCH(b, mg_setup_levels_group(g.S, g.s));
     BLAUNCH(b, launch_lanczos_omega(g.pat, g.T, g.cg, gr, g.nbt, g.segs, n, factor, g.Mi[0].lanczos_steps, g.s));

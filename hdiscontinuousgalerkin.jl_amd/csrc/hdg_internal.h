@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <utility>
 
 namespace hdg {
 
@@ -156,10 +157,74 @@ struct TraceSystem {
   // cell k (row a, column b; DevPattern::nbr)
   double* blk = nullptr;
   double* rhs = nullptr;   // assembled rhs (N)
-  double* Dinv = nullptr;  // block-Jacobi inverse: Dinv[b*N + row] (0 on Dirichlet dofs)
+  double* Dinv = nullptr;  // block-Jacobi inverse (dinv_count; 0 on Dirichlet dofs)
   const unsigned char* dmask = nullptr;  // 1 = Dirichlet dof
   const double* uD = nullptr;            // Dirichlet values (0 elsewhere)
 };
+
+// The block-Jacobi inverse of a face is symmetric (K_ff is, and the Dirichlet
+// rows and columns are masked alike).  Where the NT rows of a face fill whole
+// lane quads (NT = 1, 2, 4: rows f NT .. f NT + NT - 1 in consecutive lanes of
+// one wavefront) only its upper triangle is kept, entry (a, b) of face f at
+// Dinv[dinv_slot(nt, a, b) * nf + f] (nt (nt + 1) / 2 per face), and the rows
+// of a face trade their operands by DPP; at NT = 3 and 5 the whole block is
+// kept by columns, Dinv[b N + r], and the operands go through LDS (the
+// triangle's reads there touch more cache lines per load than they save).
+template <int NT>
+constexpr bool kQuadRows = NT == 1 || NT == 2 || NT == 4;
+inline bool quad_rows(int nt) { return nt == 1 || nt == 2 || nt == 4; }
+__host__ __device__ constexpr int dinv_slot(int nt, int a, int b) {
+  return a <= b ? a * nt - a * (a - 1) / 2 + (b - a) : b * nt - b * (b - 1) / 2 + (a - b);
+}
+inline size_t dinv_count(int nt, int64_t nf) {
+  return (size_t)(quad_rows(nt) ? nt * (nt + 1) / 2 : nt * nt) * (size_t)nf;
+}
+// DPP quad_perm control that gives lane j of every quad the value of lane
+// (j & ~(NT - 1)) + B: operand b of the caller's face
+template <int NT, int B>
+__host__ __device__ constexpr int quad_ctrl() {
+  int c = 0;
+  for (int j = 0; j < 4; ++j) c |= ((j & ~(NT - 1)) + B) << (2 * j);
+  return c;
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_quad(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int NT, class W, int... B>
+__device__ __forceinline__ double dinv_lanes(const W* __restrict__ D, int64_t nf, int64_t r, double x,
+                                             std::integer_sequence<int, B...>) {
+  const int64_t f = r / NT;
+  const int a = (int)(r - f * NT);
+  double z = 0.0;
+  ((z = fma((double)D[(int64_t)dinv_slot(NT, a, B) * nf + f], dpp_quad<quad_ctrl<NT, B>()>(x), z)), ...);
+  return z;
+}
+// fn(sum_b Dinv_f[a][b] x_(f, b)) on the rows with `in` set, x one value per
+// row r = f NT + a (one row per thread, a block's rows consecutive from a
+// multiple of NT).  Every thread of the block calls it (the LDS form has
+// barriers; rs holds blockDim doubles) and the rows of a face share `in`.
+template <int NT, class W, class F>
+__device__ __forceinline__ void jacobi_apply(const W* __restrict__ D, int64_t nf, double* rs, double x, int64_t r,
+                                             bool in, F&& fn) {
+  if constexpr (kQuadRows<NT>) {
+    if (in) fn(dinv_lanes<NT>(D, nf, r, x, std::make_integer_sequence<int, NT>{}));
+  } else {
+    rs[threadIdx.x] = x;
+    __syncthreads();
+    if (in) {
+      const double* xs = rs + (threadIdx.x / NT) * NT;
+      const int64_t N = nf * NT;
+      double z = 0.0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) z = fma((double)D[(int64_t)b * N + r], xs[b], z);
+      fn(z);
+    }
+    __syncthreads();
+  }
+}
 
 // The trace operator's storage is there for the kernels that read it: the
 // symmetric blocks and their references at four modes, the face-ELL values
@@ -208,7 +273,7 @@ struct TraceOp {
   int nt;
   int64_t nf, N;
   const W* vals;   // face-ELL, vals[(s*nt + b)*N + row]
-  const W* Dinv;   // Dinv[b*N + row]
+  const W* Dinv;   // TraceSystem::Dinv
   const unsigned char* dmask;
   const W* blk;    // symmetric blocks (TraceSystem::blk) where kSymBlocks<nt>
   // every field named at every construction: an operator without its

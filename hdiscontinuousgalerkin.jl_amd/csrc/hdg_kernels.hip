@@ -679,15 +679,18 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
         for (int j = 0; j < 2 * NT; ++j) a[i][j] -= l * a[c][j];
       }
   }
+  if constexpr (kQuadRows<NT>) {
+    // the upper triangle of the mean of the inverse and its transpose
 #pragma unroll
-  for (int j = 0; j < NT; ++j) {
-    double* out = T.Dinv + (int64_t)j * T.N + f * NT;
-    if constexpr (NT % 2 == 0) {
+    for (int i = 0; i < NT; ++i)
 #pragma unroll
-      for (int i = 0; i < NT; i += 2)
-        *reinterpret_cast<double2*>(out + i) = make_double2((dm[i] || dm[j]) ? 0.0 : a[i][NT + j],
-                                                             (dm[i + 1] || dm[j]) ? 0.0 : a[i + 1][NT + j]);
-    } else {
+      for (int j = i; j < NT; ++j)
+        T.Dinv[(int64_t)dinv_slot(NT, i, j) * T.nf + f] =
+            (dm[i] || dm[j]) ? 0.0 : 0.5 * (a[i][NT + j] + a[j][NT + i]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      double* out = T.Dinv + (int64_t)j * T.N + f * NT;
 #pragma unroll
       for (int i = 0; i < NT; ++i) out[i] = (dm[i] || dm[j]) ? 0.0 : a[i][NT + j];
     }
@@ -845,18 +848,11 @@ __global__ __launch_bounds__(64 * NT) void cg_update_kernel(TraceSystem T, CgVec
         v.r[r] = rn;
       }
     }
-    rs[threadIdx.x] = rn;
-    __syncthreads();
-    if (r < S.hi) {
-      const int lr = (threadIdx.x / NT) * NT;
-      double zr = 0.0;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) zr = fma(T.Dinv[(int64_t)b * T.N + r], rs[lr + b], zr);
+    jacobi_apply<NT>(T.Dinv, T.nf, rs, rn, r, r < S.hi, [&](double zr) {
       v.z[r] = zr;
       prz = fma(rn, zr, prz);
       prr = fma(rn, rn, prr);
-    }
-    __syncthreads();
+    });
   }
   const double a = block_sum<NT>(prz, sh);
   const double b = block_sum<NT>(prr, sh);
@@ -914,18 +910,11 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
       v.p0[r] = 0.0;
       v.p1[r] = 0.0;
     }
-    rs[threadIdx.x] = b;
-    __syncthreads();
-    if (r < S.hi) {
-      const int lr = (threadIdx.x / NT) * NT;
-      double zr = 0.0;
-#pragma unroll
-      for (int c = 0; c < NT; ++c) zr = fma(T.Dinv[(int64_t)c * T.N + r], rs[lr + c], zr);
+    jacobi_apply<NT>(T.Dinv, T.nf, rs, b, r, r < S.hi, [&](double zr) {
       v.z[r] = zr;
       prz = fma(b, zr, prz);
       prr = fma(b, b, prr);
-    }
-    __syncthreads();
+    });
   }
   const double a = block_sum<NT>(prz, sh);
   const double bb = block_sum<NT>(prr, sh);

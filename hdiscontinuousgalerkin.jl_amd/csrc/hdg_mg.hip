@@ -468,16 +468,7 @@ __global__ __launch_bounds__(64 * NT) void mg_update_kernel(TraceOp<W> T, Krylov
       v.r[r] = rn;
       prr = fma(rn, rn, prr);
     }
-    rs[threadIdx.x] = rn;
-    __syncthreads();
-    if (r < S.hi) {
-      const int lr = (threadIdx.x / NT) * NT;
-      double z = 0.0;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
-      mv.e0[r] = (W)(om * z);
-    }
-    __syncthreads();
+    jacobi_apply<NT>(T.Dinv, T.nf, rs, rn, r, r < S.hi, [&](double z) { mv.e0[r] = (W)(om * z); });
   }
   const double b = block_sum_mg<NT>(prr, sh);
   if (threadIdx.x == 0) v.part[nbt * 3 + blockIdx.x] = b;
@@ -522,16 +513,7 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
       v.p1[r] = 0.0;
       prr = fma(b, b, prr);
     }
-    rs[threadIdx.x] = b;
-    __syncthreads();
-    if (r < S.hi) {
-      const int lr = (threadIdx.x / NT) * NT;
-      double z = 0.0;
-#pragma unroll
-      for (int c = 0; c < NT; ++c) z = fma((double)T.Dinv[(int64_t)c * T.N + r], rs[lr + c], z);
-      mv.e0[r] = (W)(om * z);
-    }
-    __syncthreads();
+    jacobi_apply<NT>(T.Dinv, T.nf, rs, b, r, r < S.hi, [&](double z) { mv.e0[r] = (W)(om * z); });
   }
   const double bb = block_sum_mg<NT>(prr, sh);
   if (threadIdx.x == 0) v.part[(size_t)v.nblocks * 3 + blockIdx.x] = bb;
@@ -637,18 +619,11 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
         pe = a == 0 ? 0.5 * (el + eh) : kS3 * (eh - el);
       }
     }
-    rs[threadIdx.x] = rr;
-    __syncthreads();
-    if (r < S.hi) {
-      const int lr = (threadIdx.x / NT) * NT;
-      double z = 0.0;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) z = fma((double)T.Dinv[(int64_t)b * T.N + r], rs[lr + b], z);
+    jacobi_apply<NT>(T.Dinv, T.nf, rs, rr, r, r < S.hi, [&](double z) {
       const double zv = fma(om, z, pe);
       v.z[r] = zv;
       prz = fma(rv, zv, prz);
-    }
-    __syncthreads();
+    });
   }
   const double b = block_sum_mg<NT>(prz, sh);
   if (threadIdx.x == 0) v.part[(size_t)v.nblocks * (1 + ((i + 1) & 1)) + blockIdx.x] = b;
