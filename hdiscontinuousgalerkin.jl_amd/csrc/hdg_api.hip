@@ -552,9 +552,11 @@ int enqueue_chunk(hdg_ctx* c, double rtol2, int maxit) {
     LAUNCHCHK(c, launch_cg_spmv(c->pat, op64(c), kv64(c), 0, rtol2, maxit, 0, c->stream));
     if (mixed(c)) {
       LAUNCHCHK(c, launch_mg_update(op32(c), kv64(c), mv32(c), c->stream));
+      LAUNCHCHK(c, launch_cg_check(kv64(c), c->T.N, rtol2, maxit, c->stream));
       LAUNCHCHK(c, launch_mg_precond(c->pat, op32(c), kv64(c), c->mg, c->d_mg, mv32(c), c->stream));
     } else if (use_mg(c)) {
       LAUNCHCHK(c, launch_mg_update(op64(c), kv64(c), mv64(c), c->stream));
+      LAUNCHCHK(c, launch_cg_check(kv64(c), c->T.N, rtol2, maxit, c->stream));
       LAUNCHCHK(c, launch_mg_precond(c->pat, op64(c), kv64(c), c->mg, c->d_mg, mv64(c), c->stream));
     } else {
       LAUNCHCHK(c, launch_cg_update(c->T, c->cg, 0, 0, c->stream));
@@ -652,23 +654,39 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
   } else {
     LAUNCHCHK(c, launch_cg_init(c->pat, c->T, c->cg, c->stream));
   }
-  // one chunk stays in flight while the previous chunk's state is read
-  // back; launches after convergence return at once
+  // one chunk stays queued behind the one whose state is read back, unless
+  // that one is forecast to converge (chunk_converges); launches after
+  // convergence return at once
   hipEvent_t evs[2];
   HIPCHK(c, hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   HIPCHK(c, hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+  const double rtol2 = rtol * rtol;
   int slot = 0;
-  TRY(launch_chunk(c, slot, rtol * rtol, maxit));
+  TRY(launch_chunk(c, slot, rtol2, maxit));
   HIPCHK(c, hipEventRecord(evs[slot], c->stream));
+  CgState prev{}, last{};
+  int nread = 0;
+  bool behind = false;
   for (;;) {
     const int nxt = slot ^ 1;
-    TRY(launch_chunk(c, nxt, rtol * rtol, maxit));
-    HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
+    behind = nread < 2 || !chunk_converges(prev, last, chunk_of(c), rtol2);
+    if (behind) {
+      TRY(launch_chunk(c, nxt, rtol2, maxit));
+      HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
+    }
     HIPCHK(c, hipEventSynchronize(evs[slot]));
-    if (c->h_st[slot].done) break;
+    prev = last;
+    last = c->h_st[slot];
+    ++nread;
+    if (last.done) break;
+    if (!behind) {  // the forecast missed
+      TRY(launch_chunk(c, nxt, rtol2, maxit));
+      HIPCHK(c, hipEventRecord(evs[nxt], c->stream));
+      behind = true;
+    }
     slot = nxt;
   }
-  HIPCHK(c, hipEventSynchronize(evs[slot ^ 1]));
+  if (behind) HIPCHK(c, hipEventSynchronize(evs[slot ^ 1]));
   (void)hipEventDestroy(evs[0]);
   (void)hipEventDestroy(evs[1]);
   ++c->solves;

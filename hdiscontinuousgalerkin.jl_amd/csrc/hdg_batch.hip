@@ -643,16 +643,21 @@ int enqueue_setup(hdg_batch* b, Group& g) {
   return HDG_OK;
 }
 
+// PCG iterations per chunk
+int chunk_iters(const Group& g) { return g.mg ? 4 : 64; }
+
 int enqueue_chunk(hdg_batch* b, Group& g) {
-  const int iters = g.mg ? 4 : 64;
+  const int iters = chunk_iters(g);
   const Grouped gr = rows(g);
   for (int it = 0; it < iters; ++it) {
     BLAUNCH(b, launch_cg_spmv_t(g.pat, op64(g), kv(g), gr, g.nbt, 0, rtol2_of(g), maxit_of(g), 0, g.s));
     if (g.mixed) {
       BLAUNCH(b, launch_mg_update_group(op32(g), kv(g), mv32(g), gr, g.nbt, g.s));
+      BLAUNCH(b, launch_cg_check_group(kv(g), gr, (int)g.in.size(), rtol2_of(g), maxit_of(g), g.s));
       BLAUNCH(b, launch_mg_precond_group(g.pat, op32(g), kv(g), g.Mu, g.G, mv32(g), gr, g.nbt, g.s));
     } else if (g.mg) {
       BLAUNCH(b, launch_mg_update_group(op64(g), kv(g), mv64(g), gr, g.nbt, g.s));
+      BLAUNCH(b, launch_cg_check_group(kv(g), gr, (int)g.in.size(), rtol2_of(g), maxit_of(g), g.s));
       BLAUNCH(b, launch_mg_precond_group(g.pat, op64(g), kv(g), g.Mu, g.G, mv64(g), gr, g.nbt, g.s));
     } else {
       BLAUNCH(b, launch_cg_update_t(g.T, g.cg, gr, g.nbt, 0, 0, g.s));
@@ -821,16 +826,32 @@ namespace {
 int run_group(hdg_batch* b, Group& g, std::chrono::steady_clock::time_point t0) {
   if (hipSetDevice(b->device) != hipSuccess) return fail(b, HDG_ERR_HIP, "hipSetDevice failed");
   BTRY(enqueue_setup(b, g));
+  // as for one problem: nothing is queued behind a chunk in which every
+  // instance is forecast to converge (chunk_converges)
+  const size_t n = g.in.size();
+  std::vector<CgState> prev(n), last(n);
+  int nread = 0;
+  bool behind = false;
   int slot = 0;
   BTRY(launch_chunk(b, g, slot));
   for (;;) {
     const int nxt = slot ^ 1;
-    BTRY(launch_chunk(b, g, nxt));
+    behind = nread < 2;
+    for (size_t j = 0; j < n && !behind; ++j)
+      behind = !chunk_converges(prev[j], last[j], chunk_iters(g), rtol2_of(g));
+    if (behind) BTRY(launch_chunk(b, g, nxt));
     BHIP(b, hipEventSynchronize(g.ev[slot]));
+    prev.swap(last);
+    std::copy(g.h_st + slot * n, g.h_st + (slot + 1) * n, last.begin());
+    ++nread;
     if (all_done(g, slot)) break;
+    if (!behind) {  // the forecast missed
+      BTRY(launch_chunk(b, g, nxt));
+      behind = true;
+    }
     slot = nxt;
   }
-  BHIP(b, hipEventSynchronize(g.ev[slot ^ 1]));
+  if (behind) BHIP(b, hipEventSynchronize(g.ev[slot ^ 1]));
   BTRY(enqueue_finish(b, g));
   BHIP(b, hipStreamSynchronize(g.s));
   g.t_done = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <mutex>
 #include <utility>
@@ -147,6 +148,22 @@ struct CgState {
   double lz[kLanczos + 1];
   double lpq[kLanczos + 1];
 };
+
+// The host loops queue chunks of n CG iterations and keep one chunk queued
+// behind the one whose read-back they wait for.  From two read-backs of an
+// instance's state (prev, then last) the per-iteration log-rate of (r, r)
+// forecasts whether the n convergence tests of the chunk now running reach
+// (r, r) <= rtol2 (b, b); when they should, the loop queues nothing behind it
+// (a chunk there would only run launches that return at once).  A wrong
+// forecast costs one read-back round trip; the launches that do work and
+// their results are the same either way.
+inline bool chunk_converges(const CgState& prev, const CgState& last, int n, double rtol2) {
+  if (last.done) return true;
+  const int di = last.it - prev.it;
+  if (prev.done || di <= 0 || !(last.rr > 0.0) || !(prev.rr > last.rr) || !(last.bb > 0.0)) return false;
+  const double rate = std::log(last.rr / prev.rr) / di;
+  return rate * n <= std::log(rtol2 * last.bb / last.rr);
+}
 
 struct TraceSystem {
   int nt = 0;
@@ -540,6 +557,10 @@ int launch_apply(const DevPattern& P, const TraceSystem& T, double* part, double
 // grid = blocks of the launch; the Whole forms above call these).
 template <class G>
 int launch_cg_init_t(const DevPattern& P, const TraceSystem& T, CgVectors& v, G g, int grid, hipStream_t s);
+// the next product's convergence test right after the multigrid update (one
+// instance, or one block per instance of a group)
+int launch_cg_check(const Krylov<double>& v, int64_t N, double rtol2, int maxit, hipStream_t s);
+int launch_cg_check_group(const Krylov<double>& v, Grouped g, int ninst, double rtol2, int maxit, hipStream_t s);
 template <class G>
 int launch_cg_spmv_t(const DevPattern& P, const TraceOp<double>& T, const Krylov<double>& v, G g, int grid,
                      int i_fixed, double rtol2, int maxit, int force, hipStream_t s);

@@ -1289,6 +1289,44 @@ int launch_cg_spmv_t(const DevPattern& P, const TraceOp<double>& T, const Krylov
   return (int)hipGetLastError();
 }
 
+// The convergence test of the next CG product (the sums, decision and state
+// of cg_spmv_kernel's test), taken right after the multigrid update: a solve
+// whose residual has converged then skips that iteration's V-cycle, whose
+// kernels return at once on done.  One block of kCgThreads per instance, so
+// that the (r, r) partials are summed in the product's order.
+__device__ inline Seg check_seg(const Whole& w, int nb) { return Seg{0, w.n, 0, nb, w.st}; }
+__device__ inline Seg check_seg(const Grouped& g, int) { return g.seg[blockIdx.x]; }
+template <class G>
+__global__ __launch_bounds__(kCgThreads) void cg_check_kernel(Krylov<double> v, G gsel, int nb, double rtol2,
+                                                              int maxit) {
+  constexpr int NW = kCgThreads / 64;
+  __shared__ double sh[NW];
+  const Seg S = check_seg(gsel, nb);
+  CgState* st = S.st;
+  if (st->done) return;
+  const int i = st->i_pre + 1;  // the product that would take the test
+  const double rr = sum_partials<NW>(v.part + (size_t)v.nblocks * 3 + S.b0, S.nb, sh);
+  const double bb = st->bb;
+  const bool conv = rr <= rtol2 * bb;
+  if ((conv || i - 1 >= maxit) && threadIdx.x == 0) {
+    st->done = conv ? 1 : 2;
+    st->it = i - 1;
+    st->rr = rr;
+    st->bb = bb;
+  }
+}
+
+int launch_cg_check(const Krylov<double>& v, int64_t N, double rtol2, int maxit, hipStream_t s) {
+  hipLaunchKernelGGL((cg_check_kernel<Whole>), dim3(1), dim3(kCgThreads), 0, s, v, Whole{N, v.st}, v.nblocks, rtol2,
+                     maxit);
+  return (int)hipGetLastError();
+}
+
+int launch_cg_check_group(const Krylov<double>& v, Grouped g, int ninst, double rtol2, int maxit, hipStream_t s) {
+  hipLaunchKernelGGL((cg_check_kernel<Grouped>), dim3(ninst), dim3(kCgThreads), 0, s, v, g, 0, rtol2, maxit);
+  return (int)hipGetLastError();
+}
+
 template <class G>
 int launch_cg_update_t(const TraceSystem& T, CgVectors& v, G g, int grid, int i_fixed, int force,
                        hipStream_t s) {
