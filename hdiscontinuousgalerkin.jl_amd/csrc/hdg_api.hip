@@ -669,7 +669,7 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
   bool behind = false;
   for (;;) {
     const int nxt = slot ^ 1;
-    behind = nread < 2 || !chunk_converges(prev, last, chunk_of(c), rtol2);
+    behind = nread < 2 || !chunk_converges(prev, last, chunk_of(c) + (use_mg(c) ? 1 : 0), rtol2);
     if (behind) {
       TRY(launch_chunk(c, nxt, rtol2, maxit));
       HIPCHK(c, hipEventRecord(evs[nxt], c->stream));

@@ -838,7 +838,7 @@ int run_group(hdg_batch* b, Group& g, std::chrono::steady_clock::time_point t0) 
     const int nxt = slot ^ 1;
     behind = nread < 2;
     for (size_t j = 0; j < n && !behind; ++j)
-      behind = !chunk_converges(prev[j], last[j], chunk_iters(g), rtol2_of(g));
+      behind = !chunk_converges(prev[j], last[j], chunk_iters(g) + (g.mg ? 1 : 0), rtol2_of(g));
     if (behind) BTRY(launch_chunk(b, g, nxt));
     BHIP(b, hipEventSynchronize(g.ev[slot]));
     prev.swap(last);

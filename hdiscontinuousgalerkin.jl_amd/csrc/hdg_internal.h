@@ -149,14 +149,16 @@ struct CgState {
   double lpq[kLanczos + 1];
 };
 
-// The host loops queue chunks of n CG iterations and keep one chunk queued
+// The host loops queue chunks of CG iterations and keep one chunk queued
 // behind the one whose read-back they wait for.  From two read-backs of an
 // instance's state (prev, then last) the per-iteration log-rate of (r, r)
-// forecasts whether the n convergence tests of the chunk now running reach
-// (r, r) <= rtol2 (b, b); when they should, the loop queues nothing behind it
-// (a chunk there would only run launches that return at once).  A wrong
-// forecast costs one read-back round trip; the launches that do work and
-// their results are the same either way.
+// forecasts whether the chunk now running reaches (r, r) <= rtol2 (b, b) at
+// its last test, n iterations past last.it: the chunk's length, plus one
+// where a convergence test follows each multigrid update (the read-back
+// holds the product's test of the iteration before).  When it should, the
+// loop queues nothing behind it (a chunk there would only run launches that
+// return at once).  A wrong forecast costs one read-back round trip; the
+// launches that do work and their results are the same either way.
 inline bool chunk_converges(const CgState& prev, const CgState& last, int n, double rtol2) {
   if (last.done) return true;
   const int di = last.it - prev.it;
