@@ -56,7 +56,8 @@ def spmv_bytes(ctx):
     """Algorithmic bytes of one trace-SpMV launch (FP64): the matrix values
     once, z (gathered), p_old and q_old read once (24 N), p_new and q
     written (16 N), the Dirichlet mask (N bytes).  Values: with
-    symmetric face blocks (k = 3) 3 nt^2 per face and 4 int32 block
+    symmetric face blocks (k = 3) the upper triangle of K_ff and the two
+    owned blocks, nt (nt + 1) / 2 + 2 nt^2 per face, and 4 int32 block
     references per face (the transposed reads of a neighbour's block are
     the owner's bytes again); with the face-ELL 8 nnz and 4 bytes of block
     column id per stored block."""
@@ -65,7 +66,7 @@ def spmv_bytes(ctx):
     N = s.n_dofs
     vec = 24 * N + 16 * N + N
     if s.sym_blocks:
-        return 8 * 3 * nt * nt * s.n_faces + 16 * s.n_faces + vec
+        return 8 * (nt * (nt + 1) // 2 + 2 * nt * nt) * s.n_faces + 16 * s.n_faces + vec
     nblocks = s.nnz // (nt * nt)
     return 8 * s.nnz + 4 * nblocks + vec
 

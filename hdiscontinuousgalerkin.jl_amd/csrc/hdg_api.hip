@@ -616,7 +616,7 @@ int solve_impl(hdg_ctx* c, hdg_solve_info* info, int* err_flag = nullptr) {
     if (mx) {
       const int64_t N = c->T.N;
       if (c->T.blk)
-        LAUNCHCHK(c, launch_to_float(c->T.blk, c->f_blk, (int64_t)3 * c->nt * N, c->stream));
+        LAUNCHCHK(c, launch_to_float(c->T.blk, c->f_blk, (int64_t)sym_blk_count(c->nt, c->mesh.nf), c->stream));
       else
         LAUNCHCHK(c, launch_to_float(c->T.vals, c->f_vals, (int64_t)kSlots * c->nt * N, c->stream));
       LAUNCHCHK(c, launch_to_float(c->T.Dinv, c->f_Dinv, (int64_t)dinv_count(c->nt, c->mesh.nf), c->stream));
@@ -1033,7 +1033,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   // the trace operator: symmetric blocks at four modes per face, else the
   // face-ELL (the exports expand the blocks into a face-ELL on demand)
   if (sym_blocks(c->nt))
-    TRY(dalloc(c, &c->T.blk, (size_t)3 * c->nt * N));
+    TRY(dalloc(c, &c->T.blk, sym_blk_count(c->nt, c->mesh.nf)));
   else
     TRY(dalloc(c, &c->T.vals, (size_t)kSlots * c->nt * N));
   TRY(dalloc(c, &c->T.rhs, N));
@@ -1065,7 +1065,7 @@ int hdg_setup(hdg_ctx* c, const hdg_options* opt) {
   }
   if (c->structured && o.precision == HDG_MIXED) {
     if (c->T.blk)
-      TRY(dalloc(c, &c->f_blk, (size_t)3 * c->nt * N));
+      TRY(dalloc(c, &c->f_blk, sym_blk_count(c->nt, c->mesh.nf)));
     else
       TRY(dalloc(c, &c->f_vals, (size_t)kSlots * c->nt * N));
     TRY(dalloc(c, &c->f_Dinv, dinv_count(c->nt, c->mesh.nf)));
@@ -1563,7 +1563,7 @@ int hdg_refine_probe(hdg_ctx* c, int outer, double inner_rtol, double* relres, i
   if (inner_iters)
     for (int j = 0; j < outer; ++j) inner_iters[j] = 0;
   const int64_t N = c->T.N, nv = c->T.vals ? (int64_t)kSlots * c->nt * N : 0;
-  const int64_t nb = c->T.blk ? (int64_t)3 * c->nt * N : 0;
+  const int64_t nb = c->T.blk ? (int64_t)sym_blk_count(c->nt, c->mesh.nf) : 0;
   double *vr = nullptr, *vb = nullptr, *u = nullptr, *res = nullptr, *zero = nullptr, *part = nullptr;
   int rc = HDG_OK;
   if ((!nv || (rc = dalloc(c, &vr, nv)) == HDG_OK) && (rc = dalloc(c, &u, N)) == HDG_OK && (rc = dalloc(c, &res, N)) == HDG_OK &&

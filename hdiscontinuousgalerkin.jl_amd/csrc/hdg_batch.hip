@@ -380,7 +380,7 @@ int build_group(hdg_batch* b, Group& g) {
   g.T.nf = g.nf;
   g.T.N = N;
   if (sym_blocks(g.nt))
-    BTRY(galloc(b, g, &g.T.blk, (size_t)3 * g.nt * N));
+    BTRY(galloc(b, g, &g.T.blk, sym_blk_count(g.nt, g.nf)));
   else
     BTRY(galloc(b, g, &g.T.vals, (size_t)kSlots * g.nt * N));
   BTRY(galloc(b, g, &g.T.rhs, N));
@@ -484,7 +484,7 @@ int build_group(hdg_batch* b, Group& g) {
   BTRY(galloc(b, g, &U.sp, 4 * N));
   if (g.mixed) {
     if (g.T.blk)
-      BTRY(galloc(b, g, &g.f_blk, (size_t)3 * g.nt * N));
+      BTRY(galloc(b, g, &g.f_blk, sym_blk_count(g.nt, g.nf)));
     else
       BTRY(galloc(b, g, &g.f_vals, (size_t)kSlots * g.nt * N));
     BTRY(galloc(b, g, &g.f_Dinv, dinv_count(g.nt, g.nf)));
@@ -623,7 +623,7 @@ int enqueue_setup(hdg_batch* b, Group& g) {
   const double factor = g.Mi[0].omega_factor;
   if (g.mixed) {
     if (g.T.blk)
-      BLAUNCH(b, launch_to_float(g.T.blk, g.f_blk, (int64_t)3 * g.nt * g.N, g.s));
+      BLAUNCH(b, launch_to_float(g.T.blk, g.f_blk, (int64_t)sym_blk_count(g.nt, g.nf), g.s));
     else
       BLAUNCH(b, launch_to_float(g.T.vals, g.f_vals, (int64_t)kSlots * g.nt * g.N, g.s));
     BLAUNCH(b, launch_to_float(g.T.Dinv, g.f_Dinv, (int64_t)dinv_count(g.nt, g.nf), g.s));

@@ -171,9 +171,12 @@ struct TraceSystem {
   int nt = 0;
   int64_t nf = 0, N = 0;
   double* vals = nullptr;  // ELL, column-major: vals[(s*nt + b)*N + row]
-  // symmetric blocks (sym_blocks(nt), else null): blk[((3f + j) nt + a) nt + b],
-  // j = 0 the diagonal block K_ff, j = 1 + k the block face f owns in its
-  // cell k (row a, column b; DevPattern::nbr)
+  // symmetric blocks (sym_blocks(nt), else null), sym_blk_count(nt, nf)
+  // doubles: first K_ff's upper triangle slot-major, (a, b) of face f at
+  // blk[dinv_slot(nt, a, b) nf + f]; then the two blocks face f owns, one in
+  // each of its cells k, own[((2f + k) nt + a) nt + b] with own = blk +
+  // nt (nt + 1) / 2 nf (row a of f, column b of the neighbour coupled in
+  // that cell; DevPattern::nbr)
   double* blk = nullptr;
   double* rhs = nullptr;   // assembled rhs (N)
   double* Dinv = nullptr;  // block-Jacobi inverse (dinv_count; 0 on Dirichlet dofs)
@@ -195,6 +198,7 @@ inline bool quad_rows(int nt) { return nt == 1 || nt == 2 || nt == 4; }
 __host__ __device__ constexpr int dinv_slot(int nt, int a, int b) {
   return a <= b ? a * nt - a * (a - 1) / 2 + (b - a) : b * nt - b * (b - 1) / 2 + (a - b);
 }
+inline size_t sym_blk_count(int nt, int64_t nf) { return (size_t)(nt * (nt + 1) / 2 + 2 * nt * nt) * (size_t)nf; }
 inline size_t dinv_count(int nt, int64_t nf) {
   return (size_t)(quad_rows(nt) ? nt * (nt + 1) / 2 : nt * nt) * (size_t)nf;
 }
@@ -302,25 +306,23 @@ struct TraceOp {
       : nt(nt_), nf(nf_), N(N_), vals(vals_), Dinv(Dinv_), dmask(dmask_), blk(blk_) {}
 };
 // The entries of row a of face f of K over the symmetric blocks
-// (kSymBlocks<NT>): fn(g, b, K[(f, a), (g, b)]) block by block, K_ff, the
-// face's own blocks, then the two blocks of the neighbours that own the
-// coupling.  The diagonal block and the own blocks are read by rows (the
-// diagonal block once per product: nontemporal); the neighbours' blocks
-// by columns, their lines shared with the owners' own reads through the
-// caches.
+// (kSymBlocks<NT>, TraceSystem::blk): fn(g, b, K[(f, a), (g, b)]) block by
+// block, K_ff, the face's own blocks, then the two blocks of the neighbours
+// that own the coupling.  K_ff comes from its triangle (a wavefront's 16
+// faces read 10 consecutive runs of 128 bytes, once per product:
+// nontemporal), the own blocks by rows, the neighbours' blocks by columns,
+// their lines shared with the owners' own reads through the caches.
 template <int NT, class V, class F>
-__device__ __forceinline__ void sym_row_entries(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t f,
-                                                int a, const F& fn) {
+__device__ __forceinline__ void sym_row_entries(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t nf,
+                                                int64_t f, int a, const F& fn) {
   constexpr int B = NT * NT;
   const int4 nb = *reinterpret_cast<const int4*>(nbr + 4 * f);
-  {
-    const V* d = blk + 3 * f * B + a * NT;
 #pragma unroll
-    for (int b = 0; b < NT; ++b) fn(f, b, (double)__builtin_nontemporal_load(d + b));
-  }
+  for (int b = 0; b < NT; ++b) fn(f, b, (double)__builtin_nontemporal_load(blk + (int64_t)dinv_slot(NT, a, b) * nf + f));
+  const V* own = blk + (int64_t)(NT * (NT + 1) / 2) * nf;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    const V* d = blk + (3 * f + 1 + k) * B + a * NT;
+    const V* d = own + (2 * f + k) * B + a * NT;
     const int64_t g = k ? nb.y : nb.x;
 #pragma unroll
     for (int b = 0; b < NT; ++b) fn(g, b, (double)d[b]);
@@ -329,7 +331,7 @@ __device__ __forceinline__ void sym_row_entries(const int* __restrict__ nbr, con
   for (int k = 0; k < 2; ++k) {
     const int code = k ? nb.w : nb.z;
     const int64_t g = code >> 1;
-    const V* d = blk + (3 * g + 1 + (code & 1)) * B + a;
+    const V* d = own + (2 * g + (code & 1)) * B + a;
 #pragma unroll
     for (int b = 0; b < NT; ++b) fn(g, b, (double)d[b * NT]);
   }
@@ -340,15 +342,16 @@ __device__ __forceinline__ void sym_row_entries(const int* __restrict__ nbr, con
 // sym_row_entries' order; a block's values are read only for prescribed
 // columns, so rows away from the boundary read none.
 template <int NT>
-__device__ __forceinline__ double sym_row_lift(const int* __restrict__ nbr, const double* __restrict__ blk,
+__device__ __forceinline__ double sym_row_lift(const int* __restrict__ nbr, const double* __restrict__ blk, int64_t nf,
                                                const unsigned char* __restrict__ dmask,
                                                const double* __restrict__ uD, int64_t f, int a) {
   constexpr int B = NT * NT;
   const int4 nb = *reinterpret_cast<const int4*>(nbr + 4 * f);
   const int64_t gs[5] = {f, nb.x, nb.y, nb.z >> 1, nb.w >> 1};
-  const int64_t base[5] = {3 * f * B + a * NT, (3 * f + 1) * B + a * NT, (3 * f + 2) * B + a * NT,
-                           (3 * (int64_t)(nb.z >> 1) + 1 + (nb.z & 1)) * B + a,
-                           (3 * (int64_t)(nb.w >> 1) + 1 + (nb.w & 1)) * B + a};
+  const int64_t o = (int64_t)(NT * (NT + 1) / 2) * nf;  // the own blocks
+  const int64_t base[5] = {0, o + 2 * f * B + a * NT, o + (2 * f + 1) * B + a * NT,
+                           o + (2 * (int64_t)(nb.z >> 1) + (nb.z & 1)) * B + a,
+                           o + (2 * (int64_t)(nb.w >> 1) + (nb.w & 1)) * B + a};
   double s = 0.0;
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
@@ -356,7 +359,8 @@ __device__ __forceinline__ double sym_row_lift(const int* __restrict__ nbr, cons
 #pragma unroll
     for (int c = 0; c < NT; ++c) {
       const int64_t col = gs[j] * NT + c;
-      if (dmask[col]) s = fma(blk[base[j] + c * stride], uD[col], s);
+      const int64_t at = j == 0 ? (int64_t)dinv_slot(NT, a, c) * nf + f : base[j] + c * stride;
+      if (dmask[col]) s = fma(blk[at], uD[col], s);
     }
   }
   return s;
@@ -365,10 +369,10 @@ __device__ __forceinline__ double sym_row_lift(const int* __restrict__ nbr, cons
 // Row a of face f of K x over the symmetric blocks, x given as xv(g, b) =
 // entry b of face g.
 template <int NT, class V, class X>
-__device__ __forceinline__ double sym_row(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t f, int a,
-                                          const X& xv) {
+__device__ __forceinline__ double sym_row(const int* __restrict__ nbr, const V* __restrict__ blk, int64_t nf, int64_t f,
+                                          int a, const X& xv) {
   double acc = 0.0;
-  sym_row_entries<NT>(nbr, blk, f, a, [&](int64_t g, int b, double v) { acc = fma(v, xv(g, b), acc); });
+  sym_row_entries<NT>(nbr, blk, nf, f, a, [&](int64_t g, int b, double v) { acc = fma(v, xv(g, b), acc); });
   return acc;
 }
 

@@ -532,13 +532,24 @@ __global__ __launch_bounds__(256) void fill_kernel(DevPattern P, const double* _
   T.rhs[r] = rh;
 }
 
+// Entry a of the row v held by lane B of the caller's quad.
+template <int B>
+__device__ __forceinline__ double quad_entry(const double (&v)[4], int a) {
+  const double x0 = dpp_quad<quad_ctrl<4, B>()>(v[0]), x1 = dpp_quad<quad_ctrl<4, B>()>(v[1]);
+  const double x2 = dpp_quad<quad_ctrl<4, B>()>(v[2]), x3 = dpp_quad<quad_ctrl<4, B>()>(v[3]);
+  return a == 0 ? x0 : (a == 1 ? x1 : (a == 2 ? x2 : x3));
+}
+
 // The symmetric blocks (kSymBlocks<NT>) and the rhs, cell by cell: thread
 // per (cell c, local face l, j, row a), so that a cell's threads read its
 // element block Ate together.  j = 1: the block face f = cf[3c + l] owns in
-// c (coupling to local face l + 1), into block 1 + k of f, k the side of c
-// in fcell.  j = 0, by the cell on side 0 only: K_ff as the two cells'
-// diagonal blocks summed in cell order (as the face-ELL sums them), the rhs
-// entry, and a zero block 2 where f has one cell.
+// c (coupling to local face l + 1), into own block k of f, k the side of c
+// in fcell.  j = 0, by the cell on side 0 only: the upper triangle of K_ff,
+// the two cells' diagonal blocks summed in cell order (as the face-ELL sums
+// them), the rhs entry, and a zero own block 1 where f has one cell.  The
+// triangle holds (K_ab + K_ba) / 2: the rows a = 0..3 of one (c, l) are the
+// lanes of one quad (24 items per cell), so K_ba comes from lane b by DPP,
+// and the quad returns or goes on as one.
 template <int NT>
 __global__ __launch_bounds__(256) void fill_sym_kernel(DevMesh m, const double* __restrict__ Ate,
                                                        const double* __restrict__ bte, TraceSystem T) {
@@ -555,7 +566,7 @@ __global__ __launch_bounds__(256) void fill_sym_kernel(DevMesh m, const double* 
   if (j == 1) {
 #pragma unroll
     for (int b = 0; b < NT; ++b) v[b] = row[((l + 1) % 3) * NT + b];
-    double* out = T.blk + (3 * f + 1 + k) * B + a * NT;
+    double* out = T.blk + (int64_t)(NT * (NT + 1) / 2) * T.nf + (2 * f + k) * B + a * NT;
 #pragma unroll
     for (int b = 0; b < NT; ++b) out[b] = v[b];
     return;
@@ -565,7 +576,6 @@ __global__ __launch_bounds__(256) void fill_sym_kernel(DevMesh m, const double* 
   for (int b = 0; b < NT; ++b) v[b] = row[l * NT + b];
   double rh = bte[c * NTR + l * NT + a];
   const int64_t c2 = m.fcell[2 * f + 1];
-  double* out = T.blk + 3 * f * B + a * NT;
   if (c2 >= 0) {
     int l2 = 0;
     for (int q = 1; q < 3; ++q)
@@ -575,11 +585,16 @@ __global__ __launch_bounds__(256) void fill_sym_kernel(DevMesh m, const double* 
     for (int b = 0; b < NT; ++b) v[b] = v[b] + r2[b];
     rh = rh + bte[c2 * NTR + l2 * NT + a];
   } else {
+    double* own1 = T.blk + (int64_t)(NT * (NT + 1) / 2) * T.nf + (2 * f + 1) * B + a * NT;
 #pragma unroll
-    for (int b = 0; b < NT; ++b) out[2 * B + b] = 0.0;
+    for (int b = 0; b < NT; ++b) own1[b] = 0.0;
   }
+  // K_ff's upper triangle, the mean of K_ab and K_ba: row a from column a on
+  static_assert(NT == 4, "the quad exchange holds the four rows of a face");
+  const double w[NT] = {quad_entry<0>(v, a), quad_entry<1>(v, a), quad_entry<2>(v, a), quad_entry<3>(v, a)};
 #pragma unroll
-  for (int b = 0; b < NT; ++b) out[b] = v[b];
+  for (int b = 0; b < NT; ++b)
+    if (b >= a) T.blk[(int64_t)dinv_slot(NT, a, b) * T.nf + f] = 0.5 * (v[b] + w[b]);
   T.rhs[f * NT + a] = rh;
 }
 
@@ -595,7 +610,7 @@ __global__ __launch_bounds__(256) void expand_ell_kernel(DevPattern P, TraceSyst
   double row[5][NT];
   int64_t col[5];
   int e = 0;
-  sym_row_entries<NT>(P.nbr, T.blk, f, a, [&](int64_t g, int b, double v) {
+  sym_row_entries<NT>(P.nbr, T.blk, T.nf, f, a, [&](int64_t g, int b, double v) {
     row[e / NT][b] = v;
     col[e / NT] = g;
     ++e;
@@ -630,7 +645,7 @@ __global__ void jacobi_kernel(DevPattern P, TraceSystem T) {
     double cv[NT];
     if constexpr (kSymBlocks<NT>) {
 #pragma unroll
-      for (int i = 0; i < NT; ++i) cv[i] = T.blk[3 * f * NT * NT + i * NT + j];
+      for (int i = 0; i < NT; ++i) cv[i] = T.blk[(int64_t)dinv_slot(NT, i, j) * T.nf + f];
     } else if constexpr (NT % 2 == 0) {
       const double* col = T.vals + (int64_t)(s0 * NT + j) * T.N + f * NT;
 #pragma unroll
@@ -788,7 +803,7 @@ __global__ __launch_bounds__(kCgThreads) void cg_spmv_kernel(DevPattern P, Trace
     const int64_t f = r / NT;
     double acc = 0.0;
     if constexpr (kSymBlocks<NT>) {
-      acc = sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int b) { return z[g * NT + b]; });
+      acc = sym_row<NT>(P.nbr, T.blk, T.nf, f, (int)(r - f * NT), [&](int64_t g, int b) { return z[g * NT + b]; });
     } else {
       const int* bc = P.bcol + f * kSlots;
 #pragma unroll
@@ -893,7 +908,7 @@ __global__ __launch_bounds__(64 * NT) void cg_init_kernel(DevPattern P, TraceSys
         const int64_t f = r / NT;
         double s = T.rhs[r];
         if constexpr (kSymBlocks<NT>) {
-          s -= sym_row_lift<NT>(P.nbr, T.blk, T.dmask, T.uD, f, (int)(r - f * NT));
+          s -= sym_row_lift<NT>(P.nbr, T.blk, T.nf, T.dmask, T.uD, f, (int)(r - f * NT));
         } else {
 #pragma unroll
           for (int sl = 0; sl < kSlots; ++sl) {
@@ -980,7 +995,7 @@ __global__ __launch_bounds__(kCgThreads) void true_residual_kernel(DevPattern P,
     const int64_t f = r / NT;
     double s = -T.rhs[r];
     if constexpr (kSymBlocks<NT>) {
-      s += sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int c) { return uhat[g * NT + c]; });
+      s += sym_row<NT>(P.nbr, T.blk, T.nf, f, (int)(r - f * NT), [&](int64_t g, int c) { return uhat[g * NT + c]; });
     } else {
 #pragma unroll
       for (int sl = 0; sl < kSlots; ++sl) {

@@ -122,7 +122,7 @@ __device__ inline double s_row(const DevPattern& P, const TraceOp<V>& T, const V
   const int64_t f = r / NT;
   double acc = 0.0;
   if constexpr (kSymBlocks<NT>) {
-    acc = sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int b) { return (double)x[g * NT + b]; });
+    acc = sym_row<NT>(P.nbr, T.blk, T.nf, f, (int)(r - f * NT), [&](int64_t g, int b) { return (double)x[g * NT + b]; });
   } else {
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void mg_sp_kernel(DevPattern P, TraceSystem T,
   };
   if (!T.dmask[r]) {
     if constexpr (kSymBlocks<NT>) {
-      sym_row_entries<NT>(P.nbr, T.blk, f, (int)(r - f * NT), add);
+      sym_row_entries<NT>(P.nbr, T.blk, T.nf, f, (int)(r - f * NT), add);
     } else {
       for (int sl = 0; sl < kSlots; ++sl) {
         const int64_t g = P.bcol[f * kSlots + sl];
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(64 * NT) void mg_init_kernel(DevPattern P, TraceSys
         const int64_t f = r / NT;
         double s = T64.rhs[r];
         if constexpr (kSymBlocks<NT>) {
-          s -= sym_row_lift<NT>(P.nbr, T64.blk, T64.dmask, T64.uD, f, (int)(r - f * NT));
+          s -= sym_row_lift<NT>(P.nbr, T64.blk, T64.nf, T64.dmask, T64.uD, f, (int)(r - f * NT));
         } else {
 #pragma unroll
           for (int sl = 0; sl < kSlots; ++sl) {

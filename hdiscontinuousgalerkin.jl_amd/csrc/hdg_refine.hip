@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void refine_resid_kernel(DevPattern P, TraceSy
       const int64_t f = r / NT;
       s = T.rhs[r];
       if constexpr (kSymBlocks<NT>) {
-        s -= sym_row<NT>(P.nbr, T.blk, f, (int)(r - f * NT), [&](int64_t g, int c) { return u[g * NT + c]; });
+        s -= sym_row<NT>(P.nbr, T.blk, T.nf, f, (int)(r - f * NT), [&](int64_t g, int c) { return u[g * NT + c]; });
       } else {
 #pragma unroll
         for (int sl = 0; sl < kSlots; ++sl) {
