@@ -610,14 +610,14 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
     if (r < S.hi && !T.dmask[r]) {
       const int64_t f = r / NT;
       double s = rv + (double)mv.res0[r];
+      // the face's 4 vertices, its lower and higher end point first
+      const int4 q = *reinterpret_cast<const int4*>(M.fq4 + 4 * f);
+      const double ev[4] = {e1[q.x], e1[q.y], e1[q.z], e1[q.w]};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) s = fma(-(double)mv.sp[(int64_t)j * T.N + r], e1[M.fq4[4 * f + j]], s);
+      for (int j = 0; j < 4; ++j) s = fma(-(double)mv.sp[(int64_t)j * T.N + r], ev[j], s);
       rr = s;
       const int a = (int)(r - f * NT);
-      if (a < 2) {
-        const double el = e1[M.fv[2 * f]], eh = e1[M.fv[2 * f + 1]];
-        pe = a == 0 ? 0.5 * (el + eh) : kS3 * (eh - el);
-      }
+      if (a < 2) pe = a == 0 ? 0.5 * (ev[0] + ev[1]) : kS3 * (ev[1] - ev[0]);
     }
     jacobi_apply<NT>(T.Dinv, T.nf, rs, rr, r, r < S.hi, [&](double z) {
       const double zv = fma(om, z, pe);
