@@ -570,8 +570,20 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgD
   double s = 0.0;
   if (!L.mask[vv]) {
     if constexpr (NT % 2 == 0) {
-      // the weights the residual kernel wrote per face end
-      for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) s += M.fc[M.vf2[k]];
+      // the weights the residual kernel wrote per face end, summed in the
+      // vertex's face order; the first 8 indices are loaded before any
+      // weight, so that the gathers are in flight together
+      const int k0 = M.vf_ptr[vv], k1 = M.vf_ptr[vv + 1];
+      int id[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) id[u] = k0 + u < k1 ? M.vf2[k0 + u] : -1;
+      double w[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) w[u] = id[u] >= 0 ? M.fc[id[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (id[u] >= 0) s += w[u];
+      for (int k = k0 + 8; k < k1; ++k) s += M.fc[M.vf2[k]];
     } else {
       for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
         const int64_t f = M.vf_idx[k];
