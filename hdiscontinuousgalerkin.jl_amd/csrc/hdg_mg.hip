@@ -585,7 +585,26 @@ __global__ __launch_bounds__(256) void mg_restrict0_kernel(Krylov<double> v, MgD
         if (id[u] >= 0) s += w[u];
       for (int k = k0 + 8; k < k1; ++k) s += M.fc[M.vf2[k]];
     } else {
-      for (int k = M.vf_ptr[vv]; k < M.vf_ptr[vv + 1]; ++k) {
+      // modes 0 and 1 of the incident faces, in the vertex's face order,
+      // the first 8 faces' loads issued together
+      const int k0 = M.vf_ptr[vv], k1 = M.vf_ptr[vv + 1];
+      int64_t fa[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) fa[u] = k0 + u < k1 ? (int64_t)M.vf_idx[k0 + u] : -1;
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        t[u] = 0.0;
+        if (fa[u] >= 0) {
+          const int64_t f = fa[u];
+          const bool lo = M.fv[2 * f] == vv;
+          t[u] = 0.5 * (double)mv.res0[f * NT] + (lo ? -kS3 : kS3) * (double)mv.res0[f * NT + 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (fa[u] >= 0) s += t[u];
+      for (int k = k0 + 8; k < k1; ++k) {
         const int64_t f = M.vf_idx[k];
         const bool lo = M.fv[2 * f] == vv;
         s += 0.5 * (double)mv.res0[f * NT] + (lo ? -kS3 : kS3) * (double)mv.res0[f * NT + 1];
