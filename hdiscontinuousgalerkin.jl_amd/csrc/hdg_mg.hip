@@ -535,7 +535,10 @@ __global__ __launch_bounds__(256) void mg_resid0_kernel(DevPattern P, TraceOp<W>
                                                         MgVecs<W> mv, double* __restrict__ fc) {
   if (v.st && v.st->done) return;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < T.N; r += (int64_t)gridDim.x * blockDim.x) {
-    const W res = T.dmask[r] ? (W)0 : (W)(v.r[r] - s_row<NT, W>(P, T, mv.e0, r));
+    // every row's product, the Dirichlet rows masked afterwards: the loads
+    // do not wait for the mask
+    const double sr = s_row<NT, W>(P, T, mv.e0, r);
+    const W res = T.dmask[r] ? (W)0 : (W)(v.r[r] - sr);
     mv.res0[r] = res;
     if constexpr (NT % 2 == 0) {
       // the grid is a multiple of 64 threads and N of NT: lane r + 1 of a
@@ -637,8 +640,10 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
   for (int64_t r0 = S.lo + (int64_t)lb * NTH; r0 < S.hi; r0 += (int64_t)S.nb * NTH) {
     const int64_t r = r0 + threadIdx.x;
     double rv = 0.0, rr = 0.0, pe = 0.0;
-    if (r < S.hi) rv = v.r[r];
-    if (r < S.hi && !T.dmask[r]) {
+    if (r < S.hi) {
+      // every row formed, the Dirichlet rows masked afterwards (the loads do
+      // not wait for the mask)
+      rv = v.r[r];
       const int64_t f = r / NT;
       double s = rv + (double)mv.res0[r];
       // the face's 4 vertices, its lower and higher end point first
@@ -646,9 +651,12 @@ __global__ __launch_bounds__(64 * NT) void mg_smooth0_kernel(TraceOp<W> T, Krylo
       const double ev[4] = {e1[q.x], e1[q.y], e1[q.z], e1[q.w]};
 #pragma unroll
       for (int j = 0; j < 4; ++j) s = fma(-(double)mv.sp[(int64_t)j * T.N + r], ev[j], s);
-      rr = s;
       const int a = (int)(r - f * NT);
-      if (a < 2) pe = a == 0 ? 0.5 * (ev[0] + ev[1]) : kS3 * (ev[1] - ev[0]);
+      const double pc = a == 0 ? 0.5 * (ev[0] + ev[1]) : (a == 1 ? kS3 * (ev[1] - ev[0]) : 0.0);
+      if (!T.dmask[r]) {
+        rr = s;
+        pe = pc;
+      }
     }
     jacobi_apply<NT>(T.Dinv, T.nf, rs, rr, r, r < S.hi, [&](double z) {
       const double zv = fma(om, z, pe);
